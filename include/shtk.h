@@ -1,0 +1,141 @@
+/*
+ * shtk.h -- C ABI of libshtk_mi355x.so, the MI355X (gfx950) spherical harmonic
+ * transform backend for SHTnsKit.jl.
+ *
+ * Every entry point takes plain pointers and sizes.  Device pointers are HIP
+ * device allocations (ROCArray / torch tensor storage); `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream).  Work is enqueued on
+ * that stream and the call returns without synchronising, except where noted;
+ * results are ready once the stream reaches them.  The Julia adapter calls
+ * AMDGPU.synchronize() after each call, as ext/SHTnsKitAMDGPUExt.jl:696,746,786
+ * does, so SHTnsKit's "results ready on return" contract holds.
+ *
+ * Memory layouts are SHTnsKit's (Julia column-major):
+ *   grid      f[i + nlat*j + nlat*nlon*k]            i latitude (north first), j longitude, k field
+ *   spectrum  a[l + (lmax+1)*m + (lmax+1)*(mmax+1)*k] complex as interleaved (re, im)
+ * Entries with l < m (and orders m that are not multiples of mres) are written
+ * as zero on output and ignored on input.
+ *
+ * dtype selects the element type of the caller's arrays (SHTK_F64: double /
+ * complex double, SHTK_F32: float / complex float).  All Legendre recurrences,
+ * reductions and FFTs run in FP64 whatever the I/O type.
+ *
+ * Return value: SHTK_OK (0) or an error code; shtk_last_error() gives the
+ * message of the last failure on the calling thread.
+ */
+#ifndef SHTK_H
+#define SHTK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum shtk_status {
+    SHTK_OK = 0,
+    SHTK_EINVAL = 1,      /* bad size, pointer or keyword (DimensionMismatch / ArgumentError) */
+    SHTK_EUNSUPPORTED = 2, /* valid request this backend does not implement (BackendUnavailableError) */
+    SHTK_ENOMEM = 3,      /* device allocation failed */
+    SHTK_EHIP = 4         /* HIP runtime error */
+};
+
+enum shtk_norm { SHTK_NORM_ORTHONORMAL = 0, SHTK_NORM_FOURPI = 1, SHTK_NORM_SCHMIDT = 2 };
+enum shtk_phi_scale { SHTK_PHI_DFT = 0, SHTK_PHI_QUAD = 1 };
+enum shtk_dtype { SHTK_F64 = 0, SHTK_F32 = 1 };
+
+/* Grid and spectral descriptor.  Mirrors the keyword set of
+ * create_gauss_config(lmax, nlat; mmax, mres, nlon, norm, cs_phase, real_norm,
+ * robert_form) (src/config.jl:633-655) plus the phi scaling mode
+ * (src/SHTnsKit.jl:141-177).  Constraints as src/config.jl:465-481:
+ * nlat >= lmax+1, nlon >= 2*mmax+1, 0 <= mmax <= lmax, mres >= 1. */
+typedef struct shtk_desc {
+    int lmax, mmax, mres, nlat, nlon;
+    int norm;        /* enum shtk_norm */
+    int cs_phase;    /* 1: Condon-Shortley phase */
+    int real_norm;   /* 1: SHTns REAL_NORM convention */
+    int robert_form; /* 1: vector fields in Robert form (multiplied by sin(theta)) */
+    int phi_scale;   /* enum shtk_phi_scale */
+} shtk_desc;
+
+typedef struct shtk_plan shtk_plan;
+
+/* Builds a Gauss-Legendre plan: nodes and weights (src/legendre.jl:956-1003,
+ * north first as src/config.jl:639), FP64 recurrence coefficients, sectoral
+ * seeds with scaled exponents, FFT twiddles.  Replaces create_gauss_config
+ * (src/config.jl:633-655) and the device table builders of
+ * ext/SHTnsKitAMDGPUExt.jl:495-557 (which hold a full nlat x (lmax+1) x (mmax+1)
+ * table; this plan holds O(lmax^2 + nlat*mmax) data).  Uses the current HIP
+ * device; allocates device memory. */
+int shtk_plan_create(const shtk_desc *desc, shtk_plan **plan);
+
+/* Frees the plan and its workspace (gpu_clear_cache!, ext/GPUCommon.jl:202-441). */
+int shtk_plan_destroy(shtk_plan *plan);
+
+/* Host copies of cfg.x (cos theta) and cfg.w, north first, length nlat each. */
+int shtk_plan_nodes(const shtk_plan *plan, double *x, double *w);
+
+/* Bytes of device workspace a call with `nfields` fields uses (grown lazily). */
+size_t shtk_workspace_size(const shtk_plan *plan, int nfields, int vector);
+
+/* Scalar synthesis: alm (lmax+1, mmax+1, nfields) complex -> f (nlat, nlon, nfields)
+ * real.  ltr < 0 means lmax; 0 <= ltr <= lmax truncates the degree sum.
+ * real_output must be 1 (the Hermitian real grid of src/core_transforms.jl:294-344).
+ * Replaces synthesis / synthesis_batch (src/core_transforms.jl:271-344,
+ * src/batch_transforms.jl:455-655) behind _gpu_adapter_synthesis
+ * (src/device_utils.jl:91-93). */
+int shtk_synthesis(shtk_plan *plan, const void *alm, void *f, int nfields, int dtype, int real_output,
+                   int ltr, void *stream);
+
+/* Scalar analysis: real f (nlat, nlon, nfields) -> alm.  Replaces analysis /
+ * analysis_batch (src/core_transforms.jl:169-207, src/batch_transforms.jl:312-429)
+ * behind _gpu_adapter_analysis (src/device_utils.jl:87-89). */
+int shtk_analysis(shtk_plan *plan, const void *f, void *alm, int nfields, int dtype, int ltr, void *stream);
+
+/* Vector synthesis (S, T) -> (Vt, Vp): synthesis_sphtor / synthesis_sphtor_batch
+ * (src/sphtor_transforms.jl:162-241, src/batch_transforms.jl:745-815), behind
+ * _gpu_adapter_synthesis_sphtor (src/sphtor_transforms.jl:304-311). */
+int shtk_synthesis_sphtor(shtk_plan *plan, const void *S, const void *T, void *Vt, void *Vp, int nfields,
+                          int dtype, int real_output, int ltr, void *stream);
+
+/* Vector analysis (Vt, Vp) -> (S, T): analysis_sphtor / analysis_sphtor_batch
+ * (src/sphtor_transforms.jl:252-289, src/batch_transforms.jl:689-725), behind
+ * _gpu_adapter_analysis_sphtor (src/sphtor_transforms.jl:298-302). */
+int shtk_analysis_sphtor(shtk_plan *plan, const void *Vt, const void *Vp, void *S, void *T, int nfields,
+                         int dtype, int ltr, void *stream);
+
+/* Adjoint of synthesis: real cotangent fbar (nlat, nlon, nfields) -> almbar.
+ * _adjoint_synthesis, src/core_transforms.jl:477-519 (pullback of the
+ * synthesis rrule, ext/SHTnsKitAdvancedADExt.jl:76-89, and of the batch rrule
+ * :113-133). */
+int shtk_adjoint_synthesis(shtk_plan *plan, const void *fbar, void *almbar, int nfields, int dtype,
+                           int real_output, void *stream);
+
+/* Adjoint of analysis: almbar -> real fbar (nlat, nlon, nfields), the real part
+ * of _adjoint_analysis (src/core_transforms.jl:611-652) that ProjectTo(f) keeps
+ * for a real primal (ext/SHTnsKitAdvancedADExt.jl:55-67). */
+int shtk_adjoint_analysis(shtk_plan *plan, const void *almbar, void *fbar, int nfields, int dtype,
+                          void *stream);
+
+/* Vector adjoints: _adjoint_synthesis_sphtor (src/core_transforms.jl:537-597)
+ * and the real part of _adjoint_analysis_sphtor (src/sphtor_transforms.jl:385-452). */
+int shtk_adjoint_synthesis_sphtor(shtk_plan *plan, const void *Vtb, const void *Vpb, void *Sb, void *Tb,
+                                  int nfields, int dtype, int real_output, void *stream);
+int shtk_adjoint_analysis_sphtor(shtk_plan *plan, const void *Sb, const void *Tb, void *Vtb, void *Vpb,
+                                 int nfields, int dtype, void *stream);
+
+/* Blocks until `stream` is idle. */
+int shtk_stream_sync(void *stream);
+
+/* Copies the last error message of the calling thread into buf (NUL-terminated);
+ * returns the status code of that error. */
+int shtk_last_error(char *buf, size_t len);
+
+/* Library version string, e.g. "0.1.0". */
+const char *shtk_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHTK_H */

@@ -1,0 +1,19 @@
+"""shtnskit.jl_amd -- MI355X (gfx950) backend for SHTnsKit.jl's spherical
+harmonic transforms.
+
+The compute path is libshtk_mi355x.so (hand-written HIP kernels behind the C ABI
+of include/shtk.h).  This Python layer mirrors the reference's operator API so
+host code (and the parity tests) read like SHTnsKit's; the Julia adapter that
+binds the same C ABI lives in ``julia/``.
+"""
+from ._lib import BackendUnavailableError, EXPORTS
+from ._lib import load as load_library
+from .config import SHTConfig, create_gauss_config, create_gauss_fly_config, nlm_calc
+from .layout import LM_index, pack_lm, unpack_lm
+from .transforms import (CPU, GPU, on_device, synthesis, analysis, synthesis_batch, analysis_batch,
+                         synthesis_l, analysis_l, synthesis_sphtor, analysis_sphtor,
+                         synthesis_sphtor_batch, analysis_sphtor_batch, _adjoint_synthesis,
+                         _adjoint_analysis, _adjoint_synthesis_sphtor, _adjoint_analysis_sphtor)
+from . import ad
+
+__version__ = "0.1.0"

@@ -1,0 +1,342 @@
+"""Transforms of SHTnsKit.jl's hot path, executed by libshtk_mi355x.so.
+
+Signatures and semantics mirror the reference:
+
+* ``synthesis(cfg, alm; real_output=True, use_rfft=False)``   src/core_transforms.jl:271-344
+* ``analysis(cfg, f; use_rfft=False)``                         src/core_transforms.jl:169-207
+* ``synthesis_sphtor(cfg, S, T; real_output=True)``            src/sphtor_transforms.jl:162-241
+* ``analysis_sphtor(cfg, Vt, Vp)``                             src/sphtor_transforms.jl:252-289
+* ``synthesis_batch / analysis_batch``                         src/batch_transforms.jl:312-655
+* ``synthesis_sphtor_batch / analysis_sphtor_batch``           src/batch_transforms.jl:689-815
+* ``_adjoint_synthesis / _adjoint_analysis`` and the vector adjoints
+                                                               src/core_transforms.jl:477-652,
+                                                               src/sphtor_transforms.jl:385-452
+
+Arrays are torch tensors on a ROCm device in the reference's index order and
+Julia's column-major memory layout: a grid ``f[i, j(, k)]`` of shape
+``(nlat, nlon[, nf])`` is latitude-fastest, a spectrum ``alm[l, m(, k)]`` of shape
+``(lmax+1, mmax+1[, nf])`` is degree-fastest.  Inputs in another layout are
+copied once into that layout.  Host inputs are rejected unless the call names
+the device (``synthesis(GPU(), cfg, alm)``), which moves them to the current
+device as SHTnsKit's ``_gpu_adapter_adapt`` does.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import BackendUnavailableError
+from .config import SHTConfig
+
+
+class CPU:
+    """Typed device marker (src/devices.jl:7-13)."""
+
+
+class GPU:
+    """Typed device marker (src/devices.jl:15-17); host operands are adapted."""
+
+
+def on_device(x) -> type:
+    return GPU if isinstance(x, torch.Tensor) and x.is_cuda else CPU
+
+
+# --------------------------------------------------------------------------- helpers
+def _split_device(args):
+    if args and isinstance(args[0], (GPU, CPU)) or (args and args[0] in (GPU, CPU)):
+        dev = args[0]
+        if dev is CPU or isinstance(dev, CPU):
+            raise BackendUnavailableError("cpu", "this package is the GPU backend; the CPU path is SHTnsKit's own")
+        return True, args[1:]
+    return False, args
+
+
+def _adapt(x, adapt: bool, op: str):
+    if isinstance(x, np.ndarray):
+        x = torch.from_numpy(np.asarray(x))
+    if not isinstance(x, torch.Tensor):
+        raise TypeError(f"{op}: expected a torch tensor, got {type(x).__name__}")
+    if not x.is_cuda:
+        if not adapt:
+            raise BackendUnavailableError(
+                op, "operand is host memory; pass a device tensor or call with GPU() as first argument")
+        if not torch.cuda.is_available():
+            raise BackendUnavailableError(op, "no ROCm device is available")
+        x = x.to("cuda")
+    return x
+
+
+def _colmajor(t: torch.Tensor) -> torch.Tensor:
+    """Same logical tensor with Julia column-major memory (first index fastest)."""
+    perm = list(range(t.dim() - 1, -1, -1))
+    return t.permute(*perm).contiguous().permute(*perm)
+
+
+def _empty_colmajor(shape, dtype, device) -> torch.Tensor:
+    perm = list(range(len(shape) - 1, -1, -1))
+    return torch.empty([shape[p] for p in perm], dtype=dtype, device=device).permute(*perm)
+
+
+def _real_dtype(t: torch.Tensor) -> torch.dtype:
+    if t.dtype in (torch.float32, torch.complex64):
+        return torch.float32
+    if t.dtype in (torch.float64, torch.complex128):
+        return torch.float64
+    raise TypeError(f"unsupported element type {t.dtype}; use float32/64 or complex64/128")
+
+
+def _code(rdt: torch.dtype) -> int:
+    return _lib.SHTK_F32 if rdt == torch.float32 else _lib.SHTK_F64
+
+
+def _cplx(rdt: torch.dtype) -> torch.dtype:
+    return torch.complex64 if rdt == torch.float32 else torch.complex128
+
+
+def _stream(device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _spec_in(cfg: SHTConfig, alm, adapt: bool, op: str, batch: bool):
+    alm = _adapt(alm, adapt, op)
+    want = 3 if batch else 2
+    if alm.dim() != want:
+        raise ValueError(f"{op}: coefficients must have {want} dimensions, got shape {tuple(alm.shape)}")
+    if alm.shape[0] != cfg.lmax + 1:
+        raise ValueError(f"{op}: first dim must be lmax+1={cfg.lmax + 1}")
+    if alm.shape[1] != cfg.mmax + 1:
+        raise ValueError(f"{op}: second dim must be mmax+1={cfg.mmax + 1}")
+    if batch and alm.shape[2] < 1:
+        raise ValueError(f"{op} requires at least one field")
+    rdt = _real_dtype(alm)
+    alm = _colmajor(alm.to(_cplx(rdt)))
+    return alm, rdt
+
+
+def _grid_in(cfg: SHTConfig, f, adapt: bool, op: str, batch: bool):
+    f = _adapt(f, adapt, op)
+    want = 3 if batch else 2
+    if f.dim() != want:
+        raise ValueError(f"{op}: field must have {want} dimensions, got shape {tuple(f.shape)}")
+    if f.shape[0] != cfg.nlat:
+        raise ValueError(f"{op}: first dim must be nlat={cfg.nlat}")
+    if f.shape[1] != cfg.nlon:
+        raise ValueError(f"{op}: second dim must be nlon={cfg.nlon}")
+    if batch and f.shape[2] < 1:
+        raise ValueError(f"{op} requires at least one field")
+    if f.is_complex():
+        raise BackendUnavailableError(op, "complex-valued fields are not handled by this backend")
+    rdt = _real_dtype(f)
+    f = _colmajor(f.to(rdt))
+    return f, rdt
+
+
+def _nf(t: torch.Tensor, batch: bool) -> int:
+    return int(t.shape[2]) if batch else 1
+
+
+def _check_kw(op, real_output=True, fft_scratch=None):
+    if fft_scratch is not None and not (isinstance(fft_scratch, torch.Tensor) and fft_scratch.is_cuda):
+        raise ValueError(f"{op}: fft_scratch must be device storage")
+
+
+# --------------------------------------------------------------------------- scalar
+def _synthesis(args, kw, batch, op):
+    adapt, args = _split_device(args)
+    cfg, alm = args
+    real_output = kw.pop("real_output", True)
+    kw.pop("use_rfft", None)
+    ltr = kw.pop("ltr", -1)
+    _check_kw(op, real_output, kw.pop("fft_scratch", None))
+    if kw:
+        raise TypeError(f"{op}: unexpected keywords {sorted(kw)}")
+    alm, rdt = _spec_in(cfg, alm, adapt, op, batch)
+    nf = _nf(alm, batch)
+    shape = (cfg.nlat, cfg.nlon) + ((nf,) if batch else ())
+    out = _empty_colmajor(shape, rdt if real_output else _cplx(rdt), alm.device)
+    lib = _lib.load()
+    _lib.check(lib.shtk_synthesis(ctypes.c_void_p(cfg.plan), _ptr(alm), _ptr(out), nf, _code(rdt),
+                                  1 if real_output else 0, int(ltr), _stream(alm.device)), op)
+    return out
+
+
+def _analysis(args, kw, batch, op):
+    adapt, args = _split_device(args)
+    cfg, f = args
+    kw.pop("use_rfft", None)
+    ltr = kw.pop("ltr", -1)
+    _check_kw(op, True, kw.pop("fft_scratch", None))
+    if kw:
+        raise TypeError(f"{op}: unexpected keywords {sorted(kw)}")
+    f, rdt = _grid_in(cfg, f, adapt, op, batch)
+    nf = _nf(f, batch)
+    shape = (cfg.lmax + 1, cfg.mmax + 1) + ((nf,) if batch else ())
+    out = _empty_colmajor(shape, _cplx(rdt), f.device)
+    lib = _lib.load()
+    _lib.check(lib.shtk_analysis(ctypes.c_void_p(cfg.plan), _ptr(f), _ptr(out), nf, _code(rdt), int(ltr),
+                                 _stream(f.device)), op)
+    return out
+
+
+def synthesis(*args, **kw):
+    """``synthesis(cfg, alm; real_output=True, use_rfft=False)`` -> grid (nlat, nlon)."""
+    return _synthesis(args, dict(kw), False, "synthesis")
+
+
+def analysis(*args, **kw):
+    """``analysis(cfg, f; use_rfft=False)`` -> coefficients (lmax+1, mmax+1)."""
+    return _analysis(args, dict(kw), False, "analysis")
+
+
+def synthesis_batch(*args, **kw):
+    """``synthesis_batch(cfg, alm (lmax+1, mmax+1, nf); real_output=True)`` -> (nlat, nlon, nf)."""
+    return _synthesis(args, dict(kw), True, "synthesis_batch")
+
+
+def analysis_batch(*args, **kw):
+    """``analysis_batch(cfg, fields (nlat, nlon, nf))`` -> (lmax+1, mmax+1, nf)."""
+    return _analysis(args, dict(kw), True, "analysis_batch")
+
+
+def synthesis_l(cfg, alm, ltr: int, **kw):
+    """Degree-truncated synthesis (``_synthesis_l``, src/core_transforms.jl:346-368)."""
+    return _synthesis((cfg, alm), dict(kw, ltr=ltr), False, "synthesis_l")
+
+
+def analysis_l(cfg, f, ltr: int, **kw):
+    """Degree-truncated analysis: coefficients above ``ltr`` are zero."""
+    return _analysis((cfg, f), dict(kw, ltr=ltr), False, "analysis_l")
+
+
+# --------------------------------------------------------------------------- vector
+def _synthesis_sphtor(args, kw, batch, op):
+    adapt, args = _split_device(args)
+    cfg, S, T = args
+    real_output = kw.pop("real_output", True)
+    kw.pop("use_rfft", None)
+    ltr = kw.pop("ltr", -1)
+    if kw:
+        raise TypeError(f"{op}: unexpected keywords {sorted(kw)}")
+    S, rdt = _spec_in(cfg, S, adapt, op, batch)
+    T, rdt2 = _spec_in(cfg, T, adapt, op, batch)
+    if rdt2 != rdt:
+        rdt = torch.float64
+        S, T = S.to(torch.complex128), T.to(torch.complex128)
+        S, T = _colmajor(S), _colmajor(T)
+    if S.shape != T.shape:
+        raise ValueError(f"{op}: Slm and Tlm dims differ")
+    nf = _nf(S, batch)
+    shape = (cfg.nlat, cfg.nlon) + ((nf,) if batch else ())
+    Vt = _empty_colmajor(shape, rdt, S.device)
+    Vp = _empty_colmajor(shape, rdt, S.device)
+    lib = _lib.load()
+    _lib.check(lib.shtk_synthesis_sphtor(ctypes.c_void_p(cfg.plan), _ptr(S), _ptr(T), _ptr(Vt), _ptr(Vp), nf,
+                                         _code(rdt), 1 if real_output else 0, int(ltr), _stream(S.device)), op)
+    return Vt, Vp
+
+
+def _analysis_sphtor(args, kw, batch, op):
+    adapt, args = _split_device(args)
+    cfg, Vt, Vp = args
+    kw.pop("use_rfft", None)
+    ltr = kw.pop("ltr", -1)
+    if kw:
+        raise TypeError(f"{op}: unexpected keywords {sorted(kw)}")
+    Vt, rdt = _grid_in(cfg, Vt, adapt, op, batch)
+    Vp, rdt2 = _grid_in(cfg, Vp, adapt, op, batch)
+    if rdt2 != rdt:
+        rdt = torch.float64
+        Vt, Vp = _colmajor(Vt.to(rdt)), _colmajor(Vp.to(rdt))
+    if Vt.shape != Vp.shape:
+        raise ValueError(f"{op}: Vt and Vp dims differ")
+    nf = _nf(Vt, batch)
+    shape = (cfg.lmax + 1, cfg.mmax + 1) + ((nf,) if batch else ())
+    S = _empty_colmajor(shape, _cplx(rdt), Vt.device)
+    T = _empty_colmajor(shape, _cplx(rdt), Vt.device)
+    lib = _lib.load()
+    _lib.check(lib.shtk_analysis_sphtor(ctypes.c_void_p(cfg.plan), _ptr(Vt), _ptr(Vp), _ptr(S), _ptr(T), nf,
+                                        _code(rdt), int(ltr), _stream(Vt.device)), op)
+    return S, T
+
+
+def synthesis_sphtor(*args, **kw):
+    """``synthesis_sphtor(cfg, Slm, Tlm; real_output=True)`` -> (Vt, Vp)."""
+    return _synthesis_sphtor(args, dict(kw), False, "synthesis_sphtor")
+
+
+def analysis_sphtor(*args, **kw):
+    """``analysis_sphtor(cfg, Vt, Vp)`` -> (Slm, Tlm)."""
+    return _analysis_sphtor(args, dict(kw), False, "analysis_sphtor")
+
+
+def synthesis_sphtor_batch(*args, **kw):
+    return _synthesis_sphtor(args, dict(kw), True, "synthesis_sphtor_batch")
+
+
+def analysis_sphtor_batch(*args, **kw):
+    return _analysis_sphtor(args, dict(kw), True, "analysis_sphtor_batch")
+
+
+# --------------------------------------------------------------------------- adjoints
+def _batchdim(t: torch.Tensor) -> bool:
+    return t.dim() == 3
+
+
+def _adjoint_synthesis(cfg: SHTConfig, fbar, real_output: bool = True):
+    """Adjoint of ``synthesis`` (Zygote/torch convention ``dL/dRe + i dL/dIm``)."""
+    batch = _batchdim(fbar)
+    fbar, rdt = _grid_in(cfg, fbar, False, "_adjoint_synthesis", batch)
+    nf = _nf(fbar, batch)
+    shape = (cfg.lmax + 1, cfg.mmax + 1) + ((nf,) if batch else ())
+    out = _empty_colmajor(shape, _cplx(rdt), fbar.device)
+    _lib.check(_lib.load().shtk_adjoint_synthesis(ctypes.c_void_p(cfg.plan), _ptr(fbar), _ptr(out), nf, _code(rdt),
+                                                  1 if real_output else 0, _stream(fbar.device)),
+               "_adjoint_synthesis")
+    return out
+
+
+def _adjoint_analysis(cfg: SHTConfig, almbar):
+    """Real part of the adjoint of ``analysis`` (what ``ProjectTo`` keeps for real fields)."""
+    batch = _batchdim(almbar)
+    almbar, rdt = _spec_in(cfg, almbar, False, "_adjoint_analysis", batch)
+    nf = _nf(almbar, batch)
+    shape = (cfg.nlat, cfg.nlon) + ((nf,) if batch else ())
+    out = _empty_colmajor(shape, rdt, almbar.device)
+    _lib.check(_lib.load().shtk_adjoint_analysis(ctypes.c_void_p(cfg.plan), _ptr(almbar), _ptr(out), nf,
+                                                 _code(rdt), _stream(almbar.device)), "_adjoint_analysis")
+    return out
+
+
+def _adjoint_synthesis_sphtor(cfg: SHTConfig, Vtb, Vpb, real_output: bool = True):
+    batch = _batchdim(Vtb)
+    Vtb, rdt = _grid_in(cfg, Vtb, False, "_adjoint_synthesis_sphtor", batch)
+    Vpb, _ = _grid_in(cfg, Vpb.to(Vtb.dtype), False, "_adjoint_synthesis_sphtor", batch)
+    nf = _nf(Vtb, batch)
+    shape = (cfg.lmax + 1, cfg.mmax + 1) + ((nf,) if batch else ())
+    S = _empty_colmajor(shape, _cplx(rdt), Vtb.device)
+    T = _empty_colmajor(shape, _cplx(rdt), Vtb.device)
+    _lib.check(_lib.load().shtk_adjoint_synthesis_sphtor(ctypes.c_void_p(cfg.plan), _ptr(Vtb), _ptr(Vpb), _ptr(S),
+                                                         _ptr(T), nf, _code(rdt), 1 if real_output else 0,
+                                                         _stream(Vtb.device)), "_adjoint_synthesis_sphtor")
+    return S, T
+
+
+def _adjoint_analysis_sphtor(cfg: SHTConfig, Sb, Tb):
+    batch = _batchdim(Sb)
+    Sb, rdt = _spec_in(cfg, Sb, False, "_adjoint_analysis_sphtor", batch)
+    Tb, _ = _spec_in(cfg, Tb.to(Sb.dtype), False, "_adjoint_analysis_sphtor", batch)
+    nf = _nf(Sb, batch)
+    shape = (cfg.nlat, cfg.nlon) + ((nf,) if batch else ())
+    Vt = _empty_colmajor(shape, rdt, Sb.device)
+    Vp = _empty_colmajor(shape, rdt, Sb.device)
+    _lib.check(_lib.load().shtk_adjoint_analysis_sphtor(ctypes.c_void_p(cfg.plan), _ptr(Sb), _ptr(Tb), _ptr(Vt),
+                                                        _ptr(Vp), nf, _code(rdt), _stream(Sb.device)),
+               "_adjoint_analysis_sphtor")
+    return Vt, Vp

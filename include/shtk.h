@@ -124,6 +124,14 @@ int shtk_adjoint_synthesis_sphtor(shtk_plan *plan, const void *Vtb, const void *
 int shtk_adjoint_analysis_sphtor(shtk_plan *plan, const void *Sb, const void *Tb, void *Vtb, void *Vpb,
                                  int nfields, int dtype, void *stream);
 
+/* Per-kernel timing for benchmarks; no counterpart in the reference.  While
+ * enabled, every kernel this plan launches is bracketed by HIP events on the
+ * caller's stream.  shtk_profile_read() waits for the recorded events and
+ * returns, per kernel name (up to `max` entries, names NUL-terminated in
+ * 48-byte slots), the accumulated milliseconds and launch counts, then resets. */
+int shtk_profile_enable(shtk_plan *plan, int on);
+int shtk_profile_read(shtk_plan *plan, int max, char *names, double *total_ms, int *count, int *n_out);
+
 /* Blocks until `stream` is idle. */
 int shtk_stream_sync(void *stream);
 

@@ -20,7 +20,8 @@ EXPORTS = (
     "shtk_plan_create", "shtk_plan_destroy", "shtk_plan_nodes", "shtk_workspace_size",
     "shtk_synthesis", "shtk_analysis", "shtk_synthesis_sphtor", "shtk_analysis_sphtor",
     "shtk_adjoint_synthesis", "shtk_adjoint_analysis", "shtk_adjoint_synthesis_sphtor",
-    "shtk_adjoint_analysis_sphtor", "shtk_stream_sync", "shtk_last_error", "shtk_version",
+    "shtk_adjoint_analysis_sphtor", "shtk_profile_enable", "shtk_profile_read", "shtk_stream_sync",
+    "shtk_last_error", "shtk_version",
 )
 
 
@@ -68,6 +69,8 @@ def load():
     lib.shtk_adjoint_analysis.argtypes = [p, p, p, i, i, p]
     lib.shtk_adjoint_synthesis_sphtor.argtypes = [p, p, p, p, p, i, i, i, p]
     lib.shtk_adjoint_analysis_sphtor.argtypes = [p, p, p, p, p, i, i, p]
+    lib.shtk_profile_enable.argtypes = [p, i]
+    lib.shtk_profile_read.argtypes = [p, i, p, p, p, p]
     lib.shtk_stream_sync.argtypes = [p]
     lib.shtk_last_error.argtypes = [ctypes.c_char_p, sz]
     lib.shtk_version.restype = ctypes.c_char_p
@@ -93,3 +96,22 @@ def check(rc: int, operation: str) -> None:
     if rc == SHTK_ENOMEM:
         raise MemoryError(f"{operation}: {msg}")
     raise RuntimeError(f"{operation}: HIP error: {msg}")
+
+
+def profile_enable(plan: int, on: bool) -> None:
+    check(load().shtk_profile_enable(ctypes.c_void_p(plan), 1 if on else 0), "profile_enable")
+
+
+def profile_read(plan: int) -> dict:
+    """{kernel name: (total_ms, launches)} since the last read (waits for the events)."""
+    n = 16
+    names = ctypes.create_string_buffer(48 * n)
+    ms = (ctypes.c_double * n)()
+    cnt = (ctypes.c_int * n)()
+    nout = ctypes.c_int()
+    check(load().shtk_profile_read(ctypes.c_void_p(plan), n, names, ms, cnt, ctypes.byref(nout)), "profile_read")
+    out = {}
+    for k in range(nout.value):
+        nm = names.raw[48 * k:48 * (k + 1)].split(b"\0", 1)[0].decode()
+        out[nm] = (ms[k], cnt[k])
+    return out

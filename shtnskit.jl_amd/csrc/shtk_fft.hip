@@ -93,7 +93,7 @@ __device__ __forceinline__ void dft_small(double2 *v) {
 // stages all its inputs in registers before the barrier, then writes.
 template <int R, int SIGN>
 __device__ __forceinline__ void stockham_pass(double2 *z, int N, int Ns, const double2 *tw, int tid, int nt) {
-    constexpr int MAXB = 16 / R + 1;
+    constexpr int MAXB = 8 / R + 1;  // at most N/8 butterflies' worth of values per thread
     double2 v[MAXB][R];
     const int nb = N / R;
 #pragma unroll
@@ -131,7 +131,7 @@ __device__ void fft_lds(double2 *z, const FftArgs &a, int tid, int nt) {
     const int N = a.nlon;
     int Ns = 1;
     for (int q = 0; q < a.nrad; ++q) {
-        const int R = a.radix[q];
+        const int R = (int)((a.radix >> (4 * q)) & 15ull);
         switch (R) {
             case 8: stockham_pass<8, SIGN>(z, N, Ns, a.tw, tid, nt); break;
             case 4: stockham_pass<4, SIGN>(z, N, Ns, a.tw, tid, nt); break;
@@ -240,7 +240,7 @@ hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st
     if (nt < 64) nt = 64;
     if (nt > 1024) nt = 1024;
     nt = (nt + 63) / 64 * 64;
-    if (N > 16 * nt) return hipErrorNotSupported;
+    if (N > 8 * nt) return hipErrorNotSupported;
     const size_t lds = sizeof(double2) * (size_t)N;
     if (lds > 160 * 1024) return hipErrorNotSupported;
     const int per = (a.np + 7) / 8;

@@ -3,27 +3,31 @@
 // Reference behaviour (SHTnsKit.jl):
 //   rows       src/legendre.jl:233-322 (orthonormal + Condon-Shortley recurrence,
 //              scaled exponents for seeds that underflow), 470-555 (d/dtheta and
-//              1/sin(theta) rows), 326-338 (zonal d/dtheta)
+//              1/sin(theta) rows)
 //   synthesis  src/kernels.jl:57-65 and 155-169, orchestrated per m by
 //              src/core_transforms.jl:749-869 and src/sphtor_transforms.jl:455-525
 //   analysis   src/kernels.jl:33-40 and 205-215, orchestrated per m by
 //              src/core_transforms.jl:655-746 and src/sphtor_transforms.jl:528-612
 //
 // Design (MI355X-first, not a translation of the per-(m, latitude) loops):
-//   * Gauss nodes come in +/- x pairs; P_l^m(-x) = (-1)^(l-m) P_l^m(x), so each
+//   * Gauss nodes come in +/- x pairs; P_l^m(-x) = (-1)^(l-m) P_l^m(x), so one
 //     recurrence chain serves a north/south latitude pair and the even/odd
-//     degree sums give both hemispheres (half the recurrence work).
-//   * The recurrence coefficients (a_l, b_l) are precomputed once per plan in
-//     FP64 and read as wave-uniform scalar loads, so a step is mul + mul + fma.
-//   * Seeds that underflow FP64 carry a 2^(512k) scale; scaled chains run the
-//     recurrence without contributing (their physical value is < 2^-359) until
-//     they come back into range.  Once every lane of a wave is live the wave
-//     leaves the checked loop for a branch-free loop.
-//   * Synthesis: one lane per latitude pair, 256-lane workgroups per
-//     (field, m, latitude block); no cross-lane traffic.
-//   * Analysis: one workgroup per (field, m); each lane holds K latitude-pair
-//     chains, sums them in registers, and the per-degree sums over lanes are
-//     formed through an LDS transpose every 8 degrees.
+//     degree sums give both hemispheres.
+//   * Rows are cut into degree segments.  Within a segment the recurrence runs
+//     on Q_l = P_l / c_l, Q_{l+1} = x Q_l - beta_{l+1} Q_{l-1} (one mul and one
+//     fma per step); c_l is folded into the coefficients (synthesis) or into
+//     the reduced sums (analysis).  A plan-time kernel stores each segment's
+//     starting pair, so analysis work items are (order, segment) pairs of at
+//     most `seg` degrees: balanced work, no cross-item reductions.
+//   * Per-step coefficient records are staged once per segment in LDS and read
+//     as broadcasts; the VALU only runs the recurrence and the contraction.
+//   * Values that underflow FP64 carry a 2^(512k) scale; such chains run the
+//     recurrence without contributing (physical value < 2^-359).  Chains that
+//     stay below that level for a whole segment are skipped entirely (the
+//     polar caps of high orders); blocks whose chains are all in range at the
+//     segment start take a branch-free loop.
+//   * Analysis: each lane sums its K chains in registers; per-degree sums over
+//     lanes go through a wave-private LDS transpose every few degrees.
 #include "shtk_internal.h"
 
 namespace shtk {
@@ -49,120 +53,422 @@ __device__ __forceinline__ void st_c(T *p, size_t idx, double re, double im) {
     p[2 * idx + 1] = (T)im;
 }
 
-// ---------------------------------------------------------------------------
-// Scalar synthesis: F[f][im][i] = g * mfac(m) * wlat * sum_l P_l^m(x_i) a_lm
-// ---------------------------------------------------------------------------
-template <typename IT>
-__global__ __launch_bounds__(256) void k_leg_synth(const LegArgs a) {
-    const int im = blockIdx.y;
-    const int f = blockIdx.z;
-    const int m = im * a.mres;
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    const bool act = p < a.np;
-    const int pc = act ? p : 0;
-    const double x = a.x[pc];
-    const size_t sidx = (size_t)im * a.np + pc;
-    double cur = act ? a.seedv[sidx] : 0.0;
-    int k = act ? a.seedk[sidx] : 0;
-    double prev = 0.0;
-    const double2 *__restrict__ ab = a.ab + a.aboff[im] - m;
-    const IT *__restrict__ al = (const IT *)a.in0 + 2 * ((size_t)f * a.spec_fstride + (size_t)(a.lmax + 1) * m);
-    double er = 0.0, ei = 0.0, orr = 0.0, oi = 0.0;
-    const int ltr = a.ltr;
-    int l = m;
+template <typename T>
+__device__ __forceinline__ void add_c(T *p, size_t idx, double re, double im) {
+    atomicAdd(p + 2 * idx, (T)re);
+    atomicAdd(p + 2 * idx + 1, (T)im);
+}
 
-    if (!__all(k == 0)) {
-        double c = (k == 0) ? 1.0 : 0.0;
-        while (l <= ltr) {
-            const int lend = min(l + SCALE_CHECK, ltr + 1);
-            for (; l < lend; ++l) {
-                const double v = cur * c;
-                const double ar = (double)al[2 * l], ai = (double)al[2 * l + 1];
-                if ((l - m) & 1) {
-                    orr = fma(v, ar, orr);
-                    oi = fma(v, ai, oi);
-                } else {
-                    er = fma(v, ar, er);
-                    ei = fma(v, ai, ei);
-                }
-                const double2 co = ab[l + 1];
-                const double nx = fma(co.x * x, cur, -(co.y * prev));
-                prev = cur;
-                cur = nx;
-            }
-            if (k > 0 && fmax(fabs(cur), fabs(prev)) > kHi) {
+__device__ __forceinline__ int nseg_of(int lmax, int m, int S) { return (lmax - m + S) / S; }
+
+// ---------------------------------------------------------------------------
+// Plan-time seeds: one thread per (order, latitude pair) runs the P-form
+// recurrence with scaled exponents (src/legendre.jl:252-322) through the row
+// and stores each segment's starting pair in the segment's Q normalisation.
+// It also records, per segment, the first pair that is live anywhere in the
+// segment (pfirst) and the end of the range of pairs still scaled at its start
+// (pfast).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_seed(const SeedArgs a) {
+    const int im = blockIdx.y;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.np) return;
+    const int m = im * a.mres;
+    const int lmax = a.lmax, S = a.seg;
+    const double x = a.x[p];
+    const double2 *ab = a.ab + a.coff[im] - m;
+    const size_t sidx = (size_t)im * a.np + p;
+    double cur = a.sectv[sidx], prev = 0.0;
+    int k = a.sectk[sidx];
+    const int nseg = nseg_of(lmax, m, S);
+    int l = m;
+    for (int s = 0; s < nseg; ++s) {
+        const int gs = a.segbase[im] + s;
+        const int l0 = m + s * S;
+        const size_t q = (size_t)gs * a.np + p;
+        if (s == 0) {
+            a.seedq0[q] = 0.0;
+            a.seedq1[q] = cur;
+        } else {
+            a.seedq0[q] = prev;
+            a.seedq1[q] = cur / ab[l0].x;
+        }
+        a.seedk[q] = k;
+        if (k > 0) atomicMax(&a.pfast[gs], p + 1);
+        const int lend = min(l0 + S, lmax + 1);
+        for (; l < lend; ++l) {
+            const double2 co = ab[l + 1];
+            const double nx = fma(co.x * x, cur, -(co.y * prev));
+            prev = cur;
+            cur = nx;
+            if (k > 0 && (((l - l0) & (SCALE_CHECK - 1)) == SCALE_CHECK - 1 || l == lend - 1) &&
+                fmax(fabs(cur), fabs(prev)) > kHi) {
                 cur *= kDown;
                 prev *= kDown;
                 --k;
-                c = (k == 0) ? 1.0 : 0.0;
             }
-            if (__all(k == 0)) break;
         }
-    }
-    if (l <= ltr && ((l - m) & 1)) {
-        orr = fma(cur, (double)al[2 * l], orr);
-        oi = fma(cur, (double)al[2 * l + 1], oi);
-        const double2 co = ab[l + 1];
-        const double nx = fma(co.x * x, cur, -(co.y * prev));
-        prev = cur;
-        cur = nx;
-        ++l;
-    }
-    for (; l + 1 <= ltr; l += 2) {
-        const double ar0 = (double)al[2 * l], ai0 = (double)al[2 * l + 1];
-        const double ar1 = (double)al[2 * l + 2], ai1 = (double)al[2 * l + 3];
-        const double2 c0 = ab[l + 1], c1 = ab[l + 2];
-        er = fma(cur, ar0, er);
-        ei = fma(cur, ai0, ei);
-        const double n0 = fma(c0.x * x, cur, -(c0.y * prev));
-        orr = fma(n0, ar1, orr);
-        oi = fma(n0, ai1, oi);
-        const double n1 = fma(c1.x * x, n0, -(c1.y * cur));
-        prev = n0;
-        cur = n1;
-    }
-    if (l <= ltr) {
-        er = fma(cur, (double)al[2 * l], er);
-        ei = fma(cur, (double)al[2 * l + 1], ei);
-    }
-    if (act) {
-        const double sc = a.g * mfac_of(a.mfac_mode, m) * (a.wlat ? a.wlat[p] : 1.0);
-        double2 *Fo = a.F0 + ((size_t)f * a.nm + im) * a.nlat;
-        Fo[p] = make_double2((er + orr) * sc, (ei + oi) * sc);
-        const int ps = a.nlat - 1 - p;
-        if (ps != p) Fo[ps] = make_double2((er - orr) * sc, (ei - oi) * sc);
+        if (k == 0) atomicMin(&a.pfirst[gs], p);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Scalar analysis: a_lm = g * mfac(m) * sum_i wlat_i P_l^m(x_i) F[f][im][i]
+// Synthesis.  One workgroup of four waves per (pair block, order, field); the
+// waves take the degree segments of the order round-robin, run the P-form
+// recurrence (P_{l+1} = a x P_l - b P_{l-1}) from the segment seeds, and add
+// their partial sums through LDS at the end in a fixed order.  Per-degree
+// coefficients are wave-uniform and stream through scalar loads, prefetched
+// one block of degrees ahead.
 // ---------------------------------------------------------------------------
-template <int K, int W, typename OT>
-__global__ __launch_bounds__(64 * W) void k_leg_anal(const LegArgs a) {
-    constexpr int NT = 64 * W, CH = 8, TPL = NT / CH;
-    __shared__ double2 buf[CH * NT];
-    const int im = blockIdx.x;
-    const int f = blockIdx.y;
-    const int m = im * a.mres;
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int lmax = a.lmax, ltr = a.ltr, np = a.np, nlat = a.nlat;
-
-    double x[K], cur[K], prev[K], gpr[K], gpi[K], gmr[K], gmi[K], c[K];
-    int kk[K];
-    const double2 *Fi = a.F0 + ((size_t)f * a.nm + im) * nlat;
-    bool live = true;
+template <int K, int PAR, bool FAST>
+__device__ __forceinline__ void sstep(const double4 &r, const double *x, double *prev, double *cur, const double *cm,
+                                      double (*acc)[4]) {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        const int p = wv * 64 * K + lane + 64 * j;
-        const bool act = p < np;
+        const double v = FAST ? cur[j] : cur[j] * cm[j];
+        acc[j][2 * PAR] = fma(v, r.z, acc[j][2 * PAR]);
+        acc[j][2 * PAR + 1] = fma(v, r.w, acc[j][2 * PAR + 1]);
+        const double nx = fma(r.x * x[j], cur[j], -(r.y * prev[j]));
+        prev[j] = cur[j];
+        cur[j] = nx;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void rescale(double *prev, double *cur, int *kk, double *cm) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) > kHi) {
+            cur[j] *= kDown;
+            prev[j] *= kDown;
+            kk[j] -= 1;
+            cm[j] = kk[j] == 0 ? 1.0 : 0.0;
+        }
+    }
+}
+
+// Degrees l0 .. lstop-1 of one segment; (l - l0) has the parity of (l - m).
+template <int K, bool FAST, typename IT>
+__device__ __forceinline__ void synth_seg(const double2 *__restrict__ ab, const IT *__restrict__ al, int l0,
+                                          int lstop, int lmax, const double *x, double *prev, double *cur,
+                                          int *kk, double (*acc)[4]) {
+    double cm[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) cm[j] = kk[j] == 0 ? 1.0 : 0.0;
+    auto ld = [&](int l) {
+        const int lc = min(l, lmax);
+        const double2 c = ab[lc + 1];
+        return make_double4(c.x, c.y, (double)al[2 * lc], (double)al[2 * lc + 1]);
+    };
+    double4 r0 = ld(l0), r1 = ld(l0 + 1), r2 = ld(l0 + 2), r3 = ld(l0 + 3);
+    int l = l0;
+    for (; l + 4 <= lstop; l += 4) {
+        const double4 n0 = ld(l + 4), n1 = ld(l + 5), n2 = ld(l + 6), n3 = ld(l + 7);
+        sstep<K, 0, FAST>(r0, x, prev, cur, cm, acc);
+        sstep<K, 1, FAST>(r1, x, prev, cur, cm, acc);
+        sstep<K, 0, FAST>(r2, x, prev, cur, cm, acc);
+        sstep<K, 1, FAST>(r3, x, prev, cur, cm, acc);
+        if (!FAST && ((l - l0) & (SCALE_CHECK - 1)) == SCALE_CHECK - 4) rescale<K>(prev, cur, kk, cm);
+        r0 = n0;
+        r1 = n1;
+        r2 = n2;
+        r3 = n3;
+    }
+    if (l < lstop) sstep<K, 0, FAST>(r0, x, prev, cur, cm, acc);
+    if (l + 1 < lstop) sstep<K, 1, FAST>(r1, x, prev, cur, cm, acc);
+    if (l + 2 < lstop) sstep<K, 0, FAST>(r2, x, prev, cur, cm, acc);
+}
+
+template <int K, typename IT>
+__global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
+    __shared__ double red[4][4 * K][64];
+    const int b = blockIdx.x, im = blockIdx.y, f = blockIdx.z;
+    const int m = im * a.mres;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int S = a.seg, np = a.np, lmax = a.lmax, ltr = a.ltr;
+    const int pb0 = b * 64 * K, pb1 = min(pb0 + 64 * K, np);
+    double x[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int p = pb0 + lane + 64 * j;
+        x[j] = p < np ? a.x[p] : 0.0;
+    }
+    const double2 *__restrict__ ab = a.ab + a.coff[im] - m;
+    const IT *__restrict__ al = (const IT *)a.in0 + 2 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m);
+    double acc[K][4];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0.0;
+    const int nseg = nseg_of(lmax, m, S);
+    for (int s = w; s < nseg; s += 4) {
+        const int l0 = m + s * S;
+        if (l0 > ltr) break;
+        const int gs = a.segbase[im] + s;
+        if (pb1 <= a.pfirst[gs]) continue;
+        const bool fast = pb0 >= a.pfast[gs];
+        const int lstop = min(l0 + S, ltr + 1);
+        const double c0 = s == 0 ? 1.0 : ab[l0].x;  // P_{l0} = a_{l0} Q_{l0}
+        double prev[K], cur[K];
+        int kk[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int p = pb0 + lane + 64 * j;
+            const bool act = p < np;
+            const size_t q = (size_t)gs * np + (act ? p : 0);
+            prev[j] = act ? a.seedq0[q] : 0.0;
+            cur[j] = act ? a.seedq1[q] * c0 : 0.0;
+            kk[j] = act ? a.seedk[q] : 0;
+        }
+        if (fast) synth_seg<K, true>(ab, al, l0, lstop, lmax, x, prev, cur, kk, acc);
+        else synth_seg<K, false>(ab, al, l0, lstop, lmax, x, prev, cur, kk, acc);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[w][4 * j + q][lane] = acc[j][q];
+    __syncthreads();
+    if (w != 0) return;
+    const double gm = a.g * mfac_of(a.mfac_mode, m);
+    double2 *Fo = a.F0 + ((size_t)f * a.nm + im) * a.nlat;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int p = pb0 + lane + 64 * j;
+        if (p >= np) continue;
+        double t[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            t[q] = ((red[0][4 * j + q][lane] + red[1][4 * j + q][lane]) + red[2][4 * j + q][lane]) +
+                   red[3][4 * j + q][lane];
+        const double sc = gm * (a.wlat ? a.wlat[p] : 1.0);
+        Fo[p] = make_double2((t[0] + t[2]) * sc, (t[1] + t[3]) * sc);
+        const int ps = a.nlat - 1 - p;
+        if (ps != p) Fo[ps] = make_double2((t[0] - t[2]) * sc, (t[1] - t[3]) * sc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vector synthesis: s*Vt = sum_l [D_l S_l - i m P_l T_l], s*Vp = sum_l [i m P_l S_l + D_l T_l],
+// D_l = s dP_l/dtheta = l x P_l - g_l P_{l-1}; output times wlat (1/s, or 1 in
+// Robert form).  Parity: P(-x) = (-1)^(l-m) P(x), D(-x) = -(-1)^(l-m) D(x).
+// Accumulators per parity: A = sum D S, B = sum P T, C = sum P S, E = sum D T.
+// ---------------------------------------------------------------------------
+struct VCoef {
+    double a, b, g, l;      // recurrence a_{l+1}, b_{l+1}; g_l; l
+    double sr, si, tr, ti;  // S_l, T_l (divided by l(l+1) when requested)
+};
+
+template <int K, int PAR, bool FAST>
+__device__ __forceinline__ void vstep(const VCoef &r, const double *x, double *prev, double *cur, const double *cm,
+                                      double (*acc)[2][8]) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const double t = x[j] * cur[j];
+        double D = fma(r.l, t, -(r.g * prev[j]));
+        double P = cur[j];
+        if (!FAST) {
+            D *= cm[j];
+            P *= cm[j];
+        }
+        double *A = acc[j][PAR];
+        A[0] = fma(D, r.sr, A[0]);
+        A[1] = fma(D, r.si, A[1]);
+        A[2] = fma(P, r.tr, A[2]);
+        A[3] = fma(P, r.ti, A[3]);
+        A[4] = fma(P, r.sr, A[4]);
+        A[5] = fma(P, r.si, A[5]);
+        A[6] = fma(D, r.tr, A[6]);
+        A[7] = fma(D, r.ti, A[7]);
+        const double nx = fma(r.a, t, -(r.b * prev[j]));
+        prev[j] = cur[j];
+        cur[j] = nx;
+    }
+}
+
+template <int K, bool FAST, typename IT>
+__device__ __forceinline__ void vsynth_seg(const double2 *__restrict__ ab, const double *__restrict__ gt,
+                                           const IT *__restrict__ Sl, const IT *__restrict__ Tl, int ll1, int l0,
+                                           int lstop, int lmax, const double *x, double *prev, double *cur,
+                                           int *kk, double (*acc)[2][8]) {
+    double cm[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) cm[j] = kk[j] == 0 ? 1.0 : 0.0;
+    auto ld = [&](int l) {
+        const int lc = min(l, lmax);
+        const double2 c = ab[lc + 1];
+        VCoef r;
+        r.a = c.x;
+        r.b = c.y;
+        r.g = gt[lc];
+        r.l = (double)lc;
+        double sc = 1.0;
+        if (ll1) sc = lc > 0 ? 1.0 / ((double)lc * (lc + 1)) : 0.0;
+        r.sr = sc * (double)Sl[2 * lc];
+        r.si = sc * (double)Sl[2 * lc + 1];
+        r.tr = sc * (double)Tl[2 * lc];
+        r.ti = sc * (double)Tl[2 * lc + 1];
+        return r;
+    };
+    VCoef r0 = ld(l0), r1 = ld(l0 + 1);
+    int l = l0;
+    for (; l + 2 <= lstop; l += 2) {
+        const VCoef n0 = ld(l + 2), n1 = ld(l + 3);
+        vstep<K, 0, FAST>(r0, x, prev, cur, cm, acc);
+        vstep<K, 1, FAST>(r1, x, prev, cur, cm, acc);
+        if (!FAST && ((l - l0) & (SCALE_CHECK - 1)) == SCALE_CHECK - 2) rescale<K>(prev, cur, kk, cm);
+        r0 = n0;
+        r1 = n1;
+    }
+    if (l < lstop) vstep<K, 0, FAST>(r0, x, prev, cur, cm, acc);
+}
+
+template <int K, typename IT>
+__global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
+    __shared__ double red[4][16][64];
+    static_assert(K == 1, "vector synthesis reduces one chain per lane");
+    const int b = blockIdx.x, im = blockIdx.y, f = blockIdx.z;
+    const int m = im * a.mres;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int S = a.seg, np = a.np, lmax = a.lmax, ltr = a.ltr;
+    const int pb0 = b * 64 * K, pb1 = min(pb0 + 64 * K, np);
+    double x[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int p = pb0 + lane + 64 * j;
+        x[j] = p < np ? a.x[p] : 0.0;
+    }
+    const double2 *__restrict__ ab = a.ab + a.coff[im] - m;
+    const double *__restrict__ gt = a.gtab + a.coff[im] - m;
+    const size_t co = 2 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m);
+    const IT *__restrict__ Sl = (const IT *)a.in0 + co;
+    const IT *__restrict__ Tl = (const IT *)a.in1 + co;
+    double acc[K][2][8];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[j][q][r] = 0.0;
+    const int nseg = nseg_of(lmax, m, S);
+    for (int s = w; s < nseg; s += 4) {
+        const int l0 = m + s * S;
+        if (l0 > ltr) break;
+        const int gs = a.segbase[im] + s;
+        if (pb1 <= a.pfirst[gs]) continue;
+        const bool fast = pb0 >= a.pfast[gs];
+        const int lstop = min(l0 + S, ltr + 1);
+        const double c0 = s == 0 ? 1.0 : ab[l0].x;
+        double prev[K], cur[K];
+        int kk[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int p = pb0 + lane + 64 * j;
+            const bool act = p < np;
+            const size_t q = (size_t)gs * np + (act ? p : 0);
+            prev[j] = act ? a.seedq0[q] : 0.0;
+            cur[j] = act ? a.seedq1[q] * c0 : 0.0;
+            kk[j] = act ? a.seedk[q] : 0;
+        }
+        if (fast) vsynth_seg<K, true>(ab, gt, Sl, Tl, a.ll1_mode, l0, lstop, lmax, x, prev, cur, kk, acc);
+        else vsynth_seg<K, false>(ab, gt, Sl, Tl, a.ll1_mode, l0, lstop, lmax, x, prev, cur, kk, acc);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int r = 0; r < 8; ++r) red[w][8 * q + r][lane] = acc[0][q][r];
+    __syncthreads();
+    if (w != 0) return;
+    const int p = pb0 + lane;
+    if (p >= np) return;
+    double e[8], o[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        e[r] = ((red[0][r][lane] + red[1][r][lane]) + red[2][r][lane]) + red[3][r][lane];
+        o[r] = ((red[0][8 + r][lane] + red[1][8 + r][lane]) + red[2][8 + r][lane]) + red[3][8 + r][lane];
+    }
+    const double mm = (double)m;
+#pragma unroll
+    for (int r = 2; r < 6; ++r) {
+        e[r] *= mm;
+        o[r] *= mm;
+    }
+    const double gm = a.g * mfac_of(a.mfac_mode, m);
+    const double sc = gm * (a.wlat ? a.wlat[p] : 1.0);
+    // [0..1] = A, [2..3] = m B, [4..5] = m C, [6..7] = E
+    const double tNr = e[0] + o[0] + (e[3] + o[3]), tNi = e[1] + o[1] - (e[2] + o[2]);
+    const double tSr = -e[0] + o[0] + (e[3] - o[3]), tSi = -e[1] + o[1] - (e[2] - o[2]);
+    const double pNr = -(e[5] + o[5]) + e[6] + o[6], pNi = e[4] + o[4] + e[7] + o[7];
+    const double pSr = -(e[5] - o[5]) - e[6] + o[6], pSi = (e[4] - o[4]) - e[7] + o[7];
+    const size_t fo = ((size_t)f * a.nm + im) * a.nlat;
+    a.F0[fo + p] = make_double2(tNr * sc, tNi * sc);
+    a.F1[fo + p] = make_double2(pNr * sc, pNi * sc);
+    const int ps = a.nlat - 1 - p;
+    if (ps != p) {
+        a.F0[fo + ps] = make_double2(tSr * sc, tSi * sc);
+        a.F1[fo + ps] = make_double2(pSr * sc, pSi * sc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Analysis work items: (global segment, pair block of 512, field).
+// ---------------------------------------------------------------------------
+struct Item {
+    int gs, im, s, m, l0, lend, lstop, pb0, pb1, pstart, f;
+    bool fast;
+};
+
+template <int PBLK>
+__device__ __forceinline__ Item make_item(const LegArgs &a) {
+    Item it;
+    it.gs = blockIdx.x;
+    it.im = a.segm[it.gs];
+    it.s = it.gs - a.segbase[it.im];
+    it.m = it.im * a.mres;
+    it.l0 = it.m + it.s * a.seg;
+    it.lend = min(it.l0 + a.seg, a.lmax + 1);
+    it.lstop = min(it.lend, a.ltr + 1);
+    it.pb0 = blockIdx.y * PBLK;
+    it.pb1 = min(it.pb0 + PBLK, a.np);
+    it.pstart = max(it.pb0, a.pfirst[it.gs]);
+    it.fast = it.pstart >= a.pfast[it.gs];
+    it.f = blockIdx.z;
+    return it;
+}
+
+// Zero the outputs of an item that no reduction writes (l beyond ltr, l < m,
+// orders between multiples of mres); skipped in accumulate mode (pre-zeroed).
+template <typename OT>
+__device__ void zero_fill(const LegArgs &a, const Item &it, OT *col, bool all_dead, int lane) {
+    if (a.atomic || blockIdx.y != 0) return;
+    const int lz0 = all_dead ? it.l0 : it.lstop;
+    for (int l = lz0 + lane; l < it.lend; l += 64) st_c(col, l, 0.0, 0.0);
+    if (it.s == 0) {
+        for (int l = lane; l < it.m; l += 64) st_c(col, l, 0.0, 0.0);
+        for (int c = 1; c < a.mres && it.m + c <= a.mmax; ++c)
+            for (int l = lane; l <= a.lmax; l += 64) st_c(col + 2 * (size_t)c * (a.lmax + 1), l, 0.0, 0.0);
+    }
+}
+
+// Scalar analysis of one item with K chains per lane.
+template <int K, typename OT>
+__device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, const double2 *__restrict__ qcm,
+                                           double2 *tb, OT *col, int lane) {
+    constexpr int CH = 8;
+    const int np = a.np, nlat = a.nlat;
+    const double2 *Fi = a.F0 + ((size_t)it.f * a.nm + it.im) * nlat;
+    double x[K], prev[K], cur[K], gpr[K], gpi[K], gmr[K], gmi[K], cm[K];
+    int kk[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int p = it.pstart + lane + 64 * j;
+        const bool act = p < it.pb1;
         const int pc = act ? p : 0;
         x[j] = a.x[pc];
-        const size_t sidx = (size_t)im * np + pc;
-        cur[j] = act ? a.seedv[sidx] : 0.0;
-        kk[j] = act ? a.seedk[sidx] : 0;
-        prev[j] = 0.0;
-        c[j] = kk[j] == 0 ? 1.0 : 0.0;
-        live = live && kk[j] == 0;
+        const size_t q = (size_t)it.gs * np + pc;
+        prev[j] = act ? a.seedq0[q] : 0.0;
+        cur[j] = act ? a.seedq1[q] : 0.0;
+        kk[j] = act ? a.seedk[q] : 0;
+        cm[j] = (it.fast || kk[j] == 0) ? 1.0 : 0.0;
         double2 FN = make_double2(0.0, 0.0), FS = make_double2(0.0, 0.0);
         double wl = 0.0;
         if (act) {
@@ -176,37 +482,40 @@ __global__ __launch_bounds__(64 * W) void k_leg_anal(const LegArgs a) {
         gmr[j] = wl * (FN.x - FS.x);
         gmi[j] = wl * (FN.y - FS.y);
     }
-    bool wave_live = __all(live);
-
-    const double2 *__restrict__ ab = a.ab + a.aboff[im] - m;
-    OT *out = (OT *)a.out0 + 2 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m);
-    const double gm = a.g * mfac_of(a.mfac_mode, m);
-
-    // zero the entries of this column (and of skipped columns m+1..m+mres-1) that no chunk writes
-    for (int q = t; q < (lmax + 1) * a.mres; q += NT) {
-        const int col = q / (lmax + 1), l = q % (lmax + 1);
-        if (m + col > a.mmax) continue;
-        if (col == 0 && l >= m && l <= ltr) continue;
-        st_c(out, (size_t)col * (lmax + 1) + l, 0.0, 0.0);
-    }
-
-    for (int l0 = m; l0 <= ltr; l0 += CH) {
-        if (wave_live) {
+    const double gm = a.g * mfac_of(a.mfac_mode, it.m);
+    const int nst = it.lstop - it.l0;
+    const int lmax = a.lmax;
+    // coefficient stream: beta_{l+1} for the recurrence and c_l for the reduced
+    // sums, one chunk prefetched ahead (wave-uniform, scalar loads)
+    auto ldc = [&](int t, double *bt, double *ct) {
 #pragma unroll
-            for (int s = 0; s < CH; ++s) {
-                const int l = l0 + s;
-                if (l <= ltr) {
+        for (int st = 0; st < CH; ++st) {
+            const int l = min(it.l0 + t + st, lmax);
+            bt[st] = qcm[l + 1].x;
+            ct[st] = qcm[l].y;
+        }
+    };
+    double bcur[CH], ccur[CH];
+    ldc(0, bcur, ccur);
+    for (int c0 = 0; c0 < nst; c0 += CH) {
+        double bnx[CH], cnx[CH];
+        ldc(c0 + CH, bnx, cnx);
+        if (it.fast) {
+#pragma unroll
+            for (int st = 0; st < CH; ++st) {
+                const int t = c0 + st;
+                if (t < nst) {
                     double pr = 0.0, pi = 0.0;
 #pragma unroll
                     for (int j = 0; j < K; ++j) {
-                        pr = fma(cur[j], (s & 1) ? gmr[j] : gpr[j], pr);
-                        pi = fma(cur[j], (s & 1) ? gmi[j] : gpi[j], pi);
+                        pr = fma(cur[j], (st & 1) ? gmr[j] : gpr[j], pr);
+                        pi = fma(cur[j], (st & 1) ? gmi[j] : gpi[j], pi);
                     }
-                    buf[s * NT + t] = make_double2(pr, pi);
-                    const double2 co = ab[l + 1];
+                    tb[st * 64 + lane] = make_double2(pr, pi);
+                    const double beta = bcur[st];
 #pragma unroll
                     for (int j = 0; j < K; ++j) {
-                        const double nx = fma(co.x * x[j], cur[j], -(co.y * prev[j]));
+                        const double nx = fma(x[j], cur[j], -(beta * prev[j]));
                         prev[j] = cur[j];
                         cur[j] = nx;
                     }
@@ -214,60 +523,276 @@ __global__ __launch_bounds__(64 * W) void k_leg_anal(const LegArgs a) {
             }
         } else {
 #pragma unroll
-            for (int s = 0; s < CH; ++s) {
-                const int l = l0 + s;
-                if (l <= ltr) {
+            for (int st = 0; st < CH; ++st) {
+                const int t = c0 + st;
+                if (t < nst) {
                     double pr = 0.0, pi = 0.0;
 #pragma unroll
                     for (int j = 0; j < K; ++j) {
-                        const double v = cur[j] * c[j];
-                        pr = fma(v, (s & 1) ? gmr[j] : gpr[j], pr);
-                        pi = fma(v, (s & 1) ? gmi[j] : gpi[j], pi);
+                        const double v = cur[j] * cm[j];
+                        pr = fma(v, (st & 1) ? gmr[j] : gpr[j], pr);
+                        pi = fma(v, (st & 1) ? gmi[j] : gpi[j], pi);
                     }
-                    buf[s * NT + t] = make_double2(pr, pi);
-                    const double2 co = ab[l + 1];
+                    tb[st * 64 + lane] = make_double2(pr, pi);
+                    const double beta = bcur[st];
 #pragma unroll
                     for (int j = 0; j < K; ++j) {
-                        const double nx = fma(co.x * x[j], cur[j], -(co.y * prev[j]));
+                        const double nx = fma(x[j], cur[j], -(beta * prev[j]));
                         prev[j] = cur[j];
                         cur[j] = nx;
                     }
                 }
             }
-            bool all_live = true;
+            const double cl = ccur[min(CH, nst - c0) - 1];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) > kHi) {
+                if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
                     cur[j] *= kDown;
                     prev[j] *= kDown;
                     kk[j] -= 1;
-                    c[j] = kk[j] == 0 ? 1.0 : 0.0;
+                    cm[j] = kk[j] == 0 ? 1.0 : 0.0;
                 }
-                all_live = all_live && kk[j] == 0;
             }
-            wave_live = __all(all_live);
+        }
+        __syncthreads();
+        const int li = lane >> 3, q = lane & 7;
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double2 v = tb[li * 64 + q + 8 * i];
+            sr += v.x;
+            si += v.y;
+        }
+#pragma unroll
+        for (int msk = 4; msk >= 1; msk >>= 1) {
+            sr += __shfl_xor(sr, msk, 8);
+            si += __shfl_xor(si, msk, 8);
+        }
+        const int t = c0 + li;
+        double cli = ccur[0];
+#pragma unroll
+        for (int st = 1; st < CH; ++st) cli = li == st ? ccur[st] : cli;
+        if (q == 0 && t < nst) {
+            const double sc = gm * cli;
+            if (a.atomic) add_c(col, it.l0 + t, sc * sr, sc * si);
+            else st_c(col, it.l0 + t, sc * sr, sc * si);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int st = 0; st < CH; ++st) {
+            bcur[st] = bnx[st];
+            ccur[st] = cnx[st];
+        }
+    }
+}
+
+template <typename OT>
+__global__ __launch_bounds__(64) void k_anal_s(const LegArgs a) {
+    __shared__ double2 tb[8 * 64];
+    const Item it = make_item<512>(a);
+    const int lane = threadIdx.x;
+    OT *col = (OT *)a.out0 + 2 * ((size_t)it.f * a.spec_fstride + (size_t)(a.lmax + 1) * it.m);
+    const bool dead = it.lstop <= it.l0 || it.pstart >= it.pb1;
+    zero_fill(a, it, col, dead, lane);
+    if (dead) return;
+    const double2 *__restrict__ qcm = a.qc + a.coff[it.im] - it.m;
+    const int keff = (it.pb1 - it.pstart + 63) / 64;
+    switch (keff) {
+        case 1: anal_s_run<1>(a, it, qcm, tb, col, lane); break;
+        case 2: anal_s_run<2>(a, it, qcm, tb, col, lane); break;
+        case 3: anal_s_run<3>(a, it, qcm, tb, col, lane); break;
+        case 4: anal_s_run<4>(a, it, qcm, tb, col, lane); break;
+        case 5: anal_s_run<5>(a, it, qcm, tb, col, lane); break;
+        case 6: anal_s_run<6>(a, it, qcm, tb, col, lane); break;
+        case 7: anal_s_run<7>(a, it, qcm, tb, col, lane); break;
+        default: anal_s_run<8>(a, it, qcm, tb, col, lane); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vector analysis: with G = wlat F (wlat = w scale / s),
+//   S_l = sum_i [D_l Gt - i m P_l Gp],  T_l = sum_i [i m P_l Gt + D_l Gp]
+// (times 1/(l(l+1)) for the analysis proper).  Folded per pair: even l-m uses
+// (Gt-, Gp+, Gt+, Gp-) for (SD, SQ, TQ, TD), odd l-m the opposite signs.
+// ---------------------------------------------------------------------------
+template <int K, typename OT>
+__device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, const double2 *__restrict__ qcm,
+                                           const double2 *__restrict__ qdm, double2 *tb, double2 *red, OT *cs,
+                                           OT *ct, int lane) {
+    constexpr int CH = 4;
+    const int np = a.np, nlat = a.nlat;
+    const size_t fo = ((size_t)it.f * a.nm + it.im) * nlat;
+    double x[K], prev[K], cur[K], cm[K];
+    double gt[K][4], gp[K][4];  // (+re, +im, -re, -im)
+    int kk[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int p = it.pstart + lane + 64 * j;
+        const bool act = p < it.pb1;
+        const int pc = act ? p : 0;
+        x[j] = a.x[pc];
+        const size_t q = (size_t)it.gs * np + pc;
+        prev[j] = act ? a.seedq0[q] : 0.0;
+        cur[j] = act ? a.seedq1[q] : 0.0;
+        kk[j] = act ? a.seedk[q] : 0;
+        cm[j] = (it.fast || kk[j] == 0) ? 1.0 : 0.0;
+        double2 tN = make_double2(0.0, 0.0), tS = tN, pN = tN, pS = tN;
+        double wl = 0.0;
+        if (act) {
+            tN = a.F0[fo + p];
+            pN = a.F1[fo + p];
+            const int ps = nlat - 1 - p;
+            if (ps != p) {
+                tS = a.F0[fo + ps];
+                pS = a.F1[fo + ps];
+            }
+            wl = a.wlat ? a.wlat[p] : 1.0;
+        }
+        gt[j][0] = wl * (tN.x + tS.x);
+        gt[j][1] = wl * (tN.y + tS.y);
+        gt[j][2] = wl * (tN.x - tS.x);
+        gt[j][3] = wl * (tN.y - tS.y);
+        gp[j][0] = wl * (pN.x + pS.x);
+        gp[j][1] = wl * (pN.y + pS.y);
+        gp[j][2] = wl * (pN.x - pS.x);
+        gp[j][3] = wl * (pN.y - pS.y);
+    }
+    const double gm = a.g * mfac_of(a.mfac_mode, it.m);
+    const double mm = (double)it.m;
+    const int nst = it.lstop - it.l0;
+    const int lmax = a.lmax;
+    // per degree: beta_{l+1}, l c_l, g_l c_{l-1}, c_l -- scalar loads, one chunk ahead
+    auto ldc = [&](int t0, double4 *r) {
+#pragma unroll
+        for (int st = 0; st < CH; ++st) {
+            const int l = min(it.l0 + t0 + st, lmax);
+            const double2 d = qdm[l];
+            r[st] = make_double4(qcm[l + 1].x, d.x, d.y, qcm[l].y);
+        }
+    };
+    double4 rcur[CH];
+    ldc(0, rcur);
+    for (int c0 = 0; c0 < nst; c0 += CH) {
+        double4 rnx[CH];
+        ldc(c0 + CH, rnx);
+#pragma unroll
+        for (int st = 0; st < CH; ++st) {
+            const int t = c0 + st;
+            if (t < nst) {
+                const double4 r = rcur[st];  // beta_{l+1}, l c_l, g_l c_{l-1}, c_l
+                // even: SD<-Gt-, SQ<-Gp+, TQ<-Gt+, TD<-Gp- ; odd: swap signs
+                const int e = (st & 1) ? 0 : 2, o = (st & 1) ? 2 : 0;
+                double sdr = 0, sdi = 0, sqr = 0, sqi = 0, tqr = 0, tqi = 0, tdr = 0, tdi = 0;
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const double xc = x[j] * cur[j];
+                    const double D = fma(r.y, xc, -(r.z * prev[j])) * cm[j];
+                    const double Q = cur[j] * cm[j];
+                    sdr = fma(D, gt[j][e], sdr);
+                    sdi = fma(D, gt[j][e + 1], sdi);
+                    sqr = fma(Q, gp[j][o], sqr);
+                    sqi = fma(Q, gp[j][o + 1], sqi);
+                    tqr = fma(Q, gt[j][o], tqr);
+                    tqi = fma(Q, gt[j][o + 1], tqi);
+                    tdr = fma(D, gp[j][e], tdr);
+                    tdi = fma(D, gp[j][e + 1], tdi);
+                    const double nx = fma(-r.x, prev[j], xc);
+                    prev[j] = cur[j];
+                    cur[j] = nx;
+                }
+                tb[(st * 4 + 0) * 64 + lane] = make_double2(sdr, sdi);
+                tb[(st * 4 + 1) * 64 + lane] = make_double2(sqr, sqi);
+                tb[(st * 4 + 2) * 64 + lane] = make_double2(tqr, tqi);
+                tb[(st * 4 + 3) * 64 + lane] = make_double2(tdr, tdi);
+            }
+        }
+        if (!it.fast && ((c0 + CH) & (SCALE_CHECK - 1)) == 0) {
+            double cl = rcur[0].w;
+#pragma unroll
+            for (int st = 1; st < CH; ++st) cl = st < nst - c0 ? rcur[st].w : cl;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
+                    cur[j] *= kDown;
+                    prev[j] *= kDown;
+                    kk[j] -= 1;
+                    cm[j] = kk[j] == 0 ? 1.0 : 0.0;
+                }
+            }
         }
         __syncthreads();
         {
-            const int li = t / TPL, q = t % TPL;
+            // 16 (step, quantity) combos x 64 partials: 4 lanes per combo
+            const int combo = lane >> 2, q = lane & 3;
             double sr = 0.0, si = 0.0;
 #pragma unroll
-            for (int jj = 0; jj < CH; ++jj) {
-                const double2 v = buf[li * NT + q + TPL * jj];
+            for (int i = 0; i < 16; ++i) {
+                const double2 v = tb[combo * 64 + q + 4 * i];
                 sr += v.x;
                 si += v.y;
             }
-#pragma unroll
-            for (int msk = TPL / 2; msk >= 1; msk >>= 1) {
-                sr += __shfl_xor(sr, msk, TPL);
-                si += __shfl_xor(si, msk, TPL);
-            }
-            const int l = l0 + li;
-            if (q == 0 && l <= ltr) st_c(out, (size_t)l, gm * sr, gm * si);
+            sr += __shfl_xor(sr, 2, 4);
+            si += __shfl_xor(si, 2, 4);
+            sr += __shfl_xor(sr, 1, 4);
+            si += __shfl_xor(si, 1, 4);
+            if (q == 0) red[combo] = make_double2(sr, si);
         }
         __syncthreads();
+        if (lane < CH) {
+            const int t = c0 + lane;
+            if (t < nst) {
+                const int l = it.l0 + t;
+                double cl = rcur[0].w;
+#pragma unroll
+                for (int st = 1; st < CH; ++st) cl = lane == st ? rcur[st].w : cl;
+                const double2 SD = red[lane * 4 + 0], SQ = red[lane * 4 + 1];
+                const double2 TQ = red[lane * 4 + 2], TD = red[lane * 4 + 3];
+                double sc = gm;
+                if (a.ll1_mode) sc *= l > 0 ? 1.0 / ((double)l * (l + 1)) : 0.0;
+                const double mc = mm * cl;
+                // S = SD - i m c SQ ; T = i m c TQ + TD
+                const double sre = sc * (SD.x + mc * SQ.y), sim = sc * (SD.y - mc * SQ.x);
+                const double tre = sc * (TD.x - mc * TQ.y), tim = sc * (TD.y + mc * TQ.x);
+                if (a.atomic) {
+                    add_c(cs, l, sre, sim);
+                    add_c(ct, l, tre, tim);
+                } else {
+                    st_c(cs, l, sre, sim);
+                    st_c(ct, l, tre, tim);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int st = 0; st < CH; ++st) rcur[st] = rnx[st];
     }
 }
+
+template <typename OT>
+__global__ __launch_bounds__(64) void k_anal_v(const LegArgs a) {
+    __shared__ double2 tb[4 * 4 * 64];
+    __shared__ double2 red[16];
+    const Item it = make_item<256>(a);
+    const int lane = threadIdx.x;
+    const size_t co = 2 * ((size_t)it.f * a.spec_fstride + (size_t)(a.lmax + 1) * it.m);
+    OT *cs = (OT *)a.out0 + co;
+    OT *ct = (OT *)a.out1 + co;
+    const bool dead = it.lstop <= it.l0 || it.pstart >= it.pb1;
+    zero_fill(a, it, cs, dead, lane);
+    zero_fill(a, it, ct, dead, lane);
+    if (dead) return;
+    const double2 *__restrict__ qcm = a.qc + a.coff[it.im] - it.m;
+    const double2 *__restrict__ qdm = a.qd + a.coff[it.im] - it.m;
+    const int keff = (it.pb1 - it.pstart + 63) / 64;
+    switch (keff) {
+        case 1: anal_v_run<1>(a, it, qcm, qdm, tb, red, cs, ct, lane); break;
+        case 2: anal_v_run<2>(a, it, qcm, qdm, tb, red, cs, ct, lane); break;
+        case 3: anal_v_run<3>(a, it, qcm, qdm, tb, red, cs, ct, lane); break;
+        default: anal_v_run<4>(a, it, qcm, qdm, tb, red, cs, ct, lane); break;
+    }
+}
+
+
 
 // ---------------------------------------------------------------------------
 // Spectral conversion: out = in * scale (or in / scale), zero for l < m and
@@ -297,39 +822,53 @@ __global__ void k_spec_scale(const IT *in, OT *out, const double *scale, int div
     out[2 * q + 1] = (OT)im;
 }
 
-template <int K, int W>
-hipError_t launch_anal_kw(const LegArgs &a, int nfields, hipStream_t st) {
-    dim3 grid(a.nm, nfields);
-    if (a.io_f32)
-        hipLaunchKernelGGL((k_leg_anal<K, W, float>), grid, dim3(64 * W), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_leg_anal<K, W, double>), grid, dim3(64 * W), 0, st, a);
-    return hipGetLastError();
+__global__ void k_zero(unsigned long long *p, size_t n8) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n8) p[q] = 0ull;
 }
 
 }  // namespace
 
+hipError_t launch_seed(const SeedArgs &a, hipStream_t st) {
+    dim3 grid((a.np + 255) / 256, a.nm);
+    hipLaunchKernelGGL(k_seed, grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_leg_synth(const LegArgs &a, int nfields, int vector, hipStream_t st) {
-    if (vector) return hipErrorNotSupported;
-    dim3 grid((a.np + 255) / 256, a.nm, nfields);
-    if (a.io_f32)
-        hipLaunchKernelGGL(k_leg_synth<float>, grid, dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL(k_leg_synth<double>, grid, dim3(256), 0, st, a);
+    if (a.seg > SEG_MAX || (a.seg & 7)) return hipErrorInvalidValue;
+    constexpr int K = 1;
+    dim3 grid((a.np + 64 * K - 1) / (64 * K), a.nm, nfields);
+    if (vector) {
+        if (a.io_f32)
+            hipLaunchKernelGGL((k_synth_v<K, float>), grid, dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_synth_v<K, double>), grid, dim3(256), 0, st, a);
+    } else {
+        if (a.io_f32)
+            hipLaunchKernelGGL((k_synth_s<K, float>), grid, dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_synth_s<K, double>), grid, dim3(256), 0, st, a);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_leg_anal(const LegArgs &a, int nfields, int vector, hipStream_t st) {
-    if (vector) return hipErrorNotSupported;
-    // chains per lane K and waves per workgroup W cover all latitude pairs
-    const int np = a.np;
-    if (np <= 64) return launch_anal_kw<1, 1>(a, nfields, st);
-    if (np <= 128) return launch_anal_kw<2, 1>(a, nfields, st);
-    if (np <= 256) return launch_anal_kw<2, 2>(a, nfields, st);
-    if (np <= 512) return launch_anal_kw<4, 2>(a, nfields, st);
-    if (np <= 1024) return launch_anal_kw<4, 4>(a, nfields, st);
-    if (np <= 2048) return launch_anal_kw<8, 4>(a, nfields, st);
-    return hipErrorNotSupported;
+    if (a.seg > SEG_MAX || (a.seg & 7)) return hipErrorInvalidValue;
+    const int pblk = vector ? 256 : 512;
+    dim3 grid(a.nseg_total, (a.np + pblk - 1) / pblk, nfields);
+    if (vector) {
+        if (a.io_f32)
+            hipLaunchKernelGGL(k_anal_v<float>, grid, dim3(64), 0, st, a);
+        else
+            hipLaunchKernelGGL(k_anal_v<double>, grid, dim3(64), 0, st, a);
+    } else {
+        if (a.io_f32)
+            hipLaunchKernelGGL(k_anal_s<float>, grid, dim3(64), 0, st, a);
+        else
+            hipLaunchKernelGGL(k_anal_s<double>, grid, dim3(64), 0, st, a);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_spec_scale(const void *in, void *out, int in_f32, int out_f32, const double *scale, int divide,
@@ -348,6 +887,13 @@ hipError_t launch_spec_scale(const void *in, void *out, int in_f32, int out_f32,
     else
         hipLaunchKernelGGL((k_spec_scale<double, double>), grid, dim3(256), 0, st, (const double *)in,
                            (double *)out, scale, divide, lmax, mmax, mres, per_field, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t st) {
+    const size_t n8 = bytes / 8;
+    dim3 grid((unsigned)((n8 + 255) / 256));
+    hipLaunchKernelGGL(k_zero, grid, dim3(256), 0, st, (unsigned long long *)p, n8);
     return hipGetLastError();
 }
 

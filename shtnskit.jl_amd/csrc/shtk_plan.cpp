@@ -2,14 +2,16 @@
 //
 // A plan is the device-resident counterpart of SHTnsKit's SHTConfig
 // (src/config.jl:170-206, create_gauss_config :633-655) for one Gauss grid:
-// nodes and weights, the FP64 recurrence coefficients of every order, the
-// sectoral seeds with their scale counts, the coefficient convention map
-// (src/normalization.jl:142-153) and the FFT twiddles.  Transforms enqueue the
-// kernels of shtk_legendre.hip and shtk_fft.hip on the caller's stream.
+// nodes and weights, the FP64 recurrence coefficients of every order in the
+// segmented Q form, per-segment seeds (computed once by a device kernel), the
+// coefficient convention map (src/normalization.jl:142-153) and the FFT
+// twiddles.  Transforms enqueue the kernels of shtk_legendre.hip and
+// shtk_fft.hip on the caller's stream.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -33,10 +35,16 @@ int hip_fail(hipError_t e, const char *what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr, what)                          \
-    do {                                             \
-        hipError_t e_ = (expr);                      \
+#define HIP_TRY(expr, what)                              \
+    do {                                                 \
+        hipError_t e_ = (expr);                          \
         if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+#define RC_TRY(expr)             \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_) return rc_;     \
     } while (0)
 
 template <typename T>
@@ -103,14 +111,15 @@ std::vector<int> factor_radix(int n) {
             r.push_back(c);
             n /= c;
         }
-    if (n != 1) r.clear();
+    if (n != 1 || r.size() > 16) r.clear();
     return r;
 }
 
 void free_plan(shtk_plan *p) {
     if (!p) return;
-    void *ptrs[] = {p->x, p->s, p->w, p->wscaled, p->wadj, p->ab, p->aboff, p->dd, p->seedv, p->seedk,
-                    p->convscale, p->ll1inv, p->tw, p->ws};
+    void *ptrs[] = {p->x,      p->s,      p->wscaled, p->winvs,  p->winvs2, p->invs,  p->qc,  p->ab, p->gtab,
+                    p->qd,     p->coff,   p->segbase, p->seedq0, p->seedq1, p->seedk, p->pfirst,
+                    p->pfast,  p->segm,   p->convscale, p->tw,   p->ws};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -169,6 +178,29 @@ int ensure_ws(shtk_plan *p, size_t bytes) {
     return SHTK_OK;
 }
 
+hipEvent_t pool_event(shtk_plan *p) {
+    if (!p->event_pool.empty()) {
+        hipEvent_t e = p->event_pool.back();
+        p->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Runs one kernel launcher; with profiling on, brackets it with events.
+template <typename F>
+hipError_t timed(shtk_plan *p, const char *name, hipStream_t st, F &&launch) {
+    if (!p->profiling) return launch();
+    hipEvent_t a = pool_event(p), b = pool_event(p);
+    if (a) (void)hipEventRecord(a, st);
+    hipError_t e = launch();
+    if (b) (void)hipEventRecord(b, st);
+    if (a && b) p->pending.push_back({name, a, b});
+    return e;
+}
+
 shtk::LegArgs base_args(const shtk_plan *p, int ltr) {
     shtk::LegArgs a{};
     a.lmax = p->d.lmax;
@@ -178,14 +210,21 @@ shtk::LegArgs base_args(const shtk_plan *p, int ltr) {
     a.np = p->np;
     a.nm = p->nm;
     a.ltr = ltr;
+    a.seg = p->seg;
     a.x = p->x;
-    a.s = p->s;
+    a.qc = p->qc;
+    a.qd = p->qd;
     a.ab = p->ab;
-    a.aboff = p->aboff;
-    a.seedv = p->seedv;
+    a.gtab = p->gtab;
+    a.coff = p->coff;
+    a.segbase = p->segbase;
+    a.segm = p->segm;
+    a.nseg_total = p->nseg_total;
+    a.seedq0 = p->seedq0;
+    a.seedq1 = p->seedq1;
     a.seedk = p->seedk;
-    a.dd = p->dd;
-    a.ll1inv = p->ll1inv;
+    a.pfirst = p->pfirst;
+    a.pfast = p->pfast;
     a.spec_fstride = (long long)(p->d.lmax + 1) * (p->d.mmax + 1);
     return a;
 }
@@ -199,7 +238,8 @@ shtk::FftArgs fft_args(const shtk_plan *p) {
     f.mres = p->d.mres;
     f.mmax = p->d.mmax;
     f.nrad = (int)p->radix.size();
-    for (size_t q = 0; q < p->radix.size() && q < 24; ++q) f.radix[q] = p->radix[q];
+    f.radix = 0;
+    for (size_t q = 0; q < p->radix.size(); ++q) f.radix |= (unsigned long long)p->radix[q] << (4 * q);
     f.tw = p->tw;
     f.s = p->s;
     return f;
@@ -224,16 +264,132 @@ int norm_ltr(const shtk_plan *p, int ltr, int *out) {
     return SHTK_OK;
 }
 
+// One transform direction with its scalings (see the entry points below).
+struct Job {
+    int vector;     // 0 scalar, 1 spheroidal/toroidal pair
+    int conv_div;   // non-canonical plans: spectral data divided (1) or multiplied (0) by the convention map
+    const double *wlat;
+    double g;
+    int mfac_mode, ll1_mode, fft_mode, ltr;
+    const char *leg_name, *fft_name;
+};
+
+// spectral -> grid: [convert] -> Legendre synthesis -> inverse phi FFT
+int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void *g0, void *g1, int nf, int dtype,
+                void *stream) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const WsLayout L = ws_layout(p, nf, j.vector);
+    RC_TRY(ensure_ws(p, L.total));
+    char *ws = (char *)p->ws;
+    double2 *F0 = (double2 *)ws, *F1 = (double2 *)(ws + L.f_bytes);
+    char *spec = ws + (j.vector ? 2 : 1) * L.f_bytes;
+    shtk::LegArgs a = base_args(p, j.ltr);
+    a.in0 = s0;
+    a.in1 = s1;
+    a.io_f32 = dtype == SHTK_F32;
+    if (!p->canonical) {
+        const void *src[2] = {s0, s1};
+        for (int q = 0; q <= j.vector; ++q) {
+            void *dst = spec + q * L.spec_bytes;
+            HIP_TRY(shtk::launch_spec_scale(src[q], dst, a.io_f32, 0, p->convscale, j.conv_div, p->d.lmax,
+                                            p->d.mmax, p->d.mres, a.spec_fstride, nf, st),
+                    "coefficient conversion");
+            (q == 0 ? a.in0 : a.in1) = dst;
+        }
+        a.io_f32 = 0;
+    }
+    a.F0 = F0;
+    a.F1 = F1;
+    a.wlat = j.wlat;
+    a.g = j.g;
+    a.mfac_mode = j.mfac_mode;
+    a.ll1_mode = j.ll1_mode;
+    HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_leg_synth(a, nf, j.vector, st); }),
+            "Legendre synthesis launch");
+    void *grids[2] = {g0, g1};
+    for (int q = 0; q <= j.vector; ++q) {
+        shtk::FftArgs fa = fft_args(p);
+        fa.F = q == 0 ? F0 : F1;
+        fa.grid = grids[q];
+        fa.io_f32 = dtype == SHTK_F32;
+        fa.mode = j.fft_mode;
+        HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 1, st); }), "phi synthesis launch");
+    }
+    return SHTK_OK;
+}
+
+// grid -> spectral: forward phi FFT -> Legendre analysis -> [convert]
+int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void *s0, void *s1, int nf, int dtype,
+                void *stream) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const WsLayout L = ws_layout(p, nf, j.vector);
+    RC_TRY(ensure_ws(p, L.total));
+    char *ws = (char *)p->ws;
+    double2 *F0 = (double2 *)ws, *F1 = (double2 *)(ws + L.f_bytes);
+    char *spec = ws + (j.vector ? 2 : 1) * L.f_bytes;
+    const void *grids[2] = {g0, g1};
+    for (int q = 0; q <= j.vector; ++q) {
+        shtk::FftArgs fa = fft_args(p);
+        fa.F = q == 0 ? F0 : F1;
+        fa.grid = (void *)grids[q];
+        fa.io_f32 = dtype == SHTK_F32;
+        fa.mode = j.fft_mode;
+        HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 0, st); }), "phi analysis launch");
+    }
+    shtk::LegArgs a = base_args(p, j.ltr);
+    a.F0 = F0;
+    a.F1 = F1;
+    a.wlat = j.wlat;
+    a.g = j.g;
+    a.mfac_mode = j.mfac_mode;
+    a.ll1_mode = j.ll1_mode;
+    a.out0 = s0;
+    a.out1 = s1;
+    a.io_f32 = dtype == SHTK_F32;
+    if (!p->canonical) {
+        a.out0 = spec;
+        a.out1 = spec + L.spec_bytes;
+        a.io_f32 = 0;
+    }
+    const int pblk = j.vector ? 256 : 512;
+    a.atomic = p->np > pblk;
+    if (a.atomic) {
+        const size_t bytes = (size_t)nf * a.spec_fstride * (a.io_f32 ? 8 : 16);
+        void *outs[2] = {a.out0, a.out1};
+        for (int q = 0; q <= j.vector; ++q)
+            HIP_TRY(shtk::launch_zero(outs[q], bytes, st), "output clear");
+    }
+    HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_leg_anal(a, nf, j.vector, st); }),
+            "Legendre analysis launch");
+    if (!p->canonical) {
+        void *dst[2] = {s0, s1};
+        for (int q = 0; q <= j.vector; ++q)
+            HIP_TRY(shtk::launch_spec_scale(spec + q * L.spec_bytes, dst[q], 0, dtype == SHTK_F32, p->convscale,
+                                            j.conv_div, p->d.lmax, p->d.mmax, p->d.mres, a.spec_fstride, nf, st),
+                    "coefficient conversion");
+    }
+    return SHTK_OK;
+}
+
+int segment_length() {
+    const char *e = std::getenv("SHTK_SEG");
+    int s = e ? std::atoi(e) : 128;
+    if (s < 8 || s > shtk::SEG_MAX || (s & 7)) s = 128;
+    return s;
+}
+
 }  // namespace
 
 extern "C" {
 
-const char *shtk_version(void) { return "0.1.0"; }
+const char *shtk_version(void) { return "0.2.0"; }
 
 int shtk_last_error(char *buf, size_t len) {
-    if (buf && len) {
-        std::snprintf(buf, len, "%s", g_err.c_str());
-    }
+    if (buf && len) std::snprintf(buf, len, "%s", g_err.c_str());
     return g_errcode;
 }
 
@@ -242,72 +398,73 @@ int shtk_stream_sync(void *stream) {
     return SHTK_OK;
 }
 
-int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
-    if (!out) return fail(SHTK_EINVAL, "plan output pointer is NULL");
-    *out = nullptr;
-    int rc = validate(desc);
-    if (rc) return rc;
-    shtk_plan *p = new (std::nothrow) shtk_plan();
-    if (!p) return fail(SHTK_ENOMEM, "host allocation failed");
-    p->d = *desc;
+static int plan_tables(shtk_plan *p) {
     const shtk_desc &d = p->d;
-    HIP_TRY(hipGetDevice(&p->device), "hipGetDevice");
-    p->np = (d.nlat + 1) / 2;
-    p->nm = d.mmax / d.mres + 1;
-    p->canonical = d.norm == SHTK_NORM_ORTHONORMAL && !d.real_norm && d.cs_phase;
-    p->inv_scale = d.phi_scale == SHTK_PHI_QUAD ? d.nlon / (2.0 * M_PI) : (double)d.nlon;
-    p->scale_phi = (2.0 * M_PI / d.nlon) * d.nlon / p->inv_scale;
-
-    std::vector<double> xa, wa;
-    gauss_nodes(d.nlat, xa, wa);
-    p->hx.assign(xa.rbegin(), xa.rend());  // north first (src/config.jl:639)
-    p->hw.assign(wa.rbegin(), wa.rend());
-
-    const int np = p->np, nm = p->nm, lmax = d.lmax;
-    std::vector<double> x(np), s(np), w(np), ws(np), wadj(np);
+    const int np = p->np, nm = p->nm, lmax = d.lmax, S = p->seg;
+    std::vector<double> x(np), s(np), ws(np), winvs(np), winvs2(np), invs(np);
     for (int q = 0; q < np; ++q) {
         x[q] = p->hx[q];
         const double v = (1.0 - x[q]) * (1.0 + x[q]);
         s[q] = std::sqrt(v > 0 ? v : 0.0);
-        w[q] = p->hw[q];
-        ws[q] = w[q] * p->scale_phi;
-        wadj[q] = w[q] * p->scale_phi;
+        const double w = p->hw[q];
+        ws[q] = w * p->scale_phi;
+        invs[q] = s[q] > 0 ? 1.0 / s[q] : 0.0;
+        winvs[q] = ws[q] * invs[q];
+        winvs2[q] = winvs[q] * invs[q];
     }
 
-    // Recurrence coefficients per order, l = m .. lmax+1 (src/legendre.jl:242, 277-278).
-    std::vector<double2> ab;
-    std::vector<double2> dd;
-    std::vector<long long> aboff(nm);
+    // Per-order coefficients for l = m .. lmax+1: P-form (a_l, b_l) as
+    // src/legendre.jl:242, 277-278 (seed kernel), and the segmented Q form.
+    std::vector<double2> ab, qc, qd;
+    std::vector<double> gt;
+    std::vector<long long> coff(nm);
+    std::vector<int> segbase(nm), segm;
+    int nseg_total = 0;
     for (int im = 0; im < nm; ++im) {
         const int m = im * d.mres;
-        aboff[im] = (long long)ab.size();
+        coff[im] = (long long)ab.size();
+        segbase[im] = nseg_total;
+        const int nseg = (lmax - m + S) / S;
+        for (int q = 0; q < nseg; ++q) segm.push_back(im);
+        nseg_total += nseg;
+        std::vector<double> a(lmax + 2 - m + 1, 0.0), b(lmax + 2 - m + 1, 0.0);
         for (int l = m; l <= lmax + 1; ++l) {
-            double a = 0.0, b = 0.0;
+            double al = 0.0, bl = 0.0;
             if (l == m + 1) {
-                a = std::sqrt(2.0 * m + 3.0);
+                al = std::sqrt(2.0 * m + 3.0);
             } else if (l >= m + 2) {
-                a = std::sqrt(((2.0 * l - 1.0) * (2.0 * l + 1.0)) / ((double)(l - m) * (double)(l + m)));
-                b = std::sqrt(((2.0 * l + 1.0) * (double)(l - 1 - m) * (double)(l - 1 + m)) /
-                              ((2.0 * l - 3.0) * (double)(l - m) * (double)(l + m)));
+                al = std::sqrt(((2.0 * l - 1.0) * (2.0 * l + 1.0)) / ((double)(l - m) * (double)(l + m)));
+                bl = std::sqrt(((2.0 * l + 1.0) * (double)(l - 1 - m) * (double)(l - 1 + m)) /
+                               ((2.0 * l - 3.0) * (double)(l - m) * (double)(l + m)));
             }
-            ab.push_back(make_double2(a, b));
-            // sin(theta) dP_l/dtheta = l d_{l+1,m} P_{l+1} - (l+1) d_{l,m} P_{l-1}  (src/legendre.jl:542-548)
-            const double v1 = ((double)(l + 1) * (l + 1) - (double)m * m) / (4.0 * (l + 1) * (l + 1) - 1.0);
-            const double dl1 = std::sqrt(v1 > 0 ? v1 : 0.0);
-            double dl = 0.0;
-            if (!(l == 0 || l == m)) {
-                const double v = ((double)l * l - (double)m * m) / (4.0 * l * l - 1.0);
-                dl = std::sqrt(v > 0 ? v : 0.0);
-            }
-            dd.push_back(make_double2((double)l * dl1, (double)(l + 1) * dl));
+            a[l - m] = al;
+            b[l - m] = bl;
+            ab.push_back(make_double2(al, bl));
+        }
+        double c = 1.0;
+        for (int l = m; l <= lmax + 1; ++l) {
+            const int t = l - m;
+            // segment base: m for the first segment, l0-1 for the others
+            if (l == m) c = 1.0;
+            else if (t % S == 0) c = a[t];  // l = l0 of a new segment: c_{l0-1} = 1, c_{l0} = a_{l0}
+            else c *= a[t];
+            const double beta = (l >= m + 2) ? b[t] / (a[t] * a[t - 1]) : 0.0;
+            qc.push_back(make_double2(beta, c));
+            const double cprev = (l == m) ? 0.0 : c / a[t];
+            const double gl = (l > 0 && l > m)
+                                  ? std::sqrt((2.0 * l + 1.0) * ((double)l * l - (double)m * m) / (2.0 * l - 1.0))
+                                  : 0.0;
+            qd.push_back(make_double2((double)l * c, gl * cprev));
+            gt.push_back(gl);
         }
     }
+    p->nseg_total = nseg_total;
 
     // Sectoral seeds P_m^m(x_p) = (-1)^m sqrt(1/4pi) prod_k sqrt((2k+1)/(2k)) sin(theta)
-    // (src/legendre.jl:252-271), carried as mantissa * 2^e and stored in the
-    // scaled representation of the kernels.
-    std::vector<double> seedv((size_t)nm * np);
-    std::vector<int> seedk((size_t)nm * np);
+    // (src/legendre.jl:252-271), carried as mantissa * 2^e (exact rescaling) and
+    // stored in the scaled representation of the kernels.
+    std::vector<double> sectv((size_t)nm * np);
+    std::vector<int> sectk((size_t)nm * np);
     for (int q = 0; q < np; ++q) {
         double mant = 0.28209479177387814;
         int e = 0;
@@ -321,15 +478,15 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
             if (m % d.mres) continue;
             const size_t idx = (size_t)(m / d.mres) * np + q;
             if (mant == 0.0) {
-                seedv[idx] = 0.0;
-                seedk[idx] = 0;
+                sectv[idx] = 0.0;
+                sectk[idx] = 0;
             } else if (e > shtk::SCALE_LIVE_MIN) {
-                seedv[idx] = std::ldexp(mant, e);
-                seedk[idx] = 0;
+                sectv[idx] = std::ldexp(mant, e);
+                sectk[idx] = 0;
             } else {
                 const int k = (shtk::SCALE_LIVE_MIN - e) / shtk::SCALE_SHIFT + 1;
-                seedv[idx] = std::ldexp(mant, e + k * shtk::SCALE_SHIFT);
-                seedk[idx] = k;
+                sectv[idx] = std::ldexp(mant, e + k * shtk::SCALE_SHIFT);
+                sectk[idx] = k;
             }
         }
     }
@@ -340,9 +497,6 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
         for (int m = 0; m <= d.mmax; ++m)
             for (int l = m; l <= lmax; ++l) conv[(size_t)l + (size_t)(lmax + 1) * m] = coef_scale(d, l, m);
     }
-    std::vector<double> ll1(lmax + 1, 0.0);
-    for (int l = 1; l <= lmax; ++l) ll1[l] = 1.0 / ((double)l * (l + 1));
-
     std::vector<double2> tw(d.nlon);
     for (int t = 0; t < d.nlon; ++t) {
         const long double ang = 2.0L * 3.141592653589793238462643383279502884L * t / d.nlon;
@@ -350,13 +504,86 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     }
     p->radix = factor_radix(d.nlon);
 
-    if ((rc = upload(&p->x, x, "plan tables")) || (rc = upload(&p->s, s, "plan tables")) ||
-        (rc = upload(&p->w, w, "plan tables")) || (rc = upload(&p->wscaled, ws, "plan tables")) ||
-        (rc = upload(&p->wadj, wadj, "plan tables")) || (rc = upload(&p->ab, ab, "plan tables")) ||
-        (rc = upload(&p->aboff, aboff, "plan tables")) || (rc = upload(&p->dd, dd, "plan tables")) ||
-        (rc = upload(&p->seedv, seedv, "plan tables")) || (rc = upload(&p->seedk, seedk, "plan tables")) ||
-        (rc = upload(&p->convscale, conv, "plan tables")) || (rc = upload(&p->ll1inv, ll1, "plan tables")) ||
-        (rc = upload(&p->tw, tw, "plan tables"))) {
+    RC_TRY(upload(&p->x, x, "plan tables"));
+    RC_TRY(upload(&p->s, s, "plan tables"));
+    RC_TRY(upload(&p->wscaled, ws, "plan tables"));
+    RC_TRY(upload(&p->winvs, winvs, "plan tables"));
+    RC_TRY(upload(&p->winvs2, winvs2, "plan tables"));
+    RC_TRY(upload(&p->invs, invs, "plan tables"));
+    RC_TRY(upload(&p->qc, qc, "plan tables"));
+    RC_TRY(upload(&p->qd, qd, "plan tables"));
+    RC_TRY(upload(&p->ab, ab, "plan tables"));
+    RC_TRY(upload(&p->gtab, gt, "plan tables"));
+    RC_TRY(upload(&p->coff, coff, "plan tables"));
+    RC_TRY(upload(&p->segbase, segbase, "plan tables"));
+    RC_TRY(upload(&p->segm, segm, "plan tables"));
+    RC_TRY(upload(&p->convscale, conv, "plan tables"));
+    RC_TRY(upload(&p->tw, tw, "plan tables"));
+
+    // Segment seeds, computed once on the device.
+    const size_t nseed = (size_t)nseg_total * np;
+    HIP_TRY(hipMalloc((void **)&p->seedq0, sizeof(double) * nseed), "seed tables");
+    HIP_TRY(hipMalloc((void **)&p->seedq1, sizeof(double) * nseed), "seed tables");
+    HIP_TRY(hipMalloc((void **)&p->seedk, sizeof(int) * nseed), "seed tables");
+    RC_TRY(upload(&p->pfirst, std::vector<int>(nseg_total, np), "seed tables"));
+    RC_TRY(upload(&p->pfast, std::vector<int>(nseg_total, 0), "seed tables"));
+    double *dsectv = nullptr;
+    int *dsectk = nullptr;
+    int rc = upload(&dsectv, sectv, "seed inputs");
+    if (!rc) rc = upload(&dsectk, sectk, "seed inputs");
+    if (!rc) {
+        shtk::SeedArgs sa{};
+        sa.lmax = lmax;
+        sa.mmax = d.mmax;
+        sa.mres = d.mres;
+        sa.np = np;
+        sa.nm = nm;
+        sa.seg = S;
+        sa.x = p->x;
+        sa.ab = p->ab;
+        sa.coff = p->coff;
+        sa.segbase = p->segbase;
+        sa.sectv = dsectv;
+        sa.sectk = dsectk;
+        sa.seedq0 = p->seedq0;
+        sa.seedq1 = p->seedq1;
+        sa.seedk = p->seedk;
+        sa.pfirst = p->pfirst;
+        sa.pfast = p->pfast;
+        hipError_t e = shtk::launch_seed(sa, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = hip_fail(e, "seed kernel");
+    }
+    if (dsectv) (void)hipFree(dsectv);
+    if (dsectk) (void)hipFree(dsectk);
+    return rc;
+}
+
+int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
+    if (!out) return fail(SHTK_EINVAL, "plan output pointer is NULL");
+    *out = nullptr;
+    int rc = validate(desc);
+    if (rc) return rc;
+    shtk_plan *p = new (std::nothrow) shtk_plan();
+    if (!p) return fail(SHTK_ENOMEM, "host allocation failed");
+    p->d = *desc;
+    const shtk_desc &d = p->d;
+    if (hipGetDevice(&p->device) != hipSuccess) {
+        delete p;
+        return fail(SHTK_EHIP, "hipGetDevice failed (no ROCm device)");
+    }
+    p->np = (d.nlat + 1) / 2;
+    p->nm = d.mmax / d.mres + 1;
+    p->seg = segment_length();
+    p->canonical = d.norm == SHTK_NORM_ORTHONORMAL && !d.real_norm && d.cs_phase;
+    p->inv_scale = d.phi_scale == SHTK_PHI_QUAD ? d.nlon / (2.0 * M_PI) : (double)d.nlon;
+    p->scale_phi = (2.0 * M_PI / d.nlon) * d.nlon / p->inv_scale;
+    std::vector<double> xa, wa;
+    gauss_nodes(d.nlat, xa, wa);
+    p->hx.assign(xa.rbegin(), xa.rend());  // north first (src/config.jl:639)
+    p->hw.assign(wa.rbegin(), wa.rend());
+    rc = plan_tables(p);
+    if (rc) {
         free_plan(p);
         return rc;
     }
@@ -364,10 +591,55 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     return SHTK_OK;
 }
 
+int shtk_profile_enable(shtk_plan *plan, int on) {
+    if (!plan) return fail(SHTK_EINVAL, "plan is NULL");
+    std::lock_guard<std::mutex> lk(plan->mu);
+    plan->profiling = on != 0;
+    return SHTK_OK;
+}
+
+int shtk_profile_read(shtk_plan *plan, int max, char *names, double *total_ms, int *count, int *n_out) {
+    if (!plan) return fail(SHTK_EINVAL, "plan is NULL");
+    std::lock_guard<std::mutex> lk(plan->mu);
+    DeviceGuard g(plan->device);
+    for (auto &pe : plan->pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventSynchronize(pe.stop), "profile event wait");
+        HIP_TRY(hipEventElapsedTime(&ms, pe.start, pe.stop), "profile event time");
+        bool found = false;
+        for (auto &t : plan->prof_totals)
+            if (t.first == pe.name) {
+                t.second.first += ms;
+                t.second.second += 1;
+                found = true;
+            }
+        if (!found) plan->prof_totals.push_back({pe.name, {(double)ms, 1}});
+        plan->event_pool.push_back(pe.start);
+        plan->event_pool.push_back(pe.stop);
+    }
+    plan->pending.clear();
+    int n = 0;
+    for (auto &t : plan->prof_totals) {
+        if (n >= max) break;
+        if (names) std::snprintf(names + 48 * n, 48, "%s", t.first.c_str());
+        if (total_ms) total_ms[n] = t.second.first;
+        if (count) count[n] = t.second.second;
+        ++n;
+    }
+    if (n_out) *n_out = n;
+    plan->prof_totals.clear();
+    return SHTK_OK;
+}
+
 int shtk_plan_destroy(shtk_plan *plan) {
     if (!plan) return SHTK_OK;
     DeviceGuard g(plan->device);
-    if (plan->ws) (void)hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
+    for (auto &pe : plan->pending) {
+        (void)hipEventDestroy(pe.start);
+        (void)hipEventDestroy(pe.stop);
+    }
+    for (hipEvent_t e : plan->event_pool) (void)hipEventDestroy(e);
     free_plan(plan);
     return SHTK_OK;
 }
@@ -383,188 +655,78 @@ int shtk_plan_nodes(const shtk_plan *plan, double *x, double *w) {
 
 size_t shtk_workspace_size(const shtk_plan *plan, int nfields, int vector) {
     if (!plan || nfields < 1) return 0;
-    return ws_layout(plan, nfields, vector).total;
+    return ws_layout(plan, nfields, vector ? 1 : 0).total;
 }
 
+// --- scalar ------------------------------------------------------------------
 int shtk_synthesis(shtk_plan *p, const void *alm, void *f, int nfields, int dtype, int real_output, int ltr,
                    void *stream) {
-    int rc = check_common(p, nfields, dtype, {alm, f});
-    if (rc) return rc;
+    RC_TRY(check_common(p, nfields, dtype, {alm, f}));
     if (real_output != 1)
         return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is not implemented by this backend");
-    if ((rc = norm_ltr(p, ltr, &ltr))) return rc;
-    std::lock_guard<std::mutex> lk(p->mu);
-    DeviceGuard g(p->device);
-    hipStream_t st = (hipStream_t)stream;
-    const WsLayout L = ws_layout(p, nfields, 0);
-    if ((rc = ensure_ws(p, L.total))) return rc;
-    char *ws = (char *)p->ws;
-    double2 *F0 = (double2 *)ws;
-    shtk::LegArgs a = base_args(p, ltr);
-    a.in0 = alm;
-    a.io_f32 = dtype == SHTK_F32;
-    if (!p->canonical) {
-        void *spec = ws + L.f_bytes;
-        HIP_TRY(shtk::launch_spec_scale(alm, spec, a.io_f32, 0, p->convscale, 0, p->d.lmax, p->d.mmax, p->d.mres,
-                                        a.spec_fstride, nfields, st),
-                "coefficient conversion");
-        a.in0 = spec;
-        a.io_f32 = 0;
-    }
-    a.F0 = F0;
-    a.g = p->inv_scale / p->d.nlon;
-    a.mfac_mode = 0;
-    HIP_TRY(shtk::launch_leg_synth(a, nfields, 0, st), "Legendre synthesis launch");
-    shtk::FftArgs fa = fft_args(p);
-    fa.F = F0;
-    fa.grid = f;
-    fa.io_f32 = dtype == SHTK_F32;
-    fa.mode = 0;
-    HIP_TRY(shtk::launch_fft(fa, nfields, 1, st), "phi synthesis launch");
-    return SHTK_OK;
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    // f = Re ifft(phi_inv_scale * F): the 1/nlon of the inverse FFT is folded in g
+    Job j{0, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, 0, ltr, "leg_synth", "fft_synth"};
+    return run_to_grid(p, j, alm, nullptr, f, nullptr, nfields, dtype, stream);
 }
 
 int shtk_analysis(shtk_plan *p, const void *f, void *alm, int nfields, int dtype, int ltr, void *stream) {
-    int rc = check_common(p, nfields, dtype, {alm, f});
-    if (rc) return rc;
-    if ((rc = norm_ltr(p, ltr, &ltr))) return rc;
-    std::lock_guard<std::mutex> lk(p->mu);
-    DeviceGuard g(p->device);
-    hipStream_t st = (hipStream_t)stream;
-    const WsLayout L = ws_layout(p, nfields, 0);
-    if ((rc = ensure_ws(p, L.total))) return rc;
-    char *ws = (char *)p->ws;
-    double2 *F0 = (double2 *)ws;
-    shtk::FftArgs fa = fft_args(p);
-    fa.F = F0;
-    fa.grid = (void *)f;
-    fa.io_f32 = dtype == SHTK_F32;
-    fa.mode = 0;
-    HIP_TRY(shtk::launch_fft(fa, nfields, 0, st), "phi analysis launch");
-    shtk::LegArgs a = base_args(p, ltr);
-    a.F0 = F0;
-    a.wlat = p->wscaled;
-    a.g = 1.0;
-    a.mfac_mode = 0;
-    a.out0 = alm;
-    a.io_f32 = dtype == SHTK_F32;
-    void *spec = ws + L.f_bytes;
-    if (!p->canonical) {
-        a.out0 = spec;
-        a.io_f32 = 0;
-    }
-    HIP_TRY(shtk::launch_leg_anal(a, nfields, 0, st), "Legendre analysis launch");
-    if (!p->canonical)
-        HIP_TRY(shtk::launch_spec_scale(spec, alm, 0, dtype == SHTK_F32, p->convscale, 1, p->d.lmax, p->d.mmax,
-                                        p->d.mres, a.spec_fstride, nfields, st),
-                "coefficient conversion");
-    return SHTK_OK;
+    RC_TRY(check_common(p, nfields, dtype, {alm, f}));
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    Job j{0, 1, p->wscaled, 1.0, 0, 0, 0, ltr, "leg_anal", "fft_anal"};
+    return run_to_spec(p, j, f, nullptr, alm, nullptr, nfields, dtype, stream);
 }
 
 int shtk_adjoint_synthesis(shtk_plan *p, const void *fbar, void *almbar, int nfields, int dtype, int real_output,
                            void *stream) {
-    int rc = check_common(p, nfields, dtype, {fbar, almbar});
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lk(p->mu);
-    DeviceGuard g(p->device);
-    hipStream_t st = (hipStream_t)stream;
-    const WsLayout L = ws_layout(p, nfields, 0);
-    if ((rc = ensure_ws(p, L.total))) return rc;
-    char *ws = (char *)p->ws;
-    double2 *F0 = (double2 *)ws;
-    shtk::FftArgs fa = fft_args(p);
-    fa.F = F0;
-    fa.grid = (void *)fbar;
-    fa.io_f32 = dtype == SHTK_F32;
-    HIP_TRY(shtk::launch_fft(fa, nfields, 0, st), "phi adjoint launch");
-    shtk::LegArgs a = base_args(p, p->d.lmax);
-    a.F0 = F0;
-    a.wlat = nullptr;
-    a.g = p->inv_scale / p->d.nlon;
-    a.mfac_mode = real_output ? 1 : 0;
-    a.out0 = almbar;
-    a.io_f32 = dtype == SHTK_F32;
-    void *spec = ws + L.f_bytes;
-    if (!p->canonical) {
-        a.out0 = spec;
-        a.io_f32 = 0;
-    }
-    HIP_TRY(shtk::launch_leg_anal(a, nfields, 0, st), "Legendre adjoint launch");
-    if (!p->canonical)
-        HIP_TRY(shtk::launch_spec_scale(spec, almbar, 0, dtype == SHTK_F32, p->convscale, 0, p->d.lmax,
-                                        p->d.mmax, p->d.mres, a.spec_fstride, nfields, st),
-                "coefficient conversion");
-    return SHTK_OK;
+    RC_TRY(check_common(p, nfields, dtype, {fbar, almbar}));
+    // src/core_transforms.jl:477-519: wm * phi_inv_scale/nlon * sum_i P FFT(fbar), no weights
+    Job j{0, 0, nullptr, p->inv_scale / p->d.nlon, real_output ? 1 : 0, 0, 0, p->d.lmax, "leg_anal", "fft_anal"};
+    return run_to_spec(p, j, fbar, nullptr, almbar, nullptr, nfields, dtype, stream);
 }
 
 int shtk_adjoint_analysis(shtk_plan *p, const void *almbar, void *fbar, int nfields, int dtype, void *stream) {
-    int rc = check_common(p, nfields, dtype, {fbar, almbar});
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lk(p->mu);
-    DeviceGuard g(p->device);
-    hipStream_t st = (hipStream_t)stream;
-    const WsLayout L = ws_layout(p, nfields, 0);
-    if ((rc = ensure_ws(p, L.total))) return rc;
-    char *ws = (char *)p->ws;
-    double2 *F0 = (double2 *)ws;
-    shtk::LegArgs a = base_args(p, p->d.lmax);
-    a.in0 = almbar;
-    a.io_f32 = dtype == SHTK_F32;
-    if (!p->canonical) {
-        void *spec = ws + L.f_bytes;
-        HIP_TRY(shtk::launch_spec_scale(almbar, spec, a.io_f32, 0, p->convscale, 1, p->d.lmax, p->d.mmax,
-                                        p->d.mres, a.spec_fstride, nfields, st),
-                "coefficient conversion");
-        a.in0 = spec;
-        a.io_f32 = 0;
-    }
-    a.F0 = F0;
-    a.wlat = p->wadj;
-    a.g = 1.0;
-    a.mfac_mode = 2;
-    HIP_TRY(shtk::launch_leg_synth(a, nfields, 0, st), "Legendre adjoint launch");
-    shtk::FftArgs fa = fft_args(p);
-    fa.F = F0;
-    fa.grid = fbar;
-    fa.io_f32 = dtype == SHTK_F32;
-    HIP_TRY(shtk::launch_fft(fa, nfields, 1, st), "phi adjoint launch");
-    return SHTK_OK;
+    RC_TRY(check_common(p, nfields, dtype, {fbar, almbar}));
+    // src/core_transforms.jl:611-652: F = 2pi w_i sum_l P abar, ifft over m >= 0;
+    // the real part is Re(F_0)/N + 2 Re(sum_{m>0} (F_m/2N) e^{im phi})
+    Job j{0, 1, p->wscaled, 1.0, 2, 0, 0, p->d.lmax, "leg_synth", "fft_synth"};
+    return run_to_grid(p, j, almbar, nullptr, fbar, nullptr, nfields, dtype, stream);
 }
 
+// --- vector ------------------------------------------------------------------
 int shtk_synthesis_sphtor(shtk_plan *p, const void *S, const void *T, void *Vt, void *Vp, int nfields, int dtype,
                           int real_output, int ltr, void *stream) {
-    int rc = check_common(p, nfields, dtype, {S, T, Vt, Vp});
-    if (rc) return rc;
-    (void)real_output;
-    (void)ltr;
-    (void)stream;
-    return fail(SHTK_EUNSUPPORTED, "vector synthesis is not implemented yet");
+    RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
+    if (real_output != 1)
+        return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is not implemented by this backend");
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    // kernel sums are sin(theta) V; Robert form keeps them, otherwise divide by sin(theta)
+    Job j{1, 0, p->d.robert_form ? nullptr : p->invs, p->inv_scale / p->d.nlon, 0, 0, 0, ltr,
+          "leg_synth_sphtor", "fft_synth"};
+    return run_to_grid(p, j, S, T, Vt, Vp, nfields, dtype, stream);
 }
 
 int shtk_analysis_sphtor(shtk_plan *p, const void *Vt, const void *Vp, void *S, void *T, int nfields, int dtype,
                          int ltr, void *stream) {
-    int rc = check_common(p, nfields, dtype, {S, T, Vt, Vp});
-    if (rc) return rc;
-    (void)ltr;
-    (void)stream;
-    return fail(SHTK_EUNSUPPORTED, "vector analysis is not implemented yet");
+    RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    Job j{1, 1, p->d.robert_form ? p->winvs2 : p->winvs, 1.0, 0, 1, 0, ltr, "leg_anal_sphtor", "fft_anal"};
+    return run_to_spec(p, j, Vt, Vp, S, T, nfields, dtype, stream);
 }
 
 int shtk_adjoint_synthesis_sphtor(shtk_plan *p, const void *Vtb, const void *Vpb, void *Sb, void *Tb, int nfields,
                                   int dtype, int real_output, void *stream) {
-    int rc = check_common(p, nfields, dtype, {Vtb, Vpb, Sb, Tb});
-    if (rc) return rc;
-    (void)real_output;
-    (void)stream;
-    return fail(SHTK_EUNSUPPORTED, "vector adjoint is not implemented yet");
+    RC_TRY(check_common(p, nfields, dtype, {Vtb, Vpb, Sb, Tb}));
+    Job j{1, 0, p->invs, p->inv_scale / p->d.nlon, real_output ? 1 : 0, 0, p->d.robert_form ? 2 : 0, p->d.lmax,
+          "leg_anal_sphtor", "fft_anal"};
+    return run_to_spec(p, j, Vtb, Vpb, Sb, Tb, nfields, dtype, stream);
 }
 
 int shtk_adjoint_analysis_sphtor(shtk_plan *p, const void *Sb, const void *Tb, void *Vtb, void *Vpb, int nfields,
                                  int dtype, void *stream) {
-    int rc = check_common(p, nfields, dtype, {Vtb, Vpb, Sb, Tb});
-    if (rc) return rc;
-    (void)stream;
-    return fail(SHTK_EUNSUPPORTED, "vector adjoint is not implemented yet");
+    RC_TRY(check_common(p, nfields, dtype, {Vtb, Vpb, Sb, Tb}));
+    Job j{1, 1, p->winvs, 1.0, 2, 1, p->d.robert_form ? 1 : 0, p->d.lmax, "leg_synth_sphtor", "fft_synth"};
+    return run_to_grid(p, j, Sb, Tb, Vtb, Vpb, nfields, dtype, stream);
 }
 
 }  // extern "C"

@@ -1,0 +1,318 @@
+"""Pins the CPU oracle before it is trusted as the GPU checker.
+
+Each check is independent of the oracle's own formulas:
+* SHTns 3.7 golden fixtures (upstream C library outputs held by the reference);
+* 256-bit mpmath Legendre rows (the reference's BigFloat construction);
+* Gauss nodes refined in 60-digit arithmetic and numpy's leggauss;
+* numpy.fft (pocketfft) and a brute-force DFT;
+* closed forms through l = 2, Parseval, adjoint dot-product identities and
+  central finite differences.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import _fixtures as FX
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# --------------------------------------------------------------------------- Gauss nodes
+@pytest.mark.parametrize("n", [6, 8, 128, 1024])
+def test_gauss_nodes_match_high_precision(oracle, n):
+    g = np.load(os.path.join(HERE, "golden", "gauss_mpmath.npz"))
+    x, w = oracle.gausslegendre(n)
+    assert np.max(np.abs(x - g[f"x{n}"])) < 5e-16
+    # the reference's weight formula 2/((1-z^2) P_n'^2) loses digits near the
+    # poles: relative error up to a few 1e-13 on the smallest weights
+    assert np.max(np.abs(w - g[f"w{n}"])) < 5e-16
+    assert np.max(np.abs(w - g[f"w{n}"]) / g[f"w{n}"]) < 5e-12
+    assert abs(w.sum() - 2.0) < 1e-13
+    assert np.all(x[::-1] == -x) and np.all(w[::-1] == w)
+
+
+def test_gauss_nodes_leggauss_and_exactness(oracle):
+    for n in (5, 16, 33):
+        x, w = oracle.gausslegendre(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        assert np.allclose(x, xr, atol=1e-15) and np.allclose(w, wr, atol=1e-14)
+        for d in range(0, 2 * n):  # exact for polynomials of degree <= 2n-1
+            exact = 0.0 if d % 2 else 2.0 / (d + 1)
+            assert abs(np.sum(w * x ** d) - exact) < 1e-13
+
+
+# --------------------------------------------------------------------------- Legendre rows
+def test_plm_closed_forms(oracle):
+    for x in (0.3, -0.77, 0.999):
+        s = math.sqrt(1 - x * x)
+        P0, P1, P2 = (oracle.plm_row(x, 2, m) for m in range(3))
+        assert np.allclose(P0, [math.sqrt(1 / (4 * math.pi)), math.sqrt(3 / (4 * math.pi)) * x,
+                                math.sqrt(5 / (16 * math.pi)) * (3 * x * x - 1)], rtol=3e-15, atol=1e-16)
+        assert np.allclose(P1[1:], [-math.sqrt(3 / (8 * math.pi)) * s,
+                                    -math.sqrt(15 / (8 * math.pi)) * s * x], rtol=3e-15)
+        assert abs(P2[2] - math.sqrt(15 / (32 * math.pi)) * s * s) < 3e-15
+        assert P1[0] == 0 and P2[0] == 0 and P2[1] == 0
+
+
+def test_plm_rows_match_mpmath(oracle):
+    g = np.load(os.path.join(HERE, "golden", "plm_mpmath.npz"))
+    for k in range(int(g["nrows"][0])):
+        lmax, m, s, x = g[f"row{k}_meta"]
+        lmax, m = int(lmax), int(m)
+        ref = g[f"row{k}"]
+        P = oracle.plm_row(float(x), lmax, m)
+        scale = max(1.0, np.max(np.abs(ref)))
+        # 1e-11 is the reference's bound at its own points (k < 4); the extra
+        # lmax=1023 rows include the node nearest the pole, where 1023 FP64 steps
+        # of the recurrence at x ~ 1 - 3e-6 drift by ~1.1e-11 relative
+        tol = 1e-11 if k < 4 else 3e-11
+        assert np.max(np.abs(P - ref)) <= tol * scale, (lmax, m, s)
+
+
+def test_plm_bounded_and_dtheta(oracle):
+    g = np.load(os.path.join(HERE, "golden", "plm_mpmath.npz"))
+    for m in (0, 3, 200, 512):
+        for x in (0.9999, 0.5, 0.01):
+            assert np.max(np.abs(oracle.plm_row(x, 512, m))) < 2.0 * math.sqrt(2 * 512 + 1)
+    for j in range(4):
+        x = float(g[f"zonal{j}_x"][0])
+        ref = g[f"zonal{j}"]
+        _, dP, _ = oracle.plm_dtheta_over_sinth_row(x, 32, 0)
+        if abs(x) < 1:
+            assert np.allclose(dP, ref, rtol=5e-13, atol=1e-300)
+
+
+def test_dtheta_rows_against_finite_differences(oracle):
+    """dP/dtheta from the d_{l,m} identity against a centred difference in theta."""
+    lmax = 60
+    for m in (1, 5, 30):
+        for th in (0.3, 1.2, 2.8):
+            x = math.cos(th)
+            P, dP, Ps = oracle.plm_dtheta_over_sinth_row(x, lmax, m)
+            h = 1e-6
+            fd = (oracle.plm_row(math.cos(th + h), lmax, m) - oracle.plm_row(math.cos(th - h), lmax, m)) / (2 * h)
+            assert np.allclose(dP, fd, atol=1e-7 * max(1, np.max(np.abs(fd))))
+            assert np.allclose(Ps, P / math.sin(th), rtol=1e-14, atol=1e-300)
+
+
+# --------------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [8, 10, 12, 101, 256, 2048, 1000])
+def test_fft_matches_numpy(oracle, n):
+    rng = np.random.default_rng(n)
+    v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.allclose(oracle.fft(v), np.fft.fft(v), rtol=0, atol=1e-12 * np.sqrt(n) * np.max(np.abs(v)))
+    assert np.allclose(oracle.fft(v, inverse=True), np.fft.ifft(v), atol=1e-14 * np.max(np.abs(v)))
+
+
+def test_fft_brute_force_dft(oracle):
+    n = 12
+    v = np.arange(n) * 0.3 - 1j * np.arange(n) ** 2 * 0.01
+    jk = np.outer(np.arange(n), np.arange(n))
+    assert np.allclose(oracle.fft(v), np.exp(-2j * np.pi * jk / n) @ v, atol=1e-13)
+
+
+# --------------------------------------------------------------------------- SHTns 3.7 fixtures
+def _cfg(oracle, fx):
+    return oracle.Config(fx["lmax"], fx["nlat"], **FX.config_kwargs(fx))
+
+
+def test_fixture_scalar_real_full(oracle):
+    fx = FX.fixture("scalar_real_full_gauss_f64")
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    ok, _ = FX.isapprox(cfg.synthesis(FX.unpack(fx, p["coefficients"])), p["field"], fx["atol"], fx["rtol"])
+    assert ok
+    ok, _ = FX.isapprox(FX.pack(fx, cfg.analysis(p["field"])), p["coefficients"], fx["atol"], fx["rtol"])
+    assert ok
+
+
+def test_fixture_scalar_batch_real_norm(oracle):
+    fx = FX.fixture("scalar_batch")
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    q = p["coefficients"]
+    assert FX.isapprox(cfg.synthesis(FX.unpack(fx, q)), p["field"], fx["atol"], fx["rtol"])[0]
+    got = cfg.analysis(p["field"])
+    for k in range(q.shape[1]):
+        assert FX.isapprox(FX.pack(fx, got[:, :, k]), q[:, k], fx["atol"], fx["rtol"])[0]
+
+
+def test_fixture_scalar_l_conventions(oracle):
+    """fourpi normalization, no Condon-Shortley phase, degree truncation ltr=2."""
+    fx = FX.fixture("scalar_l")
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    ltr = fx["ltr"]
+    q = p["coefficients"]
+    assert FX.isapprox(cfg.synthesis(FX.unpack(fx, q), ltr=ltr), p["field"], fx["atol"], fx["rtol"])[0]
+    got = FX.pack(fx, cfg.analysis(p["field"], ltr=ltr))
+    assert FX.isapprox(got, FX.truncated(fx, q, ltr), fx["atol"], fx["rtol"])[0]
+
+
+@pytest.mark.parametrize("fid", ["analysis_scalar_full", "analysis_scalar_l"])
+def test_fixture_analysis_scalar(oracle, fid):
+    """Non-band-limited deterministic inputs: pins the quadrature itself."""
+    fx = FX.fixture(fid)
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    ltr = fx["ltr"]
+    f = p["field"]
+    q = p["coefficients"]
+    got = cfg.analysis(f, ltr=ltr)
+    if got.ndim == 2:
+        got = got[:, :, None]
+        q = q[:, None]
+    for k in range(q.shape[1]):
+        assert FX.isapprox(FX.pack(fx, got[:, :, k]), q[:, k], fx["atol"], fx["rtol"])[0]
+
+
+def test_fixture_analysis_inputs_pristine():
+    """The analysis payloads hold the deterministic generator's bytes
+    (test/parity/shtns37_fixtures.jl:261-330)."""
+    fx = FX.fixture("analysis_scalar_full")
+    f = FX.payloads(fx)["field"]
+    nlat, nphi = fx["nlat"], fx["nphi"]
+    for k, seed in enumerate((0.2, 0.3)):
+        it, ip = np.meshgrid(np.arange(1, nlat + 1), np.arange(1, nphi + 1), indexing="ij")
+        i = (it - 1) * nphi + ip
+        assert np.allclose(f[:, :, k], seed + 0.013 * i - 0.0007 * i * i, atol=8 * 2.2e-16, rtol=8 * 2.2e-16)
+
+
+def test_fixture_sphtor_full_f32(oracle):
+    fx = FX.fixture("sphtor_full_f32")
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    Vt, Vp = cfg.synthesis_sphtor(FX.unpack(fx, p["S"]), FX.unpack(fx, p["T"]))
+    assert FX.isapprox(Vt, p["Vt"], fx["atol"], fx["rtol"])[0]
+    assert FX.isapprox(Vp, p["Vp"], fx["atol"], fx["rtol"])[0]
+    S, T = cfg.analysis_sphtor(p["Vt"].astype(np.float64), p["Vp"].astype(np.float64))
+    assert FX.isapprox(FX.pack(fx, S), p["S"], fx["atol"], fx["rtol"])[0]
+    assert FX.isapprox(FX.pack(fx, T), p["T"], fx["atol"], fx["rtol"])[0]
+
+
+def test_fixture_sphtor_batch_real_norm(oracle):
+    fx = FX.fixture("sphtor_batch")
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    Vt, Vp = cfg.synthesis_sphtor(FX.unpack(fx, p["S"]), FX.unpack(fx, p["T"]))
+    assert FX.isapprox(Vt, p["Vt"], fx["atol"], fx["rtol"])[0]
+    assert FX.isapprox(Vp, p["Vp"], fx["atol"], fx["rtol"])[0]
+    S, T = cfg.analysis_sphtor(p["Vt"], p["Vp"])
+    for k in range(2):
+        assert FX.isapprox(FX.pack(fx, S[:, :, k]), p["S"][:, k], fx["atol"], fx["rtol"])[0]
+        assert FX.isapprox(FX.pack(fx, T[:, :, k]), p["T"][:, k], fx["atol"], fx["rtol"])[0]
+
+
+@pytest.mark.parametrize("fid", ["analysis_sphtor_full", "analysis_sphtor_l"])
+def test_fixture_analysis_sphtor(oracle, fid):
+    fx = FX.fixture(fid)
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    S, T = cfg.analysis_sphtor(p["Vt"], p["Vp"], ltr=fx["ltr"])
+    Sq, Tq = p["S"], p["T"]
+    if S.ndim == 2:
+        S, T, Sq, Tq = S[:, :, None], T[:, :, None], Sq[:, None], Tq[:, None]
+    for k in range(Sq.shape[1]):
+        assert FX.isapprox(FX.pack(fx, S[:, :, k]), Sq[:, k], fx["atol"], fx["rtol"])[0]
+        assert FX.isapprox(FX.pack(fx, T[:, :, k]), Tq[:, k], fx["atol"], fx["rtol"])[0]
+
+
+# --------------------------------------------------------------------------- identities
+def test_closed_form_synthesis(oracle):
+    """Y_1^0 and Y_2^1 on the grid (test/parity/scalar_full.jl:48-80)."""
+    cfg = oracle.Config(4, 6, nlon=10)
+    th = np.arccos(cfg.x)[:, None]
+    ph = (2 * np.pi / 10) * np.arange(10)[None, :]
+    a = np.zeros((5, 5), complex)
+    a[1, 0] = 1.0
+    assert np.allclose(cfg.synthesis(a), math.sqrt(3 / (4 * math.pi)) * np.cos(th) + 0 * ph, atol=3e-12)
+    a = np.zeros((5, 5), complex)
+    a[2, 1] = 0.5 - 0.25j
+    ref = 2 * np.real((0.5 - 0.25j) * -math.sqrt(15 / (8 * math.pi)) * np.sin(th) * np.cos(th) * np.exp(1j * ph))
+    assert np.allclose(cfg.synthesis(a), ref, atol=3e-12)
+
+
+def test_parseval_and_round_trip(oracle):
+    lmax = 40
+    cfg = oracle.Config(lmax, 42, nlon=96)
+    a = oracle.random_alm(lmax, lmax, seed=5)
+    f = cfg.synthesis(a)
+    grid_energy = np.sum(cfg.w[:, None] * f ** 2) * (2 * np.pi / cfg.nlon)
+    spec_energy = np.sum(np.abs(a[:, 0]) ** 2) + 2 * np.sum(np.abs(a[:, 1:]) ** 2)
+    assert abs(grid_energy - spec_energy) < 1e-12 * spec_energy
+    assert oracle.rel_l2(cfg.analysis(f), a) < 1e-13
+
+
+def test_vector_identities(oracle):
+    """Round trip and the divergence identity: the spheroidal part S gives the
+    scalar field -l(l+1) S as the surface divergence (src/sphtor_transforms.jl:115-122)."""
+    lmax = 30
+    cfg = oracle.Config(lmax, 32, nlon=64)
+    S = oracle.random_alm(lmax, lmax, 4, 2.0, lmin=1)
+    T = oracle.random_alm(lmax, lmax, 5, 2.0, lmin=1)
+    Vt, Vp = cfg.synthesis_sphtor(S, T)
+    S2, T2 = cfg.analysis_sphtor(Vt, Vp)
+    assert oracle.rel_l2(S2, S) < 1e-13 and oracle.rel_l2(T2, T) < 1e-13
+    # toroidal-only field is divergence free: its spheroidal analysis vanishes
+    Vt, Vp = cfg.synthesis_sphtor(np.zeros_like(S), T)
+    S3, _ = cfg.analysis_sphtor(Vt, Vp)
+    assert np.max(np.abs(S3)) < 1e-13 * np.max(np.abs(T))
+
+
+def _inner_real(a, b):
+    return float(np.sum(np.real(np.conj(a) * b)))
+
+
+def test_adjoint_dot_product_identities(oracle):
+    """<synthesis(a), y> = Re<a, adj(y)> for the Zygote-scaled adjoint, which
+    counts m>0 coefficients twice (they enter the grid with their conjugates)."""
+    lmax = 20
+    cfg = oracle.Config(lmax, 22, nlon=48)
+    rng = np.random.default_rng(3)
+    a = oracle.random_alm(lmax, lmax, seed=8)
+    y = np.asfortranarray(rng.standard_normal((22, 48)))
+    g = cfg.adjoint_synthesis(y)
+    lhs = np.sum(cfg.synthesis(a) * y)
+    # adjoint returns 2 dL/dconj(a); directional derivative along a is Re<g, a>
+    assert abs(lhs - _inner_real(g, a)) < 1e-11 * abs(lhs)
+    # analysis adjoint: <analysis(f), b>_R = <f, adj(b)>
+    f = np.asfortranarray(rng.standard_normal((22, 48)))
+    b = oracle.random_alm(lmax, lmax, seed=9)
+    lhs = _inner_real(cfg.analysis(f), b)
+    rhs = float(np.sum(f * cfg.adjoint_analysis(b).real))
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
+    # vector pair
+    S = oracle.random_alm(lmax, lmax, 4, 1.0, lmin=1)
+    T = oracle.random_alm(lmax, lmax, 5, 1.0, lmin=1)
+    yt = np.asfortranarray(rng.standard_normal((22, 48)))
+    yp = np.asfortranarray(rng.standard_normal((22, 48)))
+    Vt, Vp = cfg.synthesis_sphtor(S, T)
+    Sb, Tb = cfg.adjoint_synthesis_sphtor(yt, yp)
+    lhs = np.sum(Vt * yt) + np.sum(Vp * yp)
+    assert abs(lhs - _inner_real(Sb, S) - _inner_real(Tb, T)) < 1e-11 * abs(lhs)
+    Sa, Ta = cfg.analysis_sphtor(yt, yp)
+    vt, vp = cfg.adjoint_analysis_sphtor(S, T)
+    lhs = _inner_real(Sa, S) + _inner_real(Ta, T)
+    assert abs(lhs - np.sum(yt * vt.real) - np.sum(yp * vp.real)) < 1e-11 * abs(lhs)
+
+
+def test_adjoint_synthesis_finite_difference(oracle):
+    """Gradient of L = sum w_i (f - t)^2 against a central difference (h=1e-6,
+    rtol 5e-4, test/serial/test_gradients.jl:114-149)."""
+    lmax = 12
+    cfg = oracle.Config(lmax, 14, nlon=32)
+    a = oracle.random_alm(lmax, lmax, seed=6)
+    t = cfg.synthesis(oracle.random_alm(lmax, lmax, seed=7))
+    wgrid = cfg.w[:, None] * np.ones((1, cfg.nlon))
+
+    def loss(x):
+        return float(np.sum(wgrid * (cfg.synthesis(x) - t) ** 2))
+
+    g = cfg.adjoint_synthesis(2 * wgrid * (cfg.synthesis(a) - t))
+    rng = np.random.default_rng(1)
+    d = oracle.random_alm(lmax, lmax, seed=12) * rng.standard_normal()
+    h = 1e-6
+    fd = (loss(a + h * d) - loss(a - h * d)) / (2 * h)
+    assert abs(fd - _inner_real(g, d)) <= 5e-4 * abs(fd)

@@ -44,6 +44,13 @@ enum shtk_norm { SHTK_NORM_ORTHONORMAL = 0, SHTK_NORM_FOURPI = 1, SHTK_NORM_SCHM
 enum shtk_phi_scale { SHTK_PHI_DFT = 0, SHTK_PHI_QUAD = 1 };
 enum shtk_dtype { SHTK_F64 = 0, SHTK_F32 = 1 };
 
+/* Contraction precision of the batched entry points (*_ex).
+ * SHTK_PREC_FP64: the FP64 recurrence-and-contraction kernels (the plain path).
+ * SHTK_PREC_SPLIT: FP64 Legendre values and tables, contraction as an Ozaki
+ *   split on int8 matrix cores (6 slices, exact int32 accumulation, FP64
+ *   cross-slice sums); FP64-grade results (relative L2 error ~1e-12). */
+enum shtk_precision { SHTK_PREC_FP64 = 0, SHTK_PREC_SPLIT = 1 };
+
 /* Grid and spectral descriptor.  Mirrors the keyword set of
  * create_gauss_config(lmax, nlat; mmax, mres, nlon, norm, cs_phase, real_norm,
  * robert_form) (src/config.jl:633-655) plus the phi scaling mode
@@ -91,6 +98,14 @@ int shtk_synthesis(shtk_plan *plan, const void *alm, void *f, int nfields, int d
  * analysis_batch (src/core_transforms.jl:169-207, src/batch_transforms.jl:312-429)
  * behind _gpu_adapter_analysis (src/device_utils.jl:87-89). */
 int shtk_analysis(shtk_plan *plan, const void *f, void *alm, int nfields, int dtype, int ltr, void *stream);
+
+/* Batched scalar synthesis with a contraction precision (enum shtk_precision):
+ * the batch transforms of src/batch_transforms.jl:455-655, which share one
+ * Legendre row per (m, latitude) across all fields (:519-535).  Same layouts and
+ * arguments as shtk_synthesis; SHTK_PREC_SPLIT needs a scalar canonical-or-
+ * convertible plan with lmax <= 4095. */
+int shtk_synthesis_ex(shtk_plan *plan, const void *alm, void *f, int nfields, int dtype, int real_output,
+                      int ltr, int precision, void *stream);
 
 /* Vector synthesis (S, T) -> (Vt, Vp): synthesis_sphtor / synthesis_sphtor_batch
  * (src/sphtor_transforms.jl:162-241, src/batch_transforms.jl:745-815), behind

@@ -63,7 +63,35 @@ struct LegArgs {
     void *out0, *out1;      // spectral outputs (analysis)
     long long spec_fstride; // complex elements per field in spectral arrays
     double2 *F0, *F1;       // Fourier workspace [field][im][nlat]
+    int gx, gy, gz;         // work items form a (gx, gy, gz = nfields) box, x fastest
+    int xcd;                // synthesis: 1 keeps the pair blocks of an order on one XCD
 };
+
+// Work distribution over the 8 XCDs (each with its own 4 MB L2).  The
+// hardware deals consecutive blocks of a 1-D grid round-robin over the XCDs.
+// A "unit" is a group of `per_unit` blocks that read the same order's
+// coefficients and spectra.  With xcd = 1 the grid holds 8 * per_unit *
+// ceil(nunits/8) blocks and every unit runs entirely on XCD (unit mod 8), so
+// its blocks share one L2 while consecutive orders still spread over all XCDs;
+// blocks beyond the last unit return false.  With xcd = 0 block b is simply
+// (unit b / per_unit, index b % per_unit).
+constexpr int kXcds = 8;
+__device__ inline bool unit_of_block(int xcd, int per_unit, int nunits, int *unit, int *index) {
+    const int id = blockIdx.x;
+    if (!xcd) {
+        *unit = id / per_unit;
+        *index = id % per_unit;
+        return *unit < nunits;
+    }
+    const int x = id % kXcds, k = id / kXcds;
+    *unit = x + kXcds * (k / per_unit);
+    *index = k % per_unit;
+    return *unit < nunits;
+}
+inline unsigned grid_for_units(int xcd, int per_unit, int nunits) {
+    if (!xcd) return (unsigned)per_unit * nunits;
+    return (unsigned)kXcds * per_unit * ((nunits + kXcds - 1) / kXcds);
+}
 
 struct SeedArgs {
     int lmax, mmax, mres, np, nm, seg;
@@ -95,6 +123,7 @@ struct FftArgs {
 struct shtk_plan {
     shtk_desc d{};
     int np = 0, nm = 0, seg = 128, nseg_total = 0;
+    int xcd_map = 0, synth_chains = 2;  // launch-shape knobs
     bool canonical = true;
     double inv_scale = 0, scale_phi = 0;
     std::vector<double> hx, hw;  // north-first nodes and weights (nlat)
@@ -129,14 +158,69 @@ struct shtk_plan {
     // lazily grown workspace
     void *ws = nullptr;
     size_t ws_bytes = 0;
+
+    // split-precision tables (built on first use) and their workspace
+    double *sectv = nullptr;  // [nm][np] scaled sectoral seeds
+    int *sectk = nullptr;
+    double *oz_q0 = nullptr, *oz_q1 = nullptr;  // [nm][4][np]
+    int *oz_qk = nullptr, *oz_ep = nullptr, *oz_chbase = nullptr;
+    int oz_ch_total = 0;
+    void *oz_ws = nullptr;
+    size_t oz_ws_bytes = 0;
     std::mutex mu;
 };
 
 namespace shtk {
 
+// Batched split-precision contraction (shtk_split.hip).  Fields are taken 16
+// at a time ("field groups", 32 real columns).  Per order the degrees m..lmax
+// are cut into 4 equal quarters of Kq = 32 * nch(m) degrees, nch(m) =
+// ceil((lmax-m+1)/128); lane group g of a wavefront runs the recurrence over
+// quarter g, 32 degrees per chunk.  The field operand is stored as balanced
+// base-256 digits in the exact register layout of the int8 MFMA B operand.
+constexpr int OZ_SLICES = 6;
+constexpr int OZ_COLS = 32;
+constexpr int OZ_FIELDS = OZ_COLS / 2;
+constexpr int OZ_CHUNK_V4 = 2 * OZ_SLICES * 2 * 64;  // 16-byte B operands per chunk (1536 = 24 KB)
+
+struct OzArgs {
+    int lmax, mmax, mres, nlat, np, nm, ltr, nf, nfg;
+    const double *x;
+    const double2 *ab;
+    const long long *coff;
+    const double *q0, *q1;  // [nm][4][np] P_{lq-1}, P_{lq} at the quarter starts (scaled representation)
+    const int *qk;          // [nm][4][np] scale count
+    const int *ep;          // [nm][np] |P| < 2^ep over the order's degrees
+    const int *chbase;      // [nm] first chunk of each order
+    int ch_total;
+    const void *spec_in;    // canonical coefficients (synthesis)
+    int in_f32;
+    long long spec_fstride;
+    int4 *bdig;             // [nfg][ch_total][1536] B operands
+    double *colsum;         // [nfg][nm][2][32] sum over degrees of the integer field values, per parity
+    int *eb;                // [nfg][nm][32] column exponents
+    double2 *F0;            // Fourier output [field][im][nlat]
+    double g;
+    int xcd;
+};
+
+struct OzSeedArgs {
+    int lmax, mres, np, nm;
+    const double *x;
+    const double2 *ab;
+    const long long *coff;
+    const double *sectv;
+    const int *sectk;
+    double *q0, *q1;
+    int *qk, *ep;
+};
+
+hipError_t launch_oz_seed(const OzSeedArgs &a, hipStream_t st);
+hipError_t launch_oz_synth(const OzArgs &a, hipStream_t st);
+
 // Kernel launchers (shtk_legendre.hip, shtk_fft.hip).
 hipError_t launch_seed(const SeedArgs &a, hipStream_t st);
-hipError_t launch_leg_synth(const LegArgs &a, int nfields, int vector, hipStream_t st);
+hipError_t launch_leg_synth(const LegArgs &a, int nfields, int vector, int chains, hipStream_t st);
 hipError_t launch_leg_anal(const LegArgs &a, int nfields, int vector, hipStream_t st);
 hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st);
 hipError_t launch_spec_scale(const void *in, void *out, int in_f32, int out_f32, const double *scale, int divide,

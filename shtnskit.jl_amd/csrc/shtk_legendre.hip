@@ -61,6 +61,7 @@ __device__ __forceinline__ void add_c(T *p, size_t idx, double re, double im) {
 
 __device__ __forceinline__ int nseg_of(int lmax, int m, int S) { return (lmax - m + S) / S; }
 
+
 // ---------------------------------------------------------------------------
 // Plan-time seeds: one thread per (order, latitude pair) runs the P-form
 // recurrence with scaled exponents (src/legendre.jl:252-322) through the row
@@ -182,7 +183,9 @@ __device__ __forceinline__ void synth_seg(const double2 *__restrict__ ab, const 
 template <int K, typename IT>
 __global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
     __shared__ double red[4][4 * K][64];
-    const int b = blockIdx.x, im = blockIdx.y, f = blockIdx.z;
+    int b, unit;
+    if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
+    const int im = unit % a.gy, f = unit / a.gy;
     const int m = im * a.mres;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -325,7 +328,9 @@ template <int K, typename IT>
 __global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
     __shared__ double red[4][16][64];
     static_assert(K == 1, "vector synthesis reduces one chain per lane");
-    const int b = blockIdx.x, im = blockIdx.y, f = blockIdx.z;
+    int b, unit;
+    if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
+    const int im = unit % a.gy, f = unit / a.gy;
     const int m = im * a.mres;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -413,25 +418,27 @@ __global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
 // Analysis work items: (global segment, pair block of 512, field).
 // ---------------------------------------------------------------------------
 struct Item {
-    int gs, im, s, m, l0, lend, lstop, pb0, pb1, pstart, f;
+    int gs, pbi, im, s, m, l0, lend, lstop, pb0, pb1, pstart, f;
     bool fast;
 };
 
 template <int PBLK>
 __device__ __forceinline__ Item make_item(const LegArgs &a) {
     Item it;
-    it.gs = blockIdx.x;
+    const int item = blockIdx.x;
+    it.gs = item % a.gx;
+    it.pbi = (item / a.gx) % a.gy;
     it.im = a.segm[it.gs];
     it.s = it.gs - a.segbase[it.im];
     it.m = it.im * a.mres;
     it.l0 = it.m + it.s * a.seg;
     it.lend = min(it.l0 + a.seg, a.lmax + 1);
     it.lstop = min(it.lend, a.ltr + 1);
-    it.pb0 = blockIdx.y * PBLK;
+    it.pb0 = it.pbi * PBLK;
     it.pb1 = min(it.pb0 + PBLK, a.np);
     it.pstart = max(it.pb0, a.pfirst[it.gs]);
     it.fast = it.pstart >= a.pfast[it.gs];
-    it.f = blockIdx.z;
+    it.f = item / (a.gx * a.gy);
     return it;
 }
 
@@ -439,7 +446,7 @@ __device__ __forceinline__ Item make_item(const LegArgs &a) {
 // orders between multiples of mres); skipped in accumulate mode (pre-zeroed).
 template <typename OT>
 __device__ void zero_fill(const LegArgs &a, const Item &it, OT *col, bool all_dead, int lane) {
-    if (a.atomic || blockIdx.y != 0) return;
+    if (a.atomic || it.pbi != 0) return;
     const int lz0 = all_dead ? it.l0 : it.lstop;
     for (int l = lz0 + lane; l < it.lend; l += 64) st_c(col, l, 0.0, 0.0);
     if (it.s == 0) {
@@ -449,18 +456,22 @@ __device__ void zero_fill(const LegArgs &a, const Item &it, OT *col, bool all_de
     }
 }
 
-// Scalar analysis of one item with K chains per lane.
+// Scalar analysis of one item: two waves, K <= 4 chains per lane (pairs
+// pstart + tid + 128 j).  The reduced-sum scale c_l of the segment sits in LDS;
+// beta_{l+1} streams through scalar loads one chunk ahead.
+constexpr int ANS_THREADS = 128;
+
 template <int K, typename OT>
 __device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, const double2 *__restrict__ qcm,
-                                           double2 *tb, OT *col, int lane) {
+                                           double2 *tb, const double *cseg, OT *col, int tid) {
     constexpr int CH = 8;
     const int np = a.np, nlat = a.nlat;
     const double2 *Fi = a.F0 + ((size_t)it.f * a.nm + it.im) * nlat;
-    double x[K], prev[K], cur[K], gpr[K], gpi[K], gmr[K], gmi[K], cm[K];
+    double x[K], prev[K], cur[K], gpr[K], gpi[K], gmr[K], gmi[K];
     int kk[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        const int p = it.pstart + lane + 64 * j;
+        const int p = it.pstart + tid + ANS_THREADS * j;
         const bool act = p < it.pb1;
         const int pc = act ? p : 0;
         x[j] = a.x[pc];
@@ -468,7 +479,6 @@ __device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, con
         prev[j] = act ? a.seedq0[q] : 0.0;
         cur[j] = act ? a.seedq1[q] : 0.0;
         kk[j] = act ? a.seedk[q] : 0;
-        cm[j] = (it.fast || kk[j] == 0) ? 1.0 : 0.0;
         double2 FN = make_double2(0.0, 0.0), FS = make_double2(0.0, 0.0);
         double wl = 0.0;
         if (act) {
@@ -485,21 +495,15 @@ __device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, con
     const double gm = a.g * mfac_of(a.mfac_mode, it.m);
     const int nst = it.lstop - it.l0;
     const int lmax = a.lmax;
-    // coefficient stream: beta_{l+1} for the recurrence and c_l for the reduced
-    // sums, one chunk prefetched ahead (wave-uniform, scalar loads)
-    auto ldc = [&](int t, double *bt, double *ct) {
+    auto ldb = [&](int t, double *bt) {
 #pragma unroll
-        for (int st = 0; st < CH; ++st) {
-            const int l = min(it.l0 + t + st, lmax);
-            bt[st] = qcm[l + 1].x;
-            ct[st] = qcm[l].y;
-        }
+        for (int st = 0; st < CH; ++st) bt[st] = qcm[min(it.l0 + t + st, lmax) + 1].x;
     };
-    double bcur[CH], ccur[CH];
-    ldc(0, bcur, ccur);
+    double bcur[CH];
+    ldb(0, bcur);
     for (int c0 = 0; c0 < nst; c0 += CH) {
-        double bnx[CH], cnx[CH];
-        ldc(c0 + CH, bnx, cnx);
+        double bnx[CH];
+        ldb(c0 + CH, bnx);
         if (it.fast) {
 #pragma unroll
             for (int st = 0; st < CH; ++st) {
@@ -511,7 +515,7 @@ __device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, con
                         pr = fma(cur[j], (st & 1) ? gmr[j] : gpr[j], pr);
                         pi = fma(cur[j], (st & 1) ? gmi[j] : gpi[j], pi);
                     }
-                    tb[st * 64 + lane] = make_double2(pr, pi);
+                    tb[st * ANS_THREADS + tid] = make_double2(pr, pi);
                     const double beta = bcur[st];
 #pragma unroll
                     for (int j = 0; j < K; ++j) {
@@ -529,11 +533,11 @@ __device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, con
                     double pr = 0.0, pi = 0.0;
 #pragma unroll
                     for (int j = 0; j < K; ++j) {
-                        const double v = cur[j] * cm[j];
+                        const double v = kk[j] == 0 ? cur[j] : 0.0;
                         pr = fma(v, (st & 1) ? gmr[j] : gpr[j], pr);
                         pi = fma(v, (st & 1) ? gmi[j] : gpi[j], pi);
                     }
-                    tb[st * 64 + lane] = make_double2(pr, pi);
+                    tb[st * ANS_THREADS + tid] = make_double2(pr, pi);
                     const double beta = bcur[st];
 #pragma unroll
                     for (int j = 0; j < K; ++j) {
@@ -543,69 +547,63 @@ __device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, con
                     }
                 }
             }
-            const double cl = ccur[min(CH, nst - c0) - 1];
+            const double cl = cseg[c0 + min(CH, nst - c0) - 1];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
                     cur[j] *= kDown;
                     prev[j] *= kDown;
                     kk[j] -= 1;
-                    cm[j] = kk[j] == 0 ? 1.0 : 0.0;
                 }
             }
         }
         __syncthreads();
-        const int li = lane >> 3, q = lane & 7;
+        // 8 steps x 128 partials: 16 threads per step
+        const int li = tid >> 4, q = tid & 15;
         double sr = 0.0, si = 0.0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const double2 v = tb[li * 64 + q + 8 * i];
+            const double2 v = tb[li * ANS_THREADS + q + 16 * i];
             sr += v.x;
             si += v.y;
         }
 #pragma unroll
-        for (int msk = 4; msk >= 1; msk >>= 1) {
-            sr += __shfl_xor(sr, msk, 8);
-            si += __shfl_xor(si, msk, 8);
+        for (int msk = 8; msk >= 1; msk >>= 1) {
+            sr += __shfl_xor(sr, msk, 16);
+            si += __shfl_xor(si, msk, 16);
         }
         const int t = c0 + li;
-        double cli = ccur[0];
-#pragma unroll
-        for (int st = 1; st < CH; ++st) cli = li == st ? ccur[st] : cli;
         if (q == 0 && t < nst) {
-            const double sc = gm * cli;
+            const double sc = gm * cseg[t];
             if (a.atomic) add_c(col, it.l0 + t, sc * sr, sc * si);
             else st_c(col, it.l0 + t, sc * sr, sc * si);
         }
         __syncthreads();
 #pragma unroll
-        for (int st = 0; st < CH; ++st) {
-            bcur[st] = bnx[st];
-            ccur[st] = cnx[st];
-        }
+        for (int st = 0; st < CH; ++st) bcur[st] = bnx[st];
     }
 }
 
 template <typename OT>
-__global__ __launch_bounds__(64) void k_anal_s(const LegArgs a) {
-    __shared__ double2 tb[8 * 64];
+__global__ __launch_bounds__(ANS_THREADS) void k_anal_s(const LegArgs a) {
+    __shared__ double2 tb[8 * ANS_THREADS];
+    __shared__ double cseg[SEG_MAX + 8];
     const Item it = make_item<512>(a);
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
     OT *col = (OT *)a.out0 + 2 * ((size_t)it.f * a.spec_fstride + (size_t)(a.lmax + 1) * it.m);
     const bool dead = it.lstop <= it.l0 || it.pstart >= it.pb1;
-    zero_fill(a, it, col, dead, lane);
+    if (tid < 64) zero_fill(a, it, col, dead, tid);
     if (dead) return;
     const double2 *__restrict__ qcm = a.qc + a.coff[it.im] - it.m;
-    const int keff = (it.pb1 - it.pstart + 63) / 64;
+    // c_l of this segment (l0 <= l < lstop; the tail up to a multiple of 8 is padded)
+    for (int t = tid; t < SEG_MAX + 8; t += ANS_THREADS) cseg[t] = qcm[min(it.l0 + t, a.lmax)].y;
+    __syncthreads();
+    const int keff = (it.pb1 - it.pstart + ANS_THREADS - 1) / ANS_THREADS;
     switch (keff) {
-        case 1: anal_s_run<1>(a, it, qcm, tb, col, lane); break;
-        case 2: anal_s_run<2>(a, it, qcm, tb, col, lane); break;
-        case 3: anal_s_run<3>(a, it, qcm, tb, col, lane); break;
-        case 4: anal_s_run<4>(a, it, qcm, tb, col, lane); break;
-        case 5: anal_s_run<5>(a, it, qcm, tb, col, lane); break;
-        case 6: anal_s_run<6>(a, it, qcm, tb, col, lane); break;
-        case 7: anal_s_run<7>(a, it, qcm, tb, col, lane); break;
-        default: anal_s_run<8>(a, it, qcm, tb, col, lane); break;
+        case 1: anal_s_run<1>(a, it, qcm, tb, cseg, col, tid); break;
+        case 2: anal_s_run<2>(a, it, qcm, tb, cseg, col, tid); break;
+        case 3: anal_s_run<3>(a, it, qcm, tb, cseg, col, tid); break;
+        default: anal_s_run<4>(a, it, qcm, tb, cseg, col, tid); break;
     }
 }
 
@@ -615,11 +613,13 @@ __global__ __launch_bounds__(64) void k_anal_s(const LegArgs a) {
 // (times 1/(l(l+1)) for the analysis proper).  Folded per pair: even l-m uses
 // (Gt-, Gp+, Gt+, Gp-) for (SD, SQ, TQ, TD), odd l-m the opposite signs.
 // ---------------------------------------------------------------------------
+constexpr int ANV_THREADS = 128;
+
 template <int K, typename OT>
 __device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, const double2 *__restrict__ qcm,
-                                           const double2 *__restrict__ qdm, double2 *tb, double2 *red, OT *cs,
-                                           OT *ct, int lane) {
-    constexpr int CH = 4;
+                                           const double2 *dseg, const double *cseg, double2 *tb, double2 *red,
+                                           OT *cs, OT *ct, int tid) {
+    constexpr int CH = 2;
     const int np = a.np, nlat = a.nlat;
     const size_t fo = ((size_t)it.f * a.nm + it.im) * nlat;
     double x[K], prev[K], cur[K], cm[K];
@@ -627,7 +627,7 @@ __device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, con
     int kk[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        const int p = it.pstart + lane + 64 * j;
+        const int p = it.pstart + tid + ANV_THREADS * j;
         const bool act = p < it.pb1;
         const int pc = act ? p : 0;
         x[j] = a.x[pc];
@@ -661,25 +661,23 @@ __device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, con
     const double mm = (double)it.m;
     const int nst = it.lstop - it.l0;
     const int lmax = a.lmax;
-    // per degree: beta_{l+1}, l c_l, g_l c_{l-1}, c_l -- scalar loads, one chunk ahead
-    auto ldc = [&](int t0, double4 *r) {
+    // beta_{l+1} through scalar loads one chunk ahead; (l c_l, g_l c_{l-1}) and
+    // c_l from the segment tables in LDS
+    auto ldb = [&](int t0, double *bt) {
 #pragma unroll
-        for (int st = 0; st < CH; ++st) {
-            const int l = min(it.l0 + t0 + st, lmax);
-            const double2 d = qdm[l];
-            r[st] = make_double4(qcm[l + 1].x, d.x, d.y, qcm[l].y);
-        }
+        for (int st = 0; st < CH; ++st) bt[st] = qcm[min(it.l0 + t0 + st, lmax) + 1].x;
     };
-    double4 rcur[CH];
-    ldc(0, rcur);
+    double bcur[CH];
+    ldb(0, bcur);
     for (int c0 = 0; c0 < nst; c0 += CH) {
-        double4 rnx[CH];
-        ldc(c0 + CH, rnx);
+        double bnx[CH];
+        ldb(c0 + CH, bnx);
 #pragma unroll
         for (int st = 0; st < CH; ++st) {
             const int t = c0 + st;
             if (t < nst) {
-                const double4 r = rcur[st];  // beta_{l+1}, l c_l, g_l c_{l-1}, c_l
+                const double2 d = dseg[t];  // l c_l, g_l c_{l-1}
+                const double4 r = make_double4(bcur[st], d.x, d.y, 0.0);
                 // even: SD<-Gt-, SQ<-Gp+, TQ<-Gt+, TD<-Gp- ; odd: swap signs
                 const int e = (st & 1) ? 0 : 2, o = (st & 1) ? 2 : 0;
                 double sdr = 0, sdi = 0, sqr = 0, sqi = 0, tqr = 0, tqi = 0, tdr = 0, tdi = 0;
@@ -700,16 +698,14 @@ __device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, con
                     prev[j] = cur[j];
                     cur[j] = nx;
                 }
-                tb[(st * 4 + 0) * 64 + lane] = make_double2(sdr, sdi);
-                tb[(st * 4 + 1) * 64 + lane] = make_double2(sqr, sqi);
-                tb[(st * 4 + 2) * 64 + lane] = make_double2(tqr, tqi);
-                tb[(st * 4 + 3) * 64 + lane] = make_double2(tdr, tdi);
+                tb[(st * 4 + 0) * ANV_THREADS + tid] = make_double2(sdr, sdi);
+                tb[(st * 4 + 1) * ANV_THREADS + tid] = make_double2(sqr, sqi);
+                tb[(st * 4 + 2) * ANV_THREADS + tid] = make_double2(tqr, tqi);
+                tb[(st * 4 + 3) * ANV_THREADS + tid] = make_double2(tdr, tdi);
             }
         }
         if (!it.fast && ((c0 + CH) & (SCALE_CHECK - 1)) == 0) {
-            double cl = rcur[0].w;
-#pragma unroll
-            for (int st = 1; st < CH; ++st) cl = st < nst - c0 ? rcur[st].w : cl;
+            const double cl = cseg[c0 + min(CH, nst - c0) - 1];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
@@ -722,31 +718,30 @@ __device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, con
         }
         __syncthreads();
         {
-            // 16 (step, quantity) combos x 64 partials: 4 lanes per combo
-            const int combo = lane >> 2, q = lane & 3;
+            // 8 (step, quantity) combos x 128 partials: 16 threads per combo
+            const int combo = tid >> 4, q = tid & 15;
             double sr = 0.0, si = 0.0;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const double2 v = tb[combo * 64 + q + 4 * i];
+            for (int i = 0; i < 8; ++i) {
+                const double2 v = tb[combo * ANV_THREADS + q + 16 * i];
                 sr += v.x;
                 si += v.y;
             }
-            sr += __shfl_xor(sr, 2, 4);
-            si += __shfl_xor(si, 2, 4);
-            sr += __shfl_xor(sr, 1, 4);
-            si += __shfl_xor(si, 1, 4);
+#pragma unroll
+            for (int msk = 8; msk >= 1; msk >>= 1) {
+                sr += __shfl_xor(sr, msk, 16);
+                si += __shfl_xor(si, msk, 16);
+            }
             if (q == 0) red[combo] = make_double2(sr, si);
         }
         __syncthreads();
-        if (lane < CH) {
-            const int t = c0 + lane;
+        if (tid < CH) {
+            const int t = c0 + tid;
             if (t < nst) {
                 const int l = it.l0 + t;
-                double cl = rcur[0].w;
-#pragma unroll
-                for (int st = 1; st < CH; ++st) cl = lane == st ? rcur[st].w : cl;
-                const double2 SD = red[lane * 4 + 0], SQ = red[lane * 4 + 1];
-                const double2 TQ = red[lane * 4 + 2], TD = red[lane * 4 + 3];
+                const double cl = cseg[t];
+                const double2 SD = red[tid * 4 + 0], SQ = red[tid * 4 + 1];
+                const double2 TQ = red[tid * 4 + 2], TD = red[tid * 4 + 3];
                 double sc = gm;
                 if (a.ll1_mode) sc *= l > 0 ? 1.0 / ((double)l * (l + 1)) : 0.0;
                 const double mc = mm * cl;
@@ -764,32 +759,38 @@ __device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, con
         }
         __syncthreads();
 #pragma unroll
-        for (int st = 0; st < CH; ++st) rcur[st] = rnx[st];
+        for (int st = 0; st < CH; ++st) bcur[st] = bnx[st];
     }
 }
 
 template <typename OT>
-__global__ __launch_bounds__(64) void k_anal_v(const LegArgs a) {
-    __shared__ double2 tb[4 * 4 * 64];
-    __shared__ double2 red[16];
+__global__ __launch_bounds__(ANV_THREADS) void k_anal_v(const LegArgs a) {
+    __shared__ double2 tb[2 * 4 * ANV_THREADS];
+    __shared__ double2 red[8];
+    __shared__ double2 dseg[SEG_MAX + 8];
+    __shared__ double cseg[SEG_MAX + 8];
     const Item it = make_item<256>(a);
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
     const size_t co = 2 * ((size_t)it.f * a.spec_fstride + (size_t)(a.lmax + 1) * it.m);
     OT *cs = (OT *)a.out0 + co;
     OT *ct = (OT *)a.out1 + co;
     const bool dead = it.lstop <= it.l0 || it.pstart >= it.pb1;
-    zero_fill(a, it, cs, dead, lane);
-    zero_fill(a, it, ct, dead, lane);
+    if (tid < 64) {
+        zero_fill(a, it, cs, dead, tid);
+        zero_fill(a, it, ct, dead, tid);
+    }
     if (dead) return;
     const double2 *__restrict__ qcm = a.qc + a.coff[it.im] - it.m;
     const double2 *__restrict__ qdm = a.qd + a.coff[it.im] - it.m;
-    const int keff = (it.pb1 - it.pstart + 63) / 64;
-    switch (keff) {
-        case 1: anal_v_run<1>(a, it, qcm, qdm, tb, red, cs, ct, lane); break;
-        case 2: anal_v_run<2>(a, it, qcm, qdm, tb, red, cs, ct, lane); break;
-        case 3: anal_v_run<3>(a, it, qcm, qdm, tb, red, cs, ct, lane); break;
-        default: anal_v_run<4>(a, it, qcm, qdm, tb, red, cs, ct, lane); break;
+    for (int t = tid; t < SEG_MAX + 8; t += ANV_THREADS) {
+        const int l = min(it.l0 + t, a.lmax);
+        dseg[t] = qdm[l];
+        cseg[t] = qcm[l].y;
     }
+    __syncthreads();
+    const int keff = (it.pb1 - it.pstart + ANV_THREADS - 1) / ANV_THREADS;
+    if (keff == 1) anal_v_run<1>(a, it, qcm, dseg, cseg, tb, red, cs, ct, tid);
+    else anal_v_run<2>(a, it, qcm, dseg, cseg, tb, red, cs, ct, tid);
 }
 
 
@@ -835,38 +836,51 @@ hipError_t launch_seed(const SeedArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_leg_synth(const LegArgs &a, int nfields, int vector, hipStream_t st) {
-    if (a.seg > SEG_MAX || (a.seg & 7)) return hipErrorInvalidValue;
-    constexpr int K = 1;
-    dim3 grid((a.np + 64 * K - 1) / (64 * K), a.nm, nfields);
+template <int K>
+static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
+    a.gx = (a.np + 64 * K - 1) / (64 * K);
+    a.gy = a.nm;
+    a.gz = nfields;
+    const dim3 grid(grid_for_units(a.xcd, a.gx, a.gy * a.gz));
     if (vector) {
         if (a.io_f32)
-            hipLaunchKernelGGL((k_synth_v<K, float>), grid, dim3(256), 0, st, a);
+            hipLaunchKernelGGL((k_synth_v<1, float>), grid, dim3(256), 0, st, a);
         else
-            hipLaunchKernelGGL((k_synth_v<K, double>), grid, dim3(256), 0, st, a);
+            hipLaunchKernelGGL((k_synth_v<1, double>), grid, dim3(256), 0, st, a);
     } else {
         if (a.io_f32)
             hipLaunchKernelGGL((k_synth_s<K, float>), grid, dim3(256), 0, st, a);
         else
             hipLaunchKernelGGL((k_synth_s<K, double>), grid, dim3(256), 0, st, a);
     }
+}
+
+hipError_t launch_leg_synth(const LegArgs &a, int nfields, int vector, int chains, hipStream_t st) {
+    if (a.seg > SEG_MAX || (a.seg & 7)) return hipErrorInvalidValue;
+    if (vector || chains <= 1) launch_synth_k<1>(a, nfields, vector, st);
+    else if (chains == 2) launch_synth_k<2>(a, nfields, vector, st);
+    else launch_synth_k<4>(a, nfields, vector, st);
     return hipGetLastError();
 }
 
-hipError_t launch_leg_anal(const LegArgs &a, int nfields, int vector, hipStream_t st) {
-    if (a.seg > SEG_MAX || (a.seg & 7)) return hipErrorInvalidValue;
+hipError_t launch_leg_anal(const LegArgs &a0, int nfields, int vector, hipStream_t st) {
+    if (a0.seg > SEG_MAX || (a0.seg & 7)) return hipErrorInvalidValue;
     const int pblk = vector ? 256 : 512;
-    dim3 grid(a.nseg_total, (a.np + pblk - 1) / pblk, nfields);
+    LegArgs a = a0;
+    a.gx = a.nseg_total;
+    a.gy = (a.np + pblk - 1) / pblk;
+    a.gz = nfields;
+    const dim3 grid((unsigned)a.gx * a.gy * nfields);
     if (vector) {
         if (a.io_f32)
-            hipLaunchKernelGGL(k_anal_v<float>, grid, dim3(64), 0, st, a);
+            hipLaunchKernelGGL(k_anal_v<float>, grid, dim3(ANV_THREADS), 0, st, a);
         else
-            hipLaunchKernelGGL(k_anal_v<double>, grid, dim3(64), 0, st, a);
+            hipLaunchKernelGGL(k_anal_v<double>, grid, dim3(ANV_THREADS), 0, st, a);
     } else {
         if (a.io_f32)
-            hipLaunchKernelGGL(k_anal_s<float>, grid, dim3(64), 0, st, a);
+            hipLaunchKernelGGL(k_anal_s<float>, grid, dim3(ANS_THREADS), 0, st, a);
         else
-            hipLaunchKernelGGL(k_anal_s<double>, grid, dim3(64), 0, st, a);
+            hipLaunchKernelGGL(k_anal_s<double>, grid, dim3(ANS_THREADS), 0, st, a);
     }
     return hipGetLastError();
 }

@@ -119,7 +119,8 @@ void free_plan(shtk_plan *p) {
     if (!p) return;
     void *ptrs[] = {p->x,      p->s,      p->wscaled, p->winvs,  p->winvs2, p->invs,  p->qc,  p->ab, p->gtab,
                     p->qd,     p->coff,   p->segbase, p->seedq0, p->seedq1, p->seedk, p->pfirst,
-                    p->pfast,  p->segm,   p->convscale, p->tw,   p->ws};
+                    p->pfast,  p->segm,   p->convscale, p->tw,   p->ws,     p->sectv, p->sectk, p->oz_q0,
+                    p->oz_q1,  p->oz_qk,  p->oz_ep,     p->oz_chbase, p->oz_ws};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -226,6 +227,7 @@ shtk::LegArgs base_args(const shtk_plan *p, int ltr) {
     a.pfirst = p->pfirst;
     a.pfast = p->pfast;
     a.spec_fstride = (long long)(p->d.lmax + 1) * (p->d.mmax + 1);
+    a.xcd = p->xcd_map;
     return a;
 }
 
@@ -264,6 +266,70 @@ int norm_ltr(const shtk_plan *p, int ltr, int *out) {
     return SHTK_OK;
 }
 
+// Split-precision tables: quarter seeds and per-pair exponents (one kernel,
+// first use only) and the chunk offsets of every order.
+int ensure_oz_tables(shtk_plan *p, hipStream_t st) {
+    if (p->oz_q0) return SHTK_OK;
+    const int np = p->np, nm = p->nm, lmax = p->d.lmax;
+    std::vector<int> chbase(nm);
+    int total = 0;
+    for (int im = 0; im < nm; ++im) {
+        chbase[im] = total;
+        total += (lmax - im * p->d.mres + 128) / 128;
+    }
+    const size_t nq = (size_t)nm * 4 * np;
+    HIP_TRY(hipMalloc((void **)&p->oz_q0, sizeof(double) * nq), "split tables");
+    HIP_TRY(hipMalloc((void **)&p->oz_q1, sizeof(double) * nq), "split tables");
+    HIP_TRY(hipMalloc((void **)&p->oz_qk, sizeof(int) * nq), "split tables");
+    HIP_TRY(hipMalloc((void **)&p->oz_ep, sizeof(int) * (size_t)nm * np), "split tables");
+    RC_TRY(upload(&p->oz_chbase, chbase, "split tables"));
+    p->oz_ch_total = total;
+    shtk::OzSeedArgs sa{};
+    sa.lmax = lmax;
+    sa.mres = p->d.mres;
+    sa.np = np;
+    sa.nm = nm;
+    sa.x = p->x;
+    sa.ab = p->ab;
+    sa.coff = p->coff;
+    sa.sectv = p->sectv;
+    sa.sectk = p->sectk;
+    sa.q0 = p->oz_q0;
+    sa.q1 = p->oz_q1;
+    sa.qk = p->oz_qk;
+    sa.ep = p->oz_ep;
+    HIP_TRY(shtk::launch_oz_seed(sa, st), "split seed kernel");
+    HIP_TRY(hipStreamSynchronize(st), "split seed kernel");
+    return SHTK_OK;
+}
+
+struct OzWs {
+    size_t bdig, colsum, eb, total;
+};
+
+OzWs oz_ws_layout(const shtk_plan *p, int nfg) {
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    OzWs L;
+    L.bdig = al((size_t)nfg * p->oz_ch_total * shtk::OZ_CHUNK_V4 * 16);
+    L.colsum = al(sizeof(double) * (size_t)nfg * p->nm * 2 * shtk::OZ_COLS);
+    L.eb = al(sizeof(int) * (size_t)nfg * p->nm * shtk::OZ_COLS);
+    L.total = L.bdig + L.colsum + L.eb;
+    return L;
+}
+
+int ensure_oz_ws(shtk_plan *p, size_t bytes) {
+    if (p->oz_ws_bytes >= bytes) return SHTK_OK;
+    if (p->oz_ws) {
+        HIP_TRY(hipDeviceSynchronize(), "split workspace resize sync");
+        HIP_TRY(hipFree(p->oz_ws), "split workspace free");
+        p->oz_ws = nullptr;
+        p->oz_ws_bytes = 0;
+    }
+    HIP_TRY(hipMalloc(&p->oz_ws, bytes), "split workspace allocation");
+    p->oz_ws_bytes = bytes;
+    return SHTK_OK;
+}
+
 // One transform direction with its scalings (see the entry points below).
 struct Job {
     int vector;     // 0 scalar, 1 spheroidal/toroidal pair
@@ -272,6 +338,7 @@ struct Job {
     double g;
     int mfac_mode, ll1_mode, fft_mode, ltr;
     const char *leg_name, *fft_name;
+    int prec = SHTK_PREC_FP64;
 };
 
 // spectral -> grid: [convert] -> Legendre synthesis -> inverse phi FFT
@@ -306,8 +373,47 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
     a.g = j.g;
     a.mfac_mode = j.mfac_mode;
     a.ll1_mode = j.ll1_mode;
-    HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_leg_synth(a, nf, j.vector, st); }),
-            "Legendre synthesis launch");
+    if (j.prec == SHTK_PREC_SPLIT) {
+        RC_TRY(ensure_oz_tables(p, st));
+        const int nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
+        const OzWs W = oz_ws_layout(p, nfg);
+        RC_TRY(ensure_oz_ws(p, W.total));
+        shtk::OzArgs o{};
+        o.lmax = p->d.lmax;
+        o.mmax = p->d.mmax;
+        o.mres = p->d.mres;
+        o.nlat = p->d.nlat;
+        o.np = p->np;
+        o.nm = p->nm;
+        o.ltr = j.ltr;
+        o.nf = nf;
+        o.nfg = nfg;
+        o.x = p->x;
+        o.ab = p->ab;
+        o.coff = p->coff;
+        o.q0 = p->oz_q0;
+        o.q1 = p->oz_q1;
+        o.qk = p->oz_qk;
+        o.ep = p->oz_ep;
+        o.chbase = p->oz_chbase;
+        o.ch_total = p->oz_ch_total;
+        o.spec_in = a.in0;
+        o.in_f32 = a.io_f32;
+        o.spec_fstride = a.spec_fstride;
+        char *ow = (char *)p->oz_ws;
+        o.bdig = (int4 *)ow;
+        o.colsum = (double *)(ow + W.bdig);
+        o.eb = (int *)(ow + W.bdig + W.colsum);
+        o.F0 = F0;
+        o.g = j.g;
+        o.xcd = p->xcd_map;
+        HIP_TRY(timed(p, "oz_synth", st, [&] { return shtk::launch_oz_synth(o, st); }),
+                "split synthesis launch");
+    } else {
+        HIP_TRY(timed(p, j.leg_name, st,
+                      [&] { return shtk::launch_leg_synth(a, nf, j.vector, p->synth_chains, st); }),
+                "Legendre synthesis launch");
+    }
     void *grids[2] = {g0, g1};
     for (int q = 0; q <= j.vector; ++q) {
         shtk::FftArgs fa = fft_args(p);
@@ -380,6 +486,14 @@ int segment_length() {
     int s = e ? std::atoi(e) : 128;
     if (s < 8 || s > shtk::SEG_MAX || (s & 7)) s = 128;
     return s;
+}
+
+// Launch-shape knobs read once per plan: SHTK_XCD=1 keeps the pair blocks of
+// an order on one XCD in the synthesis kernels, SHTK_SYNTH_CHAINS (1, 2 or 4)
+// sets the latitude pairs per lane of the scalar synthesis.
+int env_int(const char *name, int dflt) {
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
 }
 
 }  // namespace
@@ -554,8 +668,9 @@ static int plan_tables(shtk_plan *p) {
         if (e == hipSuccess) e = hipDeviceSynchronize();
         if (e != hipSuccess) rc = hip_fail(e, "seed kernel");
     }
-    if (dsectv) (void)hipFree(dsectv);
-    if (dsectk) (void)hipFree(dsectk);
+    // the sectoral seeds stay with the plan for the split-precision tables
+    p->sectv = dsectv;
+    p->sectk = dsectk;
     return rc;
 }
 
@@ -575,6 +690,8 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     p->np = (d.nlat + 1) / 2;
     p->nm = d.mmax / d.mres + 1;
     p->seg = segment_length();
+    p->xcd_map = env_int("SHTK_XCD", 0) != 0;
+    p->synth_chains = env_int("SHTK_SYNTH_CHAINS", 2);
     p->canonical = d.norm == SHTK_NORM_ORTHONORMAL && !d.real_norm && d.cs_phase;
     p->inv_scale = d.phi_scale == SHTK_PHI_QUAD ? d.nlon / (2.0 * M_PI) : (double)d.nlon;
     p->scale_phi = (2.0 * M_PI / d.nlon) * d.nlon / p->inv_scale;
@@ -667,6 +784,21 @@ int shtk_synthesis(shtk_plan *p, const void *alm, void *f, int nfields, int dtyp
     RC_TRY(norm_ltr(p, ltr, &ltr));
     // f = Re ifft(phi_inv_scale * F): the 1/nlon of the inverse FFT is folded in g
     Job j{0, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, 0, ltr, "leg_synth", "fft_synth"};
+    return run_to_grid(p, j, alm, nullptr, f, nullptr, nfields, dtype, stream);
+}
+
+int shtk_synthesis_ex(shtk_plan *p, const void *alm, void *f, int nfields, int dtype, int real_output, int ltr,
+                      int precision, void *stream) {
+    RC_TRY(check_common(p, nfields, dtype, {alm, f}));
+    if (real_output != 1)
+        return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is not implemented by this backend");
+    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT)
+        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
+    if (precision == SHTK_PREC_SPLIT && p->d.lmax > 4095)
+        return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    Job j{0, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, 0, ltr, "leg_synth", "fft_synth"};
+    j.prec = precision;
     return run_to_grid(p, j, alm, nullptr, f, nullptr, nfields, dtype, stream);
 }
 

@@ -142,6 +142,18 @@ def _nf(t: torch.Tensor, batch: bool) -> int:
     return int(t.shape[2]) if batch else 1
 
 
+_PRECISIONS = {"fp64": _lib.SHTK_PREC_FP64, "split": _lib.SHTK_PREC_SPLIT}
+
+
+def _precision(op, prec) -> int:
+    """Contraction precision keyword of the batch transforms: "fp64" (the plain
+    FP64 kernels) or "split" (Ozaki split on int8 matrix cores, FP64-grade)."""
+    try:
+        return _PRECISIONS[str(prec).lstrip(":").lower()]
+    except KeyError:
+        raise ValueError(f"{op}: precision must be one of {sorted(_PRECISIONS)}, got {prec!r}") from None
+
+
 def _check_kw(op, real_output=True, fft_scratch=None):
     if fft_scratch is not None and not (isinstance(fft_scratch, torch.Tensor) and fft_scratch.is_cuda):
         raise ValueError(f"{op}: fft_scratch must be device storage")
@@ -154,6 +166,7 @@ def _synthesis(args, kw, batch, op):
     real_output = kw.pop("real_output", True)
     kw.pop("use_rfft", None)
     ltr = kw.pop("ltr", -1)
+    prec = _precision(op, kw.pop("precision", "fp64"))
     _check_kw(op, real_output, kw.pop("fft_scratch", None))
     if kw:
         raise TypeError(f"{op}: unexpected keywords {sorted(kw)}")
@@ -162,8 +175,8 @@ def _synthesis(args, kw, batch, op):
     shape = (cfg.nlat, cfg.nlon) + ((nf,) if batch else ())
     out = _empty_colmajor(shape, rdt if real_output else _cplx(rdt), alm.device)
     lib = _lib.load()
-    _lib.check(lib.shtk_synthesis(ctypes.c_void_p(cfg.plan), _ptr(alm), _ptr(out), nf, _code(rdt),
-                                  1 if real_output else 0, int(ltr), _stream(alm.device)), op)
+    _lib.check(lib.shtk_synthesis_ex(ctypes.c_void_p(cfg.plan), _ptr(alm), _ptr(out), nf, _code(rdt),
+                                     1 if real_output else 0, int(ltr), prec, _stream(alm.device)), op)
     return out
 
 
@@ -196,7 +209,9 @@ def analysis(*args, **kw):
 
 
 def synthesis_batch(*args, **kw):
-    """``synthesis_batch(cfg, alm (lmax+1, mmax+1, nf); real_output=True)`` -> (nlat, nlon, nf)."""
+    """``synthesis_batch(cfg, alm (lmax+1, mmax+1, nf); real_output=True, precision="fp64")``
+    -> (nlat, nlon, nf).  ``precision="split"`` runs the shared-recurrence Ozaki
+    contraction on the int8 matrix cores (FP64-grade results)."""
     return _synthesis(args, dict(kw), True, "synthesis_batch")
 
 

@@ -107,6 +107,13 @@ int shtk_analysis(shtk_plan *plan, const void *f, void *alm, int nfields, int dt
 int shtk_synthesis_ex(shtk_plan *plan, const void *alm, void *f, int nfields, int dtype, int real_output,
                       int ltr, int precision, void *stream);
 
+/* Batched scalar analysis with a contraction precision: the batch analysis of
+ * src/batch_transforms.jl:337-429 (one Legendre row per (m, latitude) shared by
+ * all fields, :403-418).  SHTK_PREC_SPLIT accumulates with device atomics, so
+ * its last bits may differ from run to run. */
+int shtk_analysis_ex(shtk_plan *plan, const void *f, void *alm, int nfields, int dtype, int ltr, int precision,
+                     void *stream);
+
 /* Vector synthesis (S, T) -> (Vt, Vp): synthesis_sphtor / synthesis_sphtor_batch
  * (src/sphtor_transforms.jl:162-241, src/batch_transforms.jl:745-815), behind
  * _gpu_adapter_synthesis_sphtor (src/sphtor_transforms.jl:304-311). */

@@ -164,6 +164,9 @@ struct shtk_plan {
     int *sectk = nullptr;
     double *oz_q0 = nullptr, *oz_q1 = nullptr;  // [nm][4][np]
     int *oz_qk = nullptr, *oz_ep = nullptr, *oz_chbase = nullptr;
+    int *oz_emaxl = nullptr, *oz_epl = nullptr;  // per (order, degree), qc indexing
+    double *oz_asc = nullptr;
+    size_t qc_len = 0;
     int oz_ch_total = 0;
     void *oz_ws = nullptr;
     size_t oz_ws_bytes = 0;
@@ -199,9 +202,21 @@ struct OzArgs {
     int4 *bdig;             // [nfg][ch_total][1536] B operands
     double *colsum;         // [nfg][nm][2][32] sum over degrees of the integer field values, per parity
     int *eb;                // [nfg][nm][32] column exponents
-    double2 *F0;            // Fourier output [field][im][nlat]
+    double2 *F0;            // Fourier data [field][im][nlat] (synthesis output, analysis input)
     double g;
     int xcd;
+    // analysis
+    const double2 *qc;      // Q-form (beta_l, c_l) per order (offset coff[im])
+    const double *asc;      // c_l 2^(46 - epl_l) per order: digit scale of degree l
+    const int *epl;         // |P_l| < 2^epl over all pairs, per order
+    const double *seedq0, *seedq1;
+    const int *seedk, *segbase, *segm, *pfirst;
+    int seg, nseg_total, nks, nhalf;
+    const double *wlat;
+    double *spec_out;       // FP64 canonical coefficients, accumulated (pre-zeroed)
+    int4 *gdig;             // [nfg][nm][nks][2][6][2][64] B operands (pairs as k)
+    double *gcolsum;        // [nfg][nm][nks][2][32]
+    int *eg;                // [nfg][nm][2][32]
 };
 
 struct OzSeedArgs {
@@ -213,10 +228,21 @@ struct OzSeedArgs {
     const int *sectk;
     double *q0, *q1;
     int *qk, *ep;
+    int *emaxl;             // per order and degree: max over pairs of ilogb|P_l| (atomicMax)
+};
+
+struct OzAscArgs {
+    int n;                  // table length (all orders)
+    const double2 *qc;
+    const int *emaxl;
+    double *asc;
+    int *epl;
 };
 
 hipError_t launch_oz_seed(const OzSeedArgs &a, hipStream_t st);
 hipError_t launch_oz_synth(const OzArgs &a, hipStream_t st);
+hipError_t launch_oz_asc(const OzAscArgs &a, hipStream_t st);
+hipError_t launch_oz_anal(const OzArgs &a, hipStream_t st);
 
 // Kernel launchers (shtk_legendre.hip, shtk_fft.hip).
 hipError_t launch_seed(const SeedArgs &a, hipStream_t st);

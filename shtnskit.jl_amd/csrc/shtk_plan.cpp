@@ -120,7 +120,8 @@ void free_plan(shtk_plan *p) {
     void *ptrs[] = {p->x,      p->s,      p->wscaled, p->winvs,  p->winvs2, p->invs,  p->qc,  p->ab, p->gtab,
                     p->qd,     p->coff,   p->segbase, p->seedq0, p->seedq1, p->seedk, p->pfirst,
                     p->pfast,  p->segm,   p->convscale, p->tw,   p->ws,     p->sectv, p->sectk, p->oz_q0,
-                    p->oz_q1,  p->oz_qk,  p->oz_ep,     p->oz_chbase, p->oz_ws};
+                    p->oz_q1,  p->oz_qk,  p->oz_ep,     p->oz_chbase, p->oz_ws, p->oz_emaxl, p->oz_epl,
+                    p->oz_asc};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -284,6 +285,9 @@ int ensure_oz_tables(shtk_plan *p, hipStream_t st) {
     HIP_TRY(hipMalloc((void **)&p->oz_ep, sizeof(int) * (size_t)nm * np), "split tables");
     RC_TRY(upload(&p->oz_chbase, chbase, "split tables"));
     p->oz_ch_total = total;
+    RC_TRY(upload(&p->oz_emaxl, std::vector<int>(p->qc_len, -100000), "split tables"));
+    HIP_TRY(hipMalloc((void **)&p->oz_epl, sizeof(int) * p->qc_len), "split tables");
+    HIP_TRY(hipMalloc((void **)&p->oz_asc, sizeof(double) * p->qc_len), "split tables");
     shtk::OzSeedArgs sa{};
     sa.lmax = lmax;
     sa.mres = p->d.mres;
@@ -298,7 +302,15 @@ int ensure_oz_tables(shtk_plan *p, hipStream_t st) {
     sa.q1 = p->oz_q1;
     sa.qk = p->oz_qk;
     sa.ep = p->oz_ep;
+    sa.emaxl = p->oz_emaxl;
     HIP_TRY(shtk::launch_oz_seed(sa, st), "split seed kernel");
+    shtk::OzAscArgs aa{};
+    aa.n = (int)p->qc_len;
+    aa.qc = p->qc;
+    aa.emaxl = p->oz_emaxl;
+    aa.asc = p->oz_asc;
+    aa.epl = p->oz_epl;
+    HIP_TRY(shtk::launch_oz_asc(aa, st), "split scale kernel");
     HIP_TRY(hipStreamSynchronize(st), "split seed kernel");
     return SHTK_OK;
 }
@@ -307,12 +319,22 @@ struct OzWs {
     size_t bdig, colsum, eb, total;
 };
 
-OzWs oz_ws_layout(const shtk_plan *p, int nfg) {
+// synthesis: B operands per (field group, chunk); analysis: per (field group,
+// order, kstep of 64 pairs) plus an FP64 canonical accumulator when the output
+// is not canonical FP64
+OzWs oz_ws_layout(const shtk_plan *p, int nfg, bool analysis) {
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     OzWs L;
-    L.bdig = al((size_t)nfg * p->oz_ch_total * shtk::OZ_CHUNK_V4 * 16);
-    L.colsum = al(sizeof(double) * (size_t)nfg * p->nm * 2 * shtk::OZ_COLS);
-    L.eb = al(sizeof(int) * (size_t)nfg * p->nm * shtk::OZ_COLS);
+    const size_t nks = (size_t)(p->np + 63) / 64;
+    if (!analysis) {
+        L.bdig = al((size_t)nfg * p->oz_ch_total * shtk::OZ_CHUNK_V4 * 16);
+        L.colsum = al(sizeof(double) * (size_t)nfg * p->nm * 2 * shtk::OZ_COLS);
+        L.eb = al(sizeof(int) * (size_t)nfg * p->nm * shtk::OZ_COLS);
+    } else {
+        L.bdig = al((size_t)nfg * p->nm * nks * shtk::OZ_CHUNK_V4 * 16);
+        L.colsum = al(sizeof(double) * (size_t)nfg * p->nm * nks * 2 * shtk::OZ_COLS);
+        L.eb = al(sizeof(int) * (size_t)nfg * p->nm * 2 * shtk::OZ_COLS);
+    }
     L.total = L.bdig + L.colsum + L.eb;
     return L;
 }
@@ -376,7 +398,7 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
     if (j.prec == SHTK_PREC_SPLIT) {
         RC_TRY(ensure_oz_tables(p, st));
         const int nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
-        const OzWs W = oz_ws_layout(p, nfg);
+        const OzWs W = oz_ws_layout(p, nfg, false);
         RC_TRY(ensure_oz_ws(p, W.total));
         shtk::OzArgs o{};
         o.lmax = p->d.lmax;
@@ -460,6 +482,57 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
         a.out0 = spec;
         a.out1 = spec + L.spec_bytes;
         a.io_f32 = 0;
+    }
+    if (j.prec == SHTK_PREC_SPLIT) {
+        RC_TRY(ensure_oz_tables(p, st));
+        const int nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
+        const OzWs W = oz_ws_layout(p, nfg, true);
+        // FP64 canonical accumulator: the output itself, or the conversion workspace
+        const bool direct = p->canonical && dtype == SHTK_F64;
+        const size_t acc_bytes = direct ? 0 : (size_t)nf * a.spec_fstride * 16;
+        RC_TRY(ensure_oz_ws(p, W.total + acc_bytes));
+        char *ow = (char *)p->oz_ws;
+        double *acc = direct ? (double *)s0 : (double *)(ow + W.total);
+        HIP_TRY(shtk::launch_zero(acc, (size_t)nf * a.spec_fstride * 16, st), "output clear");
+        shtk::OzArgs o{};
+        o.lmax = p->d.lmax;
+        o.mmax = p->d.mmax;
+        o.mres = p->d.mres;
+        o.nlat = p->d.nlat;
+        o.np = p->np;
+        o.nm = p->nm;
+        o.ltr = j.ltr;
+        o.nf = nf;
+        o.nfg = nfg;
+        o.x = p->x;
+        o.coff = p->coff;
+        o.spec_fstride = a.spec_fstride;
+        o.F0 = F0;
+        o.g = j.g;
+        o.qc = p->qc;
+        o.asc = p->oz_asc;
+        o.epl = p->oz_epl;
+        o.seedq0 = p->seedq0;
+        o.seedq1 = p->seedq1;
+        o.seedk = p->seedk;
+        o.segbase = p->segbase;
+        o.segm = p->segm;
+        o.pfirst = p->pfirst;
+        o.seg = p->seg;
+        o.nseg_total = p->nseg_total;
+        o.nks = (p->np + 63) / 64;
+        o.nhalf = (o.nks + 3) / 4;
+        o.wlat = j.wlat;
+        o.spec_out = acc;
+        o.gdig = (int4 *)ow;
+        o.gcolsum = (double *)(ow + W.bdig);
+        o.eg = (int *)(ow + W.bdig + W.colsum);
+        HIP_TRY(timed(p, "oz_anal", st, [&] { return shtk::launch_oz_anal(o, st); }), "split analysis launch");
+        if (!direct)
+            HIP_TRY(shtk::launch_spec_scale(acc, s0, 0, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale,
+                                            j.conv_div, p->d.lmax, p->d.mmax, p->d.mres, a.spec_fstride, nf, st),
+                    "coefficient conversion");
+        return SHTK_OK;
     }
     const int pblk = j.vector ? 256 : 512;
     a.atomic = p->np > pblk;
@@ -625,6 +698,7 @@ static int plan_tables(shtk_plan *p) {
     RC_TRY(upload(&p->winvs2, winvs2, "plan tables"));
     RC_TRY(upload(&p->invs, invs, "plan tables"));
     RC_TRY(upload(&p->qc, qc, "plan tables"));
+    p->qc_len = qc.size();
     RC_TRY(upload(&p->qd, qd, "plan tables"));
     RC_TRY(upload(&p->ab, ab, "plan tables"));
     RC_TRY(upload(&p->gtab, gt, "plan tables"));
@@ -800,6 +874,19 @@ int shtk_synthesis_ex(shtk_plan *p, const void *alm, void *f, int nfields, int d
     Job j{0, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, 0, ltr, "leg_synth", "fft_synth"};
     j.prec = precision;
     return run_to_grid(p, j, alm, nullptr, f, nullptr, nfields, dtype, stream);
+}
+
+int shtk_analysis_ex(shtk_plan *p, const void *f, void *alm, int nfields, int dtype, int ltr, int precision,
+                     void *stream) {
+    RC_TRY(check_common(p, nfields, dtype, {alm, f}));
+    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT)
+        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
+    if (precision == SHTK_PREC_SPLIT && p->d.lmax > 4095)
+        return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    Job j{0, 1, p->wscaled, 1.0, 0, 0, 0, ltr, "leg_anal", "fft_anal"};
+    j.prec = precision;
+    return run_to_spec(p, j, f, nullptr, alm, nullptr, nfields, dtype, stream);
 }
 
 int shtk_analysis(shtk_plan *p, const void *f, void *alm, int nfields, int dtype, int ltr, void *stream) {

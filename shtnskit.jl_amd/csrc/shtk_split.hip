@@ -38,26 +38,34 @@ __device__ __forceinline__ int oz_nch(int lmax, int m) { return (lmax - m + 128)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_oz_seed(const OzSeedArgs a) {
     const int im = blockIdx.y;
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= a.np) return;
+    const int p0 = blockIdx.x * 256 + threadIdx.x;
+    const bool act = p0 < a.np;
+    const int p = act ? p0 : 0;
     const int m = im * a.mres, lmax = a.lmax;
     const int Kq = 32 * oz_nch(lmax, m);
     const double x = a.x[p];
     const double2 *ab = a.ab + a.coff[im] - m;
+    int *emaxl = a.emaxl + a.coff[im] - m;
     const size_t sidx = (size_t)im * a.np + p;
-    double cur = a.sectv[sidx], prev = 0.0;
-    int k = a.sectk[sidx];
+    double cur = act ? a.sectv[sidx] : 0.0, prev = 0.0;
+    int k = act ? a.sectk[sidx] : 0;
     int emax = -100000;
     int g = 0;
     for (int l = m; l <= lmax; ++l) {
-        if (l == m + g * Kq) {
+        if (act && l == m + g * Kq) {
             const size_t q = ((size_t)im * 4 + g) * a.np + p;
             a.q0[q] = prev;
             a.q1[q] = cur;
             a.qk[q] = k;
             ++g;
         }
-        if (k == 0 && cur != 0.0) emax = max(emax, ilogb(cur));
+        const int el = (act && k == 0 && cur != 0.0) ? ilogb(cur) : -100000;
+        emax = max(emax, el);
+        // wavefront maximum per degree (all lanes run the same degrees)
+        int wm = el;
+#pragma unroll
+        for (int msk = 32; msk >= 1; msk >>= 1) wm = max(wm, __shfl_xor(wm, msk));
+        if ((threadIdx.x & 63) == 0 && wm > -100000) atomicMax(&emaxl[l], wm);
         const double2 co = ab[l + 1];
         const double nx = fma(co.x * x, cur, -(co.y * prev));
         prev = cur;
@@ -68,6 +76,7 @@ __global__ __launch_bounds__(256) void k_oz_seed(const OzSeedArgs a) {
             --k;
         }
     }
+    if (!act) return;
     for (; g < 4; ++g) {
         const size_t q = ((size_t)im * 4 + g) * a.np + p;
         a.q0[q] = 0.0;
@@ -295,7 +304,235 @@ __global__ __launch_bounds__(256) void k_synth_oz(const OzArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Per-degree digit scales for the analysis: |P_l| < 2^epl over all pairs, and
+// asc = c_l 2^(46 - epl) so that one FMA turns Q_l into the digit value.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_oz_asc(const OzAscArgs a) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const int e = a.emaxl[i];
+    const int ep = e < -201 ? -200 : min(e + 1, 64);
+    a.epl[i] = ep;
+    a.asc[i] = a.qc[i].y * ldexp(1.0, 46 - ep);
+}
+
+// ---------------------------------------------------------------------------
+// Analysis field operand: one block per (order, field group).  G+- =
+// w (F_N +- F_S) per pair (even degrees use G+, odd G-), column exponents eg
+// per parity, balanced digits in the B-operand layout
+//   [kstep][parity][slice t][column tile ct][lane][16 bytes],
+// lane = c + 16 g, byte j <-> pair 64 kstep + 16 g + j; per-kstep column sums.
+// ---------------------------------------------------------------------------
+constexpr int OZ_MAX_KS = 32;  // np <= 2048
+
+__global__ __launch_bounds__(256) void k_oz_gdig(const OzArgs a) {
+    __shared__ double cmax[2][256];
+    __shared__ int egs[2][OZ_COLS];
+    __shared__ unsigned long long csum[OZ_MAX_KS][2][OZ_COLS];
+    const int im = blockIdx.x % a.nm, fg = blockIdx.x / a.nm;
+    const int np = a.np, nlat = a.nlat, nks = a.nks;
+    const int tid = threadIdx.x;
+    auto gval = [&](int c, int p, int par) -> double {
+        const int f = fg * OZ_FIELDS + (c >> 1);
+        if (f >= a.nf || p >= np) return 0.0;
+        const double2 *Fi = a.F0 + ((size_t)f * a.nm + im) * nlat;
+        const double2 FN = Fi[p];
+        const int ps = nlat - 1 - p;
+        const double2 FS = ps != p ? Fi[ps] : make_double2(0.0, 0.0);
+        const double n = (c & 1) ? FN.y : FN.x, s = (c & 1) ? FS.y : FS.x;
+        const double wl = a.wlat[p];
+        return par ? wl * (n - s) : wl * (n + s);
+    };
+    {
+        const int c = tid & (OZ_COLS - 1), sub = tid >> 5;
+        double m0 = 0.0, m1 = 0.0;
+        for (int p = sub; p < np; p += 8) {
+            m0 = fmax(m0, fabs(gval(c, p, 0)));
+            m1 = fmax(m1, fabs(gval(c, p, 1)));
+        }
+        cmax[0][tid] = m0;
+        cmax[1][tid] = m1;
+    }
+    for (int i = tid; i < nks * 2 * OZ_COLS; i += 256) (&csum[0][0][0])[i] = 0ull;
+    __syncthreads();
+    if (tid < 2 * OZ_COLS) {
+        const int par = tid >> 5, c = tid & 31;
+        double v = 0.0;
+        for (int s = 0; s < 8; ++s) v = fmax(v, cmax[par][s * OZ_COLS + c]);
+        egs[par][c] = v > 0.0 ? ilogb(v) + 1 : 0;
+    }
+    __syncthreads();
+    const size_t ubase = ((size_t)fg * a.nm + im) * nks;
+    unsigned *dst = (unsigned *)(a.gdig + ubase * OZ_CHUNK_V4);
+    const int units = nks * 1024;
+    for (int u = tid; u < units; u += 256) {
+        const int d = u & 3, lane = (u >> 2) & 63, ct = (u >> 8) & 1, par = (u >> 9) & 1, ks = u >> 10;
+        const int g = lane >> 4, c = (lane & 15) + 16 * ct;
+        const double sc = ldexp(1.0, 46 - egs[par][c]);
+        unsigned dig[OZ_SLICES];
+#pragma unroll
+        for (int t = 0; t < OZ_SLICES; ++t) dig[t] = 0u;
+        long long ysum = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int p = 64 * ks + 16 * g + 4 * d + q;
+            long long Y = llrint(gval(c, p, par) * sc);
+            ysum += Y;
+#pragma unroll
+            for (int t = OZ_SLICES - 1; t >= 0; --t) {
+                const long long dd = ((Y + 128) & 255) - 128;
+                Y = (Y - dd) >> 8;
+                dig[t] |= ((unsigned)dd & 255u) << (8 * q);
+            }
+        }
+        atomicAdd(&csum[ks][par][c], (unsigned long long)ysum);
+        const size_t k0 = (size_t)ks * OZ_CHUNK_V4 * 4;
+#pragma unroll
+        for (int t = 0; t < OZ_SLICES; ++t)
+            dst[k0 + ((size_t)((par * OZ_SLICES + t) * 2 + ct) * 64 + lane) * 4 + d] = dig[t];
+    }
+    __syncthreads();
+    for (int i = tid; i < nks * 2 * OZ_COLS; i += 256)
+        a.gcolsum[ubase * 2 * OZ_COLS + i] = (double)(long long)(&csum[0][0][0])[i];
+    if (tid < 2 * OZ_COLS) a.eg[((size_t)fg * a.nm + im) * 2 * OZ_COLS + tid] = egs[tid >> 5][tid & 31];
+}
+
+// ---------------------------------------------------------------------------
+// Analysis: one block of 4 waves per (degree segment, half of the pairs,
+// field group).  Every lane runs the Q-form recurrence of one pair from the
+// segment seeds and writes the 6 digit bytes of each degree to LDS as
+// [parity][slice][row][pair] (row = degree pair index within 32 degrees);
+// wave w then contracts parity w >> 1, column tile w & 1 over the block's 4
+// ksteps of 64 pairs (the field operand stays in registers for the whole
+// segment) and adds its rows to the FP64 output.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_anal_oz(const OzArgs a) {
+    __shared__ __align__(16) unsigned char Alds[2 * OZ_SLICES * 16 * 256];
+    const int item = blockIdx.x;
+    const int h = item % a.nhalf, gs = (item / a.nhalf) % a.nseg_total, fg = item / (a.nhalf * a.nseg_total);
+    const int im = a.segm[gs], s = gs - a.segbase[im], m = im * a.mres;
+    const int l0 = m + s * a.seg;
+    const int lstop = min(min(l0 + a.seg, a.lmax + 1), a.ltr + 1);
+    const int nst = lstop - l0;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int pbase = h * 256;
+    if (nst <= 0 || pbase + 255 < a.pfirst[gs]) return;
+    const int p = pbase + tid;
+    const bool act = p < a.np;
+    const double x = act ? a.x[p] : 0.0;
+    const size_t q = (size_t)gs * a.np + (act ? p : 0);
+    double prev = act ? a.seedq0[q] : 0.0, cur = act ? a.seedq1[q] : 0.0;
+    int kk = act ? a.seedk[q] : 0;
+    const double2 *qcm = a.qc + a.coff[im] - m;
+    const double *ascm = a.asc + a.coff[im] - m;
+    const int *eplm = a.epl + a.coff[im] - m;
+
+    // field operand of this wave: parity wp, column tile wct, 4 ksteps x 6 slices
+    const int wp = w >> 1, wct = w & 1;
+    v4i Bw[4][OZ_SLICES];
+    const size_t ubase = ((size_t)fg * a.nm + im) * a.nks;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        const int kg = 4 * h + ks;
+#pragma unroll
+        for (int t = 0; t < OZ_SLICES; ++t)
+            Bw[ks][t] = kg < a.nks ? ((const v4i *)a.gdig)[(ubase + kg) * OZ_CHUNK_V4 +
+                                                           ((wp * OZ_SLICES + t) * 2 + wct) * 64 + lane]
+                                   : v4i{0, 0, 0, 0};
+    }
+    const int c = (lane & 15) + 16 * wct;
+    double csum = 0.0;
+    for (int ks = 0; ks < 4; ++ks) {
+        const int kg = 4 * h + ks;
+        if (kg < a.nks) csum += a.gcolsum[((ubase + kg) * 2 + wp) * OZ_COLS + c];
+    }
+    const int egc = a.eg[(((size_t)fg * a.nm + im) * 2 + wp) * OZ_COLS + c];
+    const int f = fg * OZ_FIELDS + (c >> 1);
+
+    for (int t0 = 0; t0 < nst; t0 += 32) {
+        // recurrence and digits of 32 degrees
+#pragma unroll
+        for (int blk = 0; blk < 4; ++blk) {
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq) {
+                const int i = blk * 8 + qq;
+                const int t = t0 + i;
+                const int l = l0 + min(t, nst - 1);
+                const double v = (kk == 0 && t < nst) ? cur : 0.0;
+                const double y = fma(v, ascm[l], 0x1p47);
+                const double hi = floor(y * 0x1p-24);
+                const double lw = fma(hi, -0x1p24, y);
+                const unsigned uh = (unsigned)hi ^ 0x808080u, ul = (unsigned)lw ^ 0x808080u;
+                unsigned char *row = Alds + ((i & 1) * OZ_SLICES * 16 + (i >> 1)) * 256 + tid;
+                row[0 * 16 * 256] = (unsigned char)(uh >> 16);
+                row[1 * 16 * 256] = (unsigned char)(uh >> 8);
+                row[2 * 16 * 256] = (unsigned char)uh;
+                row[3 * 16 * 256] = (unsigned char)(ul >> 16);
+                row[4 * 16 * 256] = (unsigned char)(ul >> 8);
+                row[5 * 16 * 256] = (unsigned char)ul;
+                const double beta = qcm[l + 1].x;
+                const double nx = fma(x, cur, -(beta * prev));
+                prev = cur;
+                cur = nx;
+            }
+            if (kk > 0 && fmax(fabs(cur), fabs(prev)) * qcm[l0 + min(t0 + blk * 8 + 7, nst - 1)].y > kHiS) {
+                cur *= kDownS;
+                prev *= kDownS;
+                --kk;
+            }
+        }
+        __syncthreads();
+        v4i acc[OZ_SLICES];
+#pragma unroll
+        for (int v = 0; v < OZ_SLICES; ++v) acc[v] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            v4i Aop[OZ_SLICES];
+#pragma unroll
+            for (int sl = 0; sl < OZ_SLICES; ++sl)
+                Aop[sl] = *(const v4i *)(Alds + ((wp * OZ_SLICES + sl) * 16 + (lane & 15)) * 256 + 64 * ks +
+                                         16 * (lane >> 4));
+#pragma unroll
+            for (int t = 0; t < OZ_SLICES; ++t)
+#pragma unroll
+                for (int sl = 0; sl + t < OZ_SLICES; ++sl)
+                    acc[sl + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aop[sl], Bw[ks][t], acc[sl + t], 0, 0, 0);
+        }
+        // rows 4 (lane >> 4) + i are degree pairs, column c
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * (lane >> 4) + i;
+            const int t = t0 + 2 * row + wp;
+            if (t < nst && f < a.nf) {
+                const int l = l0 + t;
+                double E = kBias * csum;
+#pragma unroll
+                for (int v = 0; v < OZ_SLICES; ++v) E = fma((double)acc[v][i], ldexp(1.0, 80 - 8 * v), E);
+                const double val = E * a.g * ldexp(1.0, eplm[l] - 46 + egc - 46);
+                atomicAdd(a.spec_out + 2 * ((size_t)f * a.spec_fstride + (size_t)(a.lmax + 1) * m + l) + (c & 1),
+                          val);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+hipError_t launch_oz_asc(const OzAscArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(k_oz_asc, dim3((a.n + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_oz_anal(const OzArgs &a, hipStream_t st) {
+    if (a.nks > OZ_MAX_KS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_oz_gdig, dim3((unsigned)a.nm * a.nfg), dim3(256), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_anal_oz, dim3((unsigned)a.nhalf * a.nseg_total * a.nfg), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_oz_seed(const OzSeedArgs &a, hipStream_t st) {
     dim3 grid((a.np + 255) / 256, a.nm);

@@ -185,6 +185,7 @@ def _analysis(args, kw, batch, op):
     cfg, f = args
     kw.pop("use_rfft", None)
     ltr = kw.pop("ltr", -1)
+    prec = _precision(op, kw.pop("precision", "fp64"))
     _check_kw(op, True, kw.pop("fft_scratch", None))
     if kw:
         raise TypeError(f"{op}: unexpected keywords {sorted(kw)}")
@@ -193,8 +194,8 @@ def _analysis(args, kw, batch, op):
     shape = (cfg.lmax + 1, cfg.mmax + 1) + ((nf,) if batch else ())
     out = _empty_colmajor(shape, _cplx(rdt), f.device)
     lib = _lib.load()
-    _lib.check(lib.shtk_analysis(ctypes.c_void_p(cfg.plan), _ptr(f), _ptr(out), nf, _code(rdt), int(ltr),
-                                 _stream(f.device)), op)
+    _lib.check(lib.shtk_analysis_ex(ctypes.c_void_p(cfg.plan), _ptr(f), _ptr(out), nf, _code(rdt), int(ltr), prec,
+                                    _stream(f.device)), op)
     return out
 
 
@@ -216,7 +217,8 @@ def synthesis_batch(*args, **kw):
 
 
 def analysis_batch(*args, **kw):
-    """``analysis_batch(cfg, fields (nlat, nlon, nf))`` -> (lmax+1, mmax+1, nf)."""
+    """``analysis_batch(cfg, fields (nlat, nlon, nf); precision="fp64")`` -> (lmax+1, mmax+1, nf).
+    ``precision="split"`` contracts on the int8 matrix cores (FP64-grade results)."""
     return _analysis(args, dict(kw), True, "analysis_batch")
 
 

@@ -78,3 +78,61 @@ def test_split_conventions_and_f32(shtk, oracle, cuda):
     got32 = to_host(shtk.synthesis_batch(cfg, to_dev(alm.astype(np.complex64)), precision="split"))
     assert got32.dtype == np.float32
     assert max(_per_field(got32.astype(np.float64), ref)) < 1e-6
+
+
+@pytest.mark.parametrize("lmax,nlat,nlon,nf", [
+    (3, 8, 10, 1),
+    (63, 64, 128, 3),
+    (100, 128, 256, 17),
+    (300, 302, 640, 20),
+])
+def test_split_analysis_matches_oracle(shtk, oracle, cuda, lmax, nlat, nlon, nf):
+    """Split analysis of a non-band-limited grid (random samples, so the
+    quadrature itself is exercised) against the oracle."""
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+    ocfg = oracle.Config(lmax, nlat, nlon=nlon)
+    rng = np.random.default_rng(lmax + nf)
+    f = np.asfortranarray(rng.standard_normal((nlat, nlon, nf)))
+    ref = ocfg.analysis(f)
+    got = to_host(shtk.analysis_batch(cfg, to_dev(f), precision="split"))
+    errs = _per_field(got, ref)
+    print(f"split analysis lmax={lmax} nf={nf}: max per-field rel L2 {max(errs):.2e}")
+    assert max(errs) < TOL_SPLIT, errs
+
+
+def test_split_round_trip_lmax1023(shtk, oracle, cuda):
+    """Split synthesis then split analysis at lmax 1023 returns the spectrum
+    (band-limited round trip) and matches the oracle analysis of the grid."""
+    lmax, nlat, nlon, nf = 1023, 1024, 2048, 16
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+    ocfg = oracle.Config(lmax, nlat, nlon=nlon)
+    alm = oracle.random_alm(lmax, lmax, seed=2, power=1.5, nf=nf)
+    f = shtk.synthesis_batch(cfg, to_dev(alm), precision="split")
+    back = to_host(shtk.analysis_batch(cfg, f, precision="split"))
+    errs = _per_field(back, alm)
+    print(f"split round trip lmax=1023: max per-field rel L2 {max(errs):.2e}")
+    assert max(errs) < TOL_SPLIT
+    fh = to_host(f)
+    e = rel_l2(back[..., 3], ocfg.analysis(fh[..., 3]))
+    assert e < TOL_SPLIT
+
+
+def test_split_analysis_conventions_ltr_f32(shtk, oracle, cuda):
+    lmax, nlat, nlon, nf = 40, 42, 96, 3
+    kw = dict(nlon=nlon, norm="schmidt", cs_phase=False, real_norm=True)
+    cfg = shtk.create_gauss_config(lmax, nlat, **kw)
+    ocfg = oracle.Config(lmax, nlat, **kw)
+    rng = np.random.default_rng(8)
+    f = np.asfortranarray(rng.standard_normal((nlat, nlon, nf)))
+    ref = ocfg.analysis(f)
+    got = to_host(shtk.analysis_batch(cfg, to_dev(f), precision="split"))
+    assert max(_per_field(got, ref)) < TOL_SPLIT
+    got32 = to_host(shtk.analysis_batch(cfg, to_dev(f.astype(np.float32)), precision="split"))
+    assert got32.dtype == np.complex64
+    ref32 = ocfg.analysis(f.astype(np.float32).astype(np.float64))
+    assert max(_per_field(got32.astype(np.complex128), ref32)) < 1e-6
+    cfg2 = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+    a = to_host(shtk.analysis_batch(cfg2, to_dev(f), precision="fp64", ltr=25))
+    b = to_host(shtk.analysis_batch(cfg2, to_dev(f), precision="split", ltr=25))
+    assert rel_l2(b, a) < TOL_SPLIT
+    assert np.all(b[26:] == 0)

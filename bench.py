@@ -1,31 +1,37 @@
 #!/usr/bin/env python3
 """Round-trip spherical harmonic transform benchmark on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): plain FP64
-round trip -- synthesis then analysis -- on the Gauss grid lmax = mmax = 1023,
-nlat = 1024, nlon = 2048, one field, orthonormal + Condon-Shortley, input
-alm complex Gaussian (seed 2) scaled by (l+1)^-1.5 with m = 0 imaginary parts
-zero.  Inputs are resident in HBM before the timed region starts.
+Headline workload: plain FP64 round trip -- synthesis then analysis -- on the
+Gauss grid lmax = mmax = 1023, nlat = 1024, nlon = 2048, one field,
+orthonormal + Condon-Shortley, input alm complex Gaussian (seed 2) scaled by
+(l+1)^-1.5 with m = 0 imaginary parts zero.  A step is one round trip and
+`value` is round trips per second.
 
-A step is one round trip.  `value` is round trips per second.  The roofline
-entry is for the dominant kernel, timed with HIP events on the stream the
-library launches it on, inside the timed region.  The CPU baseline is the
-oracle's restatement of the reference CPU path (threads over m, per-step
-recurrence coefficients, no folding) on the host cores.
+Second measured line (`split_batched`): the same grid, a batch of 16 fields
+(seed 2, same spectrum), synthesis_batch then analysis_batch with the
+split-precision contraction on the int8 matrix cores; its value is field round
+trips per second of the whole batch, with the per-field relative L2 error
+against the FP64 path.
 
-HBM traffic of the dominant kernel is not measured live: it comes from two
-rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over this same script, scaled
-by what the counters read for a plain device copy of known size (run with
+Inputs are resident in HBM before every timed region.  Per-kernel durations
+come from HIP events that the library records on its launch stream inside the
+timed region (shtk_profile_*); the roofline entries are for the dominant kernel
+of each line.  The CPU baseline is the oracle's restatement of the reference
+CPU path (threads over m, per-step recurrence coefficients, no folding) on the
+host cores.
+
+HBM traffic of a dominant kernel is not measured live: it comes from two
+rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over this script, scaled by
+what the counters read for a plain device copy of known size (run with
 --calibrate-copy so that copy is in the same trace), and is read back from
-profiles/hbm_traffic.json when that file covers the dominant kernel.
+profiles/hbm_traffic.json when that file covers the kernel.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--calibrate-copy]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-split] [--calibrate-copy]
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import statistics
 import sys
@@ -36,20 +42,35 @@ sys.path.insert(0, ROOT)
 
 METRIC = "round-trip SHT/s at lmax=1023: plain FP64 vs batched split-FP32 at ≤1e-11 rel err"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix peaks coincide)
+PEAK_INT8_TOPS = 5000.0  # MI355X dense int8 matrix
 PEAK_HBM_GBS = 8000.0
+SPLIT_PRODUCTS = 21  # slice products of the 6-slice Ozaki split (digit orders k + t <= 5)
+LMAX, NLAT, NLON = 1023, 1024, 2048
+SPLIT_FIELDS = 16
+
+
+def n_triples(lmax: int, nlat: int) -> float:
+    """Folded Legendre triples T = (nlat/2) * (lmax+1)(lmax+2)/2 per field and direction."""
+    return (nlat / 2) * (lmax + 1) * (lmax + 2) / 2
 
 
 def algorithmic_flops(lmax: int, nlat: int) -> float:
-    """Folded Legendre triples T = (nlat/2) * N_lm; one direction costs 8 flops per
-    triple (4 for the recurrence step, 4 for the complex-by-real contraction)."""
-    nlm = (lmax + 1) * (lmax + 2) // 2
-    return 8.0 * (nlat / 2) * nlm
+    """One direction of the plain transform: 8 flops per folded triple (4 for
+    the recurrence step, 4 for the complex-by-real contraction)."""
+    return 8.0 * n_triples(lmax, nlat)
 
 
-def fft_bytes(nlat: int, nlon: int, mmax: int) -> float:
+def split_int8_ops(lmax: int, nlat: int, nfields: int) -> float:
+    """One direction of the split contraction: each of the 21 slice products is
+    a real GEMM over every folded triple and real column, 2 ops per multiply-add
+    and 2 real columns per field."""
+    return SPLIT_PRODUCTS * 2.0 * n_triples(lmax, nlat) * 2 * nfields
+
+
+def fft_bytes(nlat: int, nlon: int, mmax: int, nfields: int = 1) -> float:
     """Compulsory HBM bytes of one phi-FFT launch: the real grid and the
     half-spectrum intermediate, read once and written once."""
-    return 8.0 * nlat * nlon + 16.0 * nlat * (mmax + 1)
+    return nfields * (8.0 * nlat * nlon + 16.0 * nlat * (mmax + 1))
 
 
 def measured_traffic(kernel: str):
@@ -91,12 +112,78 @@ def cpu_baseline(lmax, nlat, nlon, budget_s=15.0):
                       f"OpenMP threads over m in balanced order, FP64, per-step recurrence coefficients"}
 
 
+def spectrum(np, nfields, seed=2):
+    """Complex Gaussian alm (lmax+1, mmax+1[, nf]) scaled by (l+1)^-1.5, l < m and
+    m = 0 imaginary parts zero."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shape = (LMAX + 1, LMAX + 1) + ((nfields,) if nfields > 1 else ())
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    l = np.arange(LMAX + 1).reshape((-1, 1) + ((1,) if nfields > 1 else ()))
+    m = np.arange(LMAX + 1).reshape((1, -1) + ((1,) if nfields > 1 else ()))
+    a = np.where(l >= m, a * (l + 1.0) ** -1.5, 0)
+    a[:, 0, ...] = a[:, 0, ...].real
+    return a
+
+
+def to_device(torch, np, a, dev):
+    """numpy array -> device tensor with Julia column-major memory."""
+    perm = tuple(range(a.ndim - 1, -1, -1))
+    return torch.from_numpy(np.ascontiguousarray(a.transpose(perm))).to(dev).permute(*perm)
+
+
+def timed_steps(torch, _lib, cfg, step, steps, stream):
+    """K steps bracketed by synchronisations, with the library's per-kernel
+    events on; then the same K steps without them.  Returns (ms with events,
+    ms without, {kernel: (total_ms, count)}, last output)."""
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    _lib.profile_enable(cfg.plan, True)
+    _lib.profile_read(cfg.plan)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        out = step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    with_ev = e0.elapsed_time(e1)
+    kern = _lib.profile_read(cfg.plan)
+    _lib.profile_enable(cfg.plan, False)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        out = step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return with_ev, e0.elapsed_time(e1), kern, out
+
+
+def roofline_entry(kernel, avg_us, work, peak, unit, bound, note):
+    """`work` per launch in units such that work / seconds / 1e12 is in `unit`
+    (flop or op counts for T*/s, bytes * 1e3 for GB/s)."""
+    achieved = work / (avg_us * 1e-6) / 1e12
+    return {"kernel": kernel, "bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+            "frac": round(achieved / peak, 4), "traffic": measured_traffic(kernel),
+            "algorithmic_per_launch": note, "avg_launch_us": round(avg_us, 2)}
+
+
+def plain_roofline(kernels_us):
+    dom = max(kernels_us, key=kernels_us.get)
+    us = kernels_us[dom]
+    if dom.startswith("leg"):
+        fl = algorithmic_flops(LMAX, NLAT)
+        return roofline_entry(dom, us, fl, PEAK_FP64_TFLOPS, "TFLOP/s", "mfma",
+                              f"{fl:.4e} flop = 8 x (nlat/2) x (lmax+1)(lmax+2)/2 (FP64 VALU)")
+    by = fft_bytes(NLAT, NLON, LMAX)
+    return roofline_entry(dom, us, by * 1e3, PEAK_HBM_GBS, "GB/s", "hbm", f"{by:.4e} bytes")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-split", action="store_true", help="skip the batched split-precision line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 5 device copies of 256 MiB first (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -113,17 +200,9 @@ def main() -> int:
     shtk = __graft_entry__.load_package()
     from shtnskit_jl_amd import _lib
 
-    lmax, nlat, nlon = 1023, 1024, 2048
     dev = torch.device("cuda:0")
-    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
-    rng = np.random.Generator(np.random.PCG64(2))
-    a = rng.standard_normal((lmax + 1, lmax + 1)) + 1j * rng.standard_normal((lmax + 1, lmax + 1))
-    l = np.arange(lmax + 1)[:, None]
-    m = np.arange(lmax + 1)[None, :]
-    a = np.where(l >= m, a * (l + 1.0) ** -1.5, 0)
-    a[:, 0] = a[:, 0].real
-    alm = torch.from_numpy(np.ascontiguousarray(a.T)).to(dev).T  # column-major (l fastest)
-    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    cfg = shtk.create_gauss_config(LMAX, NLAT, nlon=NLON)
 
     if args.calibrate_copy:
         src = torch.empty(1 << 25, dtype=torch.float64, device=dev).normal_()
@@ -133,6 +212,10 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
+    # ---- plain FP64, one field (the headline value)
+    alm = to_device(torch, np, spectrum(np, 1), dev)
+    torch.cuda.synchronize()
+
     def step():
         return shtk.analysis(cfg, shtk.synthesis(cfg, alm))
 
@@ -140,48 +223,9 @@ def main() -> int:
         out = step()
     torch.cuda.synchronize()
     rt_err = float(torch.linalg.vector_norm(out - alm) / torch.linalg.vector_norm(alm))
-
-    stream = torch.cuda.current_stream(dev)
-    _lib.profile_enable(cfg.plan, True)
-    _lib.profile_read(cfg.plan)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.steps):
-        out = step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    total_ms = e0.elapsed_time(e1)
-    kern = _lib.profile_read(cfg.plan)
-    _lib.profile_enable(cfg.plan, False)
-
-    # the same region without per-kernel events, to expose their overhead
-    e0.record(stream)
-    for _ in range(args.steps):
-        out = step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    plain_ms = e0.elapsed_time(e1)
-
-    ms_per_step = total_ms / args.steps
+    with_ev, no_ev, kern, _ = timed_steps(torch, _lib, cfg, step, args.steps, stream)
+    ms_per_step = with_ev / args.steps
     kernels_us = {k: 1000.0 * v[0] / v[1] for k, v in kern.items()}
-    dom = max(kern, key=lambda k: kern[k][0])
-    dom_us = kernels_us[dom]
-    if dom.startswith("leg"):
-        flops = algorithmic_flops(lmax, nlat)
-        achieved = flops / (dom_us * 1e-6) / 1e12
-        roof = {"kernel": dom, "bound": "mfma", "compute": "fp64",
-                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": measured_traffic(dom),
-                "algorithmic_per_launch": f"{flops:.4e} flop = 8 x (nlat/2) x (lmax+1)(lmax+2)/2",
-                "avg_launch_us": round(dom_us, 2)}
-    else:
-        byts = fft_bytes(nlat, nlon, lmax)
-        achieved = byts / (dom_us * 1e-6) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(dom),
-                "avg_launch_us": round(dom_us, 2)}
 
     res = {
         "metric": METRIC,
@@ -191,24 +235,68 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 5),
-        "ms_per_step_without_kernel_events": round(plain_ms / args.steps, 5),
+        "ms_per_step_without_kernel_events": round(no_ev / args.steps, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (complex Gaussian alm, seed 2, (l+1)^-1.5 spectrum)",
         "config": {"workload": "plain FP64 scalar round trip (synthesis+analysis), Gauss grid",
-                   "lmax": lmax, "mmax": lmax, "nlat": nlat, "nlon": nlon, "fields": 1,
+                   "lmax": LMAX, "mmax": LMAX, "nlat": NLAT, "nlon": NLON, "fields": 1,
                    "norm": "orthonormal+CS", "parallelism": "single GPU"},
-        "roofline": roof,
+        "roofline": plain_roofline(kernels_us),
         "kernels_avg_us": {k: round(v, 2) for k, v in kernels_us.items()},
-        "event_overhead_pct": round(100.0 * (total_ms - plain_ms) / plain_ms, 2),
         "round_trip_rel_l2": rt_err,
     }
-    if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(lmax, nlat, nlon)
-    else:
-        res["cpu_baseline"] = None
+
+    # ---- batched split precision, 16 fields
+    if not args.no_split:
+        nf = SPLIT_FIELDS
+        balm = to_device(torch, np, spectrum(np, nf), dev)
+        torch.cuda.synchronize()
+
+        def sstep():
+            return shtk.analysis_batch(cfg, shtk.synthesis_batch(cfg, balm, precision="split"), precision="split")
+
+        for _ in range(args.warmup):
+            sout = sstep()
+        ref = shtk.analysis_batch(cfg, shtk.synthesis_batch(cfg, balm))  # FP64 batch path
+        torch.cuda.synchronize()
+        err_fp64 = max(float(torch.linalg.vector_norm(sout[..., k] - ref[..., k]) /
+                             torch.linalg.vector_norm(ref[..., k])) for k in range(nf))
+        err_in = max(float(torch.linalg.vector_norm(sout[..., k] - balm[..., k]) /
+                           torch.linalg.vector_norm(balm[..., k])) for k in range(nf))
+        ssteps = max(4, args.steps // 4)
+        s_ev, s_no, skern, _ = timed_steps(torch, _lib, cfg, sstep, ssteps, stream)
+        sk_us = {k: 1000.0 * v[0] / v[1] for k, v in skern.items()}
+        dom = max(sk_us, key=sk_us.get)
+        ops = split_int8_ops(LMAX, NLAT, nf)
+        if dom.startswith("oz"):
+            sroof = roofline_entry(dom, sk_us[dom], ops, PEAK_INT8_TOPS, "TOP/s", "mfma",
+                                   f"{ops:.4e} int8 ops = 21 slice products x 2 x (nlat/2)(lmax+1)(lmax+2)/2 "
+                                   f"x 2 x {nf} fields")
+        else:
+            by = fft_bytes(NLAT, NLON, LMAX, nf)
+            sroof = roofline_entry(dom, sk_us[dom], by * 1e3, PEAK_HBM_GBS, "GB/s", "hbm", f"{by:.4e} bytes")
+        ms_batch = s_ev / ssteps
+        res["split_batched"] = {
+            "value": round(1000.0 * nf / ms_batch, 2),
+            "unit": "SHT/s",
+            "fields": nf,
+            "steps": ssteps,
+            "ms_per_batch_round_trip": round(ms_batch, 5),
+            "ms_per_batch_without_kernel_events": round(s_no / ssteps, 5),
+            "speedup_vs_plain": round((1000.0 * nf / ms_batch) / res["value"], 3),
+            "dtype": "int8 MFMA slices of FP64 Legendre values, FP64 tables and cross-slice sums",
+            "max_field_rel_l2_vs_fp64_path": err_fp64,
+            "max_field_rel_l2_round_trip": err_in,
+            "config": {"workload": "batched split-precision scalar round trip (synthesis_batch+analysis_batch)",
+                       "lmax": LMAX, "nlat": NLAT, "nlon": NLON, "fields": nf},
+            "roofline": sroof,
+            "kernels_avg_us": {k: round(v, 2) for k, v in sk_us.items()},
+        }
+
+    res["cpu_baseline"] = None if args.no_cpu else cpu_baseline(LMAX, NLAT, NLON)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -177,6 +177,72 @@ def plain_roofline(kernels_us):
     return roofline_entry(dom, us, by * 1e3, PEAK_HBM_GBS, "GB/s", "hbm", f"{by:.4e} bytes")
 
 
+def measure_uneven(torch, np, shtk, dev, steps):
+    """64 fields with lmax drawn uniformly from {63, 127, 255, 511, 1023} (seed 3),
+    nlat = lmax + 1, nlon = 2 nlat, (l+1)^-1 spectra: one split-precision batch
+    object, launch sequence captured in a hipGraph, launch shapes autotuned.
+    Value: field round trips per second of the whole batch."""
+    rng = np.random.Generator(np.random.PCG64(3))
+    draws = rng.choice([63, 127, 255, 511, 1023], size=64)
+    groups = sorted({int(L): int((draws == L).sum()) for L in set(draws.tolist())}.items(), reverse=True)
+    cfgs = [shtk.create_gauss_config(L, L + 1, nlon=2 * L + 2) for L, _ in groups]
+    nfs = [nf for _, nf in groups]
+    alms = []
+    for L, nf in groups:
+        a = rng.standard_normal((L + 1, L + 1, nf)) + 1j * rng.standard_normal((L + 1, L + 1, nf))
+        l = np.arange(L + 1)[:, None, None]
+        m = np.arange(L + 1)[None, :, None]
+        a = np.where(l >= m, a / (l + 1.0), 0)
+        a[:, 0, :] = a[:, 0, :].real
+        alms.append(to_device(torch, np, a, dev))
+    fp64 = shtk.UnevenBatch(cfgs, nfs, precision="fp64")
+    ref_f = fp64.synthesis(alms)
+    ref_a = fp64.analysis(ref_f)
+    torch.cuda.synchronize()
+    cfgs2 = [shtk.create_gauss_config(L, L + 1, nlon=2 * L + 2) for L, _ in groups]
+    b = shtk.UnevenBatch(cfgs2, nfs, precision="split")
+    tune = b.autotune([a.clone() for a in alms], [f.clone() for f in ref_f], reps=2)
+    b.close()
+    cfgs3 = [shtk.create_gauss_config(L, L + 1, nlon=2 * L + 2) for L, _ in groups]
+    g = shtk.UnevenBatch(cfgs3, nfs, precision="split", graph=True)
+    g.set_tiles(tune["persistent_blocks"], tune["fft_pairs"])
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        for _ in range(2):
+            fs = g.synthesis(alms)
+            back = g.analysis(fs)
+        torch.cuda.synchronize()
+        err = 0.0
+        for fa, ra in zip(back, ref_a):
+            for k in range(fa.shape[2]):
+                err = max(err, float(torch.linalg.vector_norm(fa[..., k] - ra[..., k]) /
+                                     torch.linalg.vector_norm(ra[..., k])))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(steps):
+            fs = g.synthesis(alms)
+            back = g.analysis(fs)
+        e1.record(s)
+        torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        fp64.analysis(fp64.synthesis(alms))
+    e1.record()
+    torch.cuda.synchronize()
+    fp64_ms = e0.elapsed_time(e1) / steps
+    g.close()
+    fp64.close()
+    return {"value": round(1000.0 * 64 / ms, 2), "unit": "field round trips/s", "fields": 64,
+            "groups": {str(L): nf for L, nf in groups}, "steps": steps,
+            "ms_per_batch_round_trip": round(ms, 4), "fp64_path_ms_per_batch_round_trip": round(fp64_ms, 4),
+            "max_field_rel_l2_vs_fp64_path": err, "autotuned": tune,
+            "launches": "one hipGraph per direction; Legendre stages as persistent kernels over all groups"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +250,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-split", action="store_true", help="skip the batched split-precision line")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-lmax batch line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 5 device copies of 256 MiB first (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -295,6 +362,9 @@ def main() -> int:
             "roofline": sroof,
             "kernels_avg_us": {k: round(v, 2) for k, v in sk_us.items()},
         }
+
+    if not args.no_split and not args.no_uneven:
+        res["uneven_batch"] = measure_uneven(torch, np, shtk, dev, max(3, args.steps // 10))
 
     res["cpu_baseline"] = None if args.no_cpu else cpu_baseline(LMAX, NLAT, NLON)
     print(json.dumps(res), flush=True)

@@ -151,6 +151,35 @@ int shtk_adjoint_analysis_sphtor(shtk_plan *plan, const void *Sb, const void *Tb
  * caller's stream.  shtk_profile_read() waits for the recorded events and
  * returns, per kernel name (up to `max` entries, names NUL-terminated in
  * 48-byte slots), the accumulated milliseconds and launch counts, then resets. */
+/* ------------------------------------------------------------------------
+ * Uneven batches: ngroups configurations (descs[g], nfields[g] fields each)
+ * transformed together, e.g. many fields of different lmax.  The reference
+ * handles such a set as one transform per field, or one batch call per
+ * configuration (src/batch_transforms.jl:312-655); here every stage is one
+ * launch for all groups: with SHTK_PREC_SPLIT the Legendre contractions run as
+ * persistent kernels over a work queue ordered largest-first.  Arrays are
+ * passed per group (alm[g] of shape (lmax+1, mmax+1, nfields[g]), f[g] of
+ * shape (nlat, nlon, nfields[g])); ngroups <= 8. */
+typedef struct shtk_batch shtk_batch;
+int shtk_batch_create(const shtk_desc *descs, const int *nfields, int ngroups, int precision, shtk_batch **batch);
+int shtk_batch_destroy(shtk_batch *batch);
+/* The plan of group g (owned by the batch). */
+int shtk_batch_plan(shtk_batch *batch, int group, shtk_plan **plan);
+/* on = 1: each call captures its launch sequence in a hipGraph on first use
+ * (per direction, arrays, dtype and stream; the stream must not be the
+ * default stream) and replays it with a single graph launch afterwards. */
+int shtk_batch_set_graph(shtk_batch *batch, int on);
+/* Launch shapes: blocks of the persistent kernels (0: two per CU) and latitude
+ * pairs per FFT block (0: automatic, or 1, 2, 4). */
+int shtk_batch_set_tiles(shtk_batch *batch, int persistent_blocks, int fft_pairs);
+int shtk_batch_synthesis(shtk_batch *batch, const void *const *alm, void *const *f, int dtype, void *stream);
+int shtk_batch_analysis(shtk_batch *batch, const void *const *f, void *const *alm, int dtype, void *stream);
+/* Autotuner: times round trips (synthesis into f, analysis back into alm, which
+ * therefore ends up holding its own projection) for every candidate launch
+ * shape and keeps the fastest; reports it if the pointers are not NULL. */
+int shtk_batch_autotune(shtk_batch *batch, void *const *alm, void *const *f, int dtype, int reps, void *stream,
+                        int *best_blocks, int *best_pairs, double *best_ms);
+
 int shtk_profile_enable(shtk_plan *plan, int on);
 int shtk_profile_read(shtk_plan *plan, int max, char *names, double *total_ms, int *count, int *n_out);
 

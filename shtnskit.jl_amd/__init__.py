@@ -15,5 +15,6 @@ from .transforms import (CPU, GPU, on_device, synthesis, analysis, synthesis_bat
                          synthesis_sphtor_batch, analysis_sphtor_batch, _adjoint_synthesis,
                          _adjoint_analysis, _adjoint_synthesis_sphtor, _adjoint_analysis_sphtor)
 from . import ad
+from .batch import UnevenBatch
 
 __version__ = "0.1.0"

@@ -21,7 +21,9 @@ EXPORTS = (
     "shtk_plan_create", "shtk_plan_destroy", "shtk_plan_nodes", "shtk_workspace_size",
     "shtk_synthesis", "shtk_analysis", "shtk_synthesis_ex", "shtk_analysis_ex", "shtk_synthesis_sphtor", "shtk_analysis_sphtor",
     "shtk_adjoint_synthesis", "shtk_adjoint_analysis", "shtk_adjoint_synthesis_sphtor",
-    "shtk_adjoint_analysis_sphtor", "shtk_profile_enable", "shtk_profile_read", "shtk_stream_sync",
+    "shtk_adjoint_analysis_sphtor", "shtk_batch_create", "shtk_batch_destroy", "shtk_batch_plan",
+    "shtk_batch_set_graph", "shtk_batch_set_tiles", "shtk_batch_synthesis", "shtk_batch_analysis",
+    "shtk_batch_autotune", "shtk_profile_enable", "shtk_profile_read", "shtk_stream_sync",
     "shtk_last_error", "shtk_version",
 )
 
@@ -72,6 +74,14 @@ def load():
     lib.shtk_adjoint_analysis.argtypes = [p, p, p, i, i, p]
     lib.shtk_adjoint_synthesis_sphtor.argtypes = [p, p, p, p, p, i, i, i, p]
     lib.shtk_adjoint_analysis_sphtor.argtypes = [p, p, p, p, p, i, i, p]
+    lib.shtk_batch_create.argtypes = [ctypes.POINTER(ShtkDesc), p, i, i, ctypes.POINTER(p)]
+    lib.shtk_batch_destroy.argtypes = [p]
+    lib.shtk_batch_plan.argtypes = [p, i, ctypes.POINTER(p)]
+    lib.shtk_batch_set_graph.argtypes = [p, i]
+    lib.shtk_batch_set_tiles.argtypes = [p, i, i]
+    lib.shtk_batch_synthesis.argtypes = [p, p, p, i, p]
+    lib.shtk_batch_analysis.argtypes = [p, p, p, i, p]
+    lib.shtk_batch_autotune.argtypes = [p, p, p, i, i, p, p, p, p]
     lib.shtk_profile_enable.argtypes = [p, i]
     lib.shtk_profile_read.argtypes = [p, i, p, p, p, p]
     lib.shtk_stream_sync.argtypes = [p]

@@ -116,6 +116,7 @@ struct FftArgs {
     int io_f32;
     int mode;          // grid row factor: 0 none, 1 1/sin(theta), 2 sin(theta) (Robert form)
     const double *s;   // [np] sin(theta) for Robert scaling
+    int fft_pairs;     // latitude pairs per block (0: chosen from the grid size)
 };
 
 }  // namespace shtk
@@ -123,7 +124,7 @@ struct FftArgs {
 struct shtk_plan {
     shtk_desc d{};
     int np = 0, nm = 0, seg = 128, nseg_total = 0;
-    int xcd_map = 0, synth_chains = 2;  // launch-shape knobs
+    int xcd_map = 0, synth_chains = 2, fft_pairs = 0;  // launch-shape knobs
     bool canonical = true;
     double inv_scale = 0, scale_phi = 0;
     std::vector<double> hx, hw;  // north-first nodes and weights (nlat)
@@ -163,7 +164,7 @@ struct shtk_plan {
     double *sectv = nullptr;  // [nm][np] scaled sectoral seeds
     int *sectk = nullptr;
     double *oz_q0 = nullptr, *oz_q1 = nullptr;  // [nm][4][np]
-    int *oz_qk = nullptr, *oz_ep = nullptr, *oz_chbase = nullptr;
+    int *oz_qk = nullptr, *oz_ep = nullptr, *oz_chbase = nullptr, *oz_chm = nullptr;
     int *oz_emaxl = nullptr, *oz_epl = nullptr;  // per (order, degree), qc indexing
     double *oz_asc = nullptr;
     size_t qc_len = 0;
@@ -195,12 +196,13 @@ struct OzArgs {
     const int *qk;          // [nm][4][np] scale count
     const int *ep;          // [nm][np] |P| < 2^ep over the order's degrees
     const int *chbase;      // [nm] first chunk of each order
+    const int *chm;         // [ch_total] order of each chunk
     int ch_total;
     const void *spec_in;    // canonical coefficients (synthesis)
     int in_f32;
     long long spec_fstride;
     int4 *bdig;             // [nfg][ch_total][1536] B operands
-    double *colsum;         // [nfg][nm][2][32] sum over degrees of the integer field values, per parity
+    unsigned long long *colsum;  // [nfg][nm][2][32] sum over degrees of the integer field values, per parity
     int *eb;                // [nfg][nm][32] column exponents
     double2 *F0;            // Fourier data [field][im][nlat] (synthesis output, analysis input)
     double g;
@@ -239,7 +241,23 @@ struct OzAscArgs {
     int *epl;
 };
 
+// Several configurations in one launch per stage, passed by value as kernel
+// arguments (so a captured graph keeps them): prefix[g] is the running count
+// of blocks (prologues) or work units (persistent contractions) before group
+// g, prefix[ngroups] the total.
+constexpr int OZ_MAX_GROUPS = 8;
+struct OzGroups {
+    OzArgs args[OZ_MAX_GROUPS];
+    int prefix[OZ_MAX_GROUPS + 1];
+    int ngroups;
+    int *counter;  // work counter of the persistent kernels (reset in the stream before each launch)
+};
+
 hipError_t launch_oz_seed(const OzSeedArgs &a, hipStream_t st);
+hipError_t launch_oz_synth_grp(const OzGroups &bexp, const OzGroups &bdig, const OzGroups &units, int in_f32,
+                               int persistent_blocks, hipStream_t st);
+hipError_t launch_oz_anal_grp(const OzGroups &gexp, const OzGroups &gdig, const OzGroups &units,
+                              int persistent_blocks, hipStream_t st);
 hipError_t launch_oz_synth(const OzArgs &a, hipStream_t st);
 hipError_t launch_oz_asc(const OzAscArgs &a, hipStream_t st);
 hipError_t launch_oz_anal(const OzArgs &a, hipStream_t st);

@@ -584,8 +584,10 @@ __device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, con
     }
 }
 
+// The read-only coefficient tables come in as restrict kernel arguments so
+// that their wave-uniform reads compile to scalar loads.
 template <typename OT>
-__global__ __launch_bounds__(ANS_THREADS) void k_anal_s(const LegArgs a) {
+__global__ __launch_bounds__(ANS_THREADS) void k_anal_s(const LegArgs a, const double2 *__restrict__ qc) {
     __shared__ double2 tb[8 * ANS_THREADS];
     __shared__ double cseg[SEG_MAX + 8];
     const Item it = make_item<512>(a);
@@ -594,7 +596,7 @@ __global__ __launch_bounds__(ANS_THREADS) void k_anal_s(const LegArgs a) {
     const bool dead = it.lstop <= it.l0 || it.pstart >= it.pb1;
     if (tid < 64) zero_fill(a, it, col, dead, tid);
     if (dead) return;
-    const double2 *__restrict__ qcm = a.qc + a.coff[it.im] - it.m;
+    const double2 *__restrict__ qcm = qc + a.coff[it.im] - it.m;
     // c_l of this segment (l0 <= l < lstop; the tail up to a multiple of 8 is padded)
     for (int t = tid; t < SEG_MAX + 8; t += ANS_THREADS) cseg[t] = qcm[min(it.l0 + t, a.lmax)].y;
     __syncthreads();
@@ -764,7 +766,8 @@ __device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, con
 }
 
 template <typename OT>
-__global__ __launch_bounds__(ANV_THREADS) void k_anal_v(const LegArgs a) {
+__global__ __launch_bounds__(ANV_THREADS) void k_anal_v(const LegArgs a, const double2 *__restrict__ qc,
+                                                         const double2 *__restrict__ qd) {
     __shared__ double2 tb[2 * 4 * ANV_THREADS];
     __shared__ double2 red[8];
     __shared__ double2 dseg[SEG_MAX + 8];
@@ -780,8 +783,8 @@ __global__ __launch_bounds__(ANV_THREADS) void k_anal_v(const LegArgs a) {
         zero_fill(a, it, ct, dead, tid);
     }
     if (dead) return;
-    const double2 *__restrict__ qcm = a.qc + a.coff[it.im] - it.m;
-    const double2 *__restrict__ qdm = a.qd + a.coff[it.im] - it.m;
+    const double2 *__restrict__ qcm = qc + a.coff[it.im] - it.m;
+    const double2 *__restrict__ qdm = qd + a.coff[it.im] - it.m;
     for (int t = tid; t < SEG_MAX + 8; t += ANV_THREADS) {
         const int l = min(it.l0 + t, a.lmax);
         dseg[t] = qdm[l];
@@ -873,14 +876,14 @@ hipError_t launch_leg_anal(const LegArgs &a0, int nfields, int vector, hipStream
     const dim3 grid((unsigned)a.gx * a.gy * nfields);
     if (vector) {
         if (a.io_f32)
-            hipLaunchKernelGGL(k_anal_v<float>, grid, dim3(ANV_THREADS), 0, st, a);
+            hipLaunchKernelGGL(k_anal_v<float>, grid, dim3(ANV_THREADS), 0, st, a, a.qc, a.qd);
         else
-            hipLaunchKernelGGL(k_anal_v<double>, grid, dim3(ANV_THREADS), 0, st, a);
+            hipLaunchKernelGGL(k_anal_v<double>, grid, dim3(ANV_THREADS), 0, st, a, a.qc, a.qd);
     } else {
         if (a.io_f32)
-            hipLaunchKernelGGL(k_anal_s<float>, grid, dim3(ANS_THREADS), 0, st, a);
+            hipLaunchKernelGGL(k_anal_s<float>, grid, dim3(ANS_THREADS), 0, st, a, a.qc);
         else
-            hipLaunchKernelGGL(k_anal_s<double>, grid, dim3(ANS_THREADS), 0, st, a);
+            hipLaunchKernelGGL(k_anal_s<double>, grid, dim3(ANS_THREADS), 0, st, a, a.qc);
     }
     return hipGetLastError();
 }

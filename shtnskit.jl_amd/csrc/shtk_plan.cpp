@@ -7,6 +7,7 @@
 // coefficient convention map (src/normalization.jl:142-153) and the FFT
 // twiddles.  Transforms enqueue the kernels of shtk_legendre.hip and
 // shtk_fft.hip on the caller's stream.
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -121,7 +122,7 @@ void free_plan(shtk_plan *p) {
                     p->qd,     p->coff,   p->segbase, p->seedq0, p->seedq1, p->seedk, p->pfirst,
                     p->pfast,  p->segm,   p->convscale, p->tw,   p->ws,     p->sectv, p->sectk, p->oz_q0,
                     p->oz_q1,  p->oz_qk,  p->oz_ep,     p->oz_chbase, p->oz_ws, p->oz_emaxl, p->oz_epl,
-                    p->oz_asc};
+                    p->oz_asc, p->oz_chm};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -245,6 +246,7 @@ shtk::FftArgs fft_args(const shtk_plan *p) {
     for (size_t q = 0; q < p->radix.size(); ++q) f.radix |= (unsigned long long)p->radix[q] << (4 * q);
     f.tw = p->tw;
     f.s = p->s;
+    f.fft_pairs = p->fft_pairs;
     return f;
 }
 
@@ -272,11 +274,13 @@ int norm_ltr(const shtk_plan *p, int ltr, int *out) {
 int ensure_oz_tables(shtk_plan *p, hipStream_t st) {
     if (p->oz_q0) return SHTK_OK;
     const int np = p->np, nm = p->nm, lmax = p->d.lmax;
-    std::vector<int> chbase(nm);
+    std::vector<int> chbase(nm), chm;
     int total = 0;
     for (int im = 0; im < nm; ++im) {
         chbase[im] = total;
-        total += (lmax - im * p->d.mres + 128) / 128;
+        const int n = (lmax - im * p->d.mres + 128) / 128;
+        total += n;
+        chm.insert(chm.end(), n, im);
     }
     const size_t nq = (size_t)nm * 4 * np;
     HIP_TRY(hipMalloc((void **)&p->oz_q0, sizeof(double) * nq), "split tables");
@@ -284,6 +288,7 @@ int ensure_oz_tables(shtk_plan *p, hipStream_t st) {
     HIP_TRY(hipMalloc((void **)&p->oz_qk, sizeof(int) * nq), "split tables");
     HIP_TRY(hipMalloc((void **)&p->oz_ep, sizeof(int) * (size_t)nm * np), "split tables");
     RC_TRY(upload(&p->oz_chbase, chbase, "split tables"));
+    RC_TRY(upload(&p->oz_chm, chm, "split tables"));
     p->oz_ch_total = total;
     RC_TRY(upload(&p->oz_emaxl, std::vector<int>(p->qc_len, -100000), "split tables"));
     HIP_TRY(hipMalloc((void **)&p->oz_epl, sizeof(int) * p->qc_len), "split tables");
@@ -319,16 +324,20 @@ struct OzWs {
     size_t bdig, colsum, eb, total;
 };
 
+// ksteps of 64 pairs in the split analysis: whole halves of 256 pairs, so that
+// every LDS position of a half has a field operand (zero beyond np)
+int oz_nks(const shtk_plan *p) { return 4 * ((p->np + 255) / 256); }
+
 // synthesis: B operands per (field group, chunk); analysis: per (field group,
 // order, kstep of 64 pairs) plus an FP64 canonical accumulator when the output
 // is not canonical FP64
 OzWs oz_ws_layout(const shtk_plan *p, int nfg, bool analysis) {
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     OzWs L;
-    const size_t nks = (size_t)(p->np + 63) / 64;
+    const size_t nks = oz_nks(p);
     if (!analysis) {
         L.bdig = al((size_t)nfg * p->oz_ch_total * shtk::OZ_CHUNK_V4 * 16);
-        L.colsum = al(sizeof(double) * (size_t)nfg * p->nm * 2 * shtk::OZ_COLS);
+        L.colsum = al(sizeof(unsigned long long) * (size_t)nfg * p->nm * 2 * shtk::OZ_COLS);
         L.eb = al(sizeof(int) * (size_t)nfg * p->nm * shtk::OZ_COLS);
     } else {
         L.bdig = al((size_t)nfg * p->nm * nks * shtk::OZ_CHUNK_V4 * 16);
@@ -350,6 +359,83 @@ int ensure_oz_ws(shtk_plan *p, size_t bytes) {
     HIP_TRY(hipMalloc(&p->oz_ws, bytes), "split workspace allocation");
     p->oz_ws_bytes = bytes;
     return SHTK_OK;
+}
+
+// Split-precision argument blocks (tables and workspaces must be in place).
+shtk::OzArgs oz_common_args(const shtk_plan *p, int nf, int ltr, double g) {
+    shtk::OzArgs o{};
+    o.lmax = p->d.lmax;
+    o.mmax = p->d.mmax;
+    o.mres = p->d.mres;
+    o.nlat = p->d.nlat;
+    o.np = p->np;
+    o.nm = p->nm;
+    o.ltr = ltr;
+    o.nf = nf;
+    o.nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
+    o.x = p->x;
+    o.ab = p->ab;
+    o.coff = p->coff;
+    o.spec_fstride = (long long)(p->d.lmax + 1) * (p->d.mmax + 1);
+    o.g = g;
+    o.xcd = p->xcd_map;
+    return o;
+}
+
+shtk::OzArgs oz_synth_args(const shtk_plan *p, const void *spec_in, int in_f32, int nf, int ltr, double g,
+                           double2 *F0) {
+    shtk::OzArgs o = oz_common_args(p, nf, ltr, g);
+    o.q0 = p->oz_q0;
+    o.q1 = p->oz_q1;
+    o.qk = p->oz_qk;
+    o.ep = p->oz_ep;
+    o.chbase = p->oz_chbase;
+    o.chm = p->oz_chm;
+    o.ch_total = p->oz_ch_total;
+    o.spec_in = spec_in;
+    o.in_f32 = in_f32;
+    const OzWs W = oz_ws_layout(p, o.nfg, false);
+    char *ow = (char *)p->oz_ws;
+    o.bdig = (int4 *)ow;
+    o.colsum = (unsigned long long *)(ow + W.bdig);
+    o.eb = (int *)(ow + W.bdig + W.colsum);
+    o.F0 = F0;
+    return o;
+}
+
+shtk::OzArgs oz_anal_args(const shtk_plan *p, double2 *F0, double *acc, int nf, int ltr, const double *wlat,
+                          double g) {
+    shtk::OzArgs o = oz_common_args(p, nf, ltr, g);
+    o.F0 = F0;
+    o.qc = p->qc;
+    o.asc = p->oz_asc;
+    o.epl = p->oz_epl;
+    o.seedq0 = p->seedq0;
+    o.seedq1 = p->seedq1;
+    o.seedk = p->seedk;
+    o.segbase = p->segbase;
+    o.segm = p->segm;
+    o.pfirst = p->pfirst;
+    o.seg = p->seg;
+    o.nseg_total = p->nseg_total;
+    o.nks = oz_nks(p);
+    o.nhalf = o.nks / 4;
+    o.wlat = wlat;
+    o.spec_out = acc;
+    const OzWs W = oz_ws_layout(p, o.nfg, true);
+    char *ow = (char *)p->oz_ws;
+    o.gdig = (int4 *)ow;
+    o.gcolsum = (double *)(ow + W.bdig);
+    o.eg = (int *)(ow + W.bdig + W.colsum);
+    return o;
+}
+
+// Bytes of split workspace for nf fields in both directions (analysis adds an
+// FP64 canonical accumulator when the output is not canonical FP64).
+size_t oz_ws_bytes(const shtk_plan *p, int nf, bool with_acc) {
+    const int nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
+    const size_t acc = with_acc ? (size_t)nf * (p->d.lmax + 1) * (p->d.mmax + 1) * 16 : 0;
+    return std::max(oz_ws_layout(p, nfg, false).total, oz_ws_layout(p, nfg, true).total + acc);
 }
 
 // One transform direction with its scalings (see the entry points below).
@@ -397,38 +483,8 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
     a.ll1_mode = j.ll1_mode;
     if (j.prec == SHTK_PREC_SPLIT) {
         RC_TRY(ensure_oz_tables(p, st));
-        const int nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
-        const OzWs W = oz_ws_layout(p, nfg, false);
-        RC_TRY(ensure_oz_ws(p, W.total));
-        shtk::OzArgs o{};
-        o.lmax = p->d.lmax;
-        o.mmax = p->d.mmax;
-        o.mres = p->d.mres;
-        o.nlat = p->d.nlat;
-        o.np = p->np;
-        o.nm = p->nm;
-        o.ltr = j.ltr;
-        o.nf = nf;
-        o.nfg = nfg;
-        o.x = p->x;
-        o.ab = p->ab;
-        o.coff = p->coff;
-        o.q0 = p->oz_q0;
-        o.q1 = p->oz_q1;
-        o.qk = p->oz_qk;
-        o.ep = p->oz_ep;
-        o.chbase = p->oz_chbase;
-        o.ch_total = p->oz_ch_total;
-        o.spec_in = a.in0;
-        o.in_f32 = a.io_f32;
-        o.spec_fstride = a.spec_fstride;
-        char *ow = (char *)p->oz_ws;
-        o.bdig = (int4 *)ow;
-        o.colsum = (double *)(ow + W.bdig);
-        o.eb = (int *)(ow + W.bdig + W.colsum);
-        o.F0 = F0;
-        o.g = j.g;
-        o.xcd = p->xcd_map;
+        RC_TRY(ensure_oz_ws(p, oz_ws_bytes(p, nf, !p->canonical || dtype != SHTK_F64)));
+        const shtk::OzArgs o = oz_synth_args(p, a.in0, a.io_f32, nf, j.ltr, j.g, F0);
         HIP_TRY(timed(p, "oz_synth", st, [&] { return shtk::launch_oz_synth(o, st); }),
                 "split synthesis launch");
     } else {
@@ -485,48 +541,12 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
     }
     if (j.prec == SHTK_PREC_SPLIT) {
         RC_TRY(ensure_oz_tables(p, st));
-        const int nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
-        const OzWs W = oz_ws_layout(p, nfg, true);
-        // FP64 canonical accumulator: the output itself, or the conversion workspace
         const bool direct = p->canonical && dtype == SHTK_F64;
-        const size_t acc_bytes = direct ? 0 : (size_t)nf * a.spec_fstride * 16;
-        RC_TRY(ensure_oz_ws(p, W.total + acc_bytes));
-        char *ow = (char *)p->oz_ws;
-        double *acc = direct ? (double *)s0 : (double *)(ow + W.total);
+        RC_TRY(ensure_oz_ws(p, oz_ws_bytes(p, nf, !direct)));
+        const int nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
+        double *acc = direct ? (double *)s0 : (double *)((char *)p->oz_ws + oz_ws_layout(p, nfg, true).total);
         HIP_TRY(shtk::launch_zero(acc, (size_t)nf * a.spec_fstride * 16, st), "output clear");
-        shtk::OzArgs o{};
-        o.lmax = p->d.lmax;
-        o.mmax = p->d.mmax;
-        o.mres = p->d.mres;
-        o.nlat = p->d.nlat;
-        o.np = p->np;
-        o.nm = p->nm;
-        o.ltr = j.ltr;
-        o.nf = nf;
-        o.nfg = nfg;
-        o.x = p->x;
-        o.coff = p->coff;
-        o.spec_fstride = a.spec_fstride;
-        o.F0 = F0;
-        o.g = j.g;
-        o.qc = p->qc;
-        o.asc = p->oz_asc;
-        o.epl = p->oz_epl;
-        o.seedq0 = p->seedq0;
-        o.seedq1 = p->seedq1;
-        o.seedk = p->seedk;
-        o.segbase = p->segbase;
-        o.segm = p->segm;
-        o.pfirst = p->pfirst;
-        o.seg = p->seg;
-        o.nseg_total = p->nseg_total;
-        o.nks = (p->np + 63) / 64;
-        o.nhalf = (o.nks + 3) / 4;
-        o.wlat = j.wlat;
-        o.spec_out = acc;
-        o.gdig = (int4 *)ow;
-        o.gcolsum = (double *)(ow + W.bdig);
-        o.eg = (int *)(ow + W.bdig + W.colsum);
+        const shtk::OzArgs o = oz_anal_args(p, F0, acc, nf, j.ltr, j.wlat, j.g);
         HIP_TRY(timed(p, "oz_anal", st, [&] { return shtk::launch_oz_anal(o, st); }), "split analysis launch");
         if (!direct)
             HIP_TRY(shtk::launch_spec_scale(acc, s0, 0, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale,
@@ -563,15 +583,350 @@ int segment_length() {
 
 // Launch-shape knobs read once per plan: SHTK_XCD=1 keeps the pair blocks of
 // an order on one XCD in the synthesis kernels, SHTK_SYNTH_CHAINS (1, 2 or 4)
-// sets the latitude pairs per lane of the scalar synthesis.
+// sets the latitude pairs per lane of the scalar synthesis, SHTK_FFT_PAIRS
+// (1, 2 or 4) the latitude pairs per FFT block (default: chosen per call).
 int env_int(const char *name, int dflt) {
     const char *e = std::getenv(name);
     return e ? std::atoi(e) : dflt;
 }
 
+// ---------------------------------------------------------------------------
+// Uneven batches: several configurations (one plan each), nf[g] fields per
+// configuration.  With SHTK_PREC_SPLIT every stage is one launch for all
+// groups (the Legendre contractions as persistent work-queue kernels); with
+// graphs on, the whole launch sequence of a call is captured once per
+// (direction, arrays, dtype, stream) and replayed with one hipGraphLaunch.
+// ---------------------------------------------------------------------------
+struct BatchGraph {
+    int dir, dtype;
+    std::vector<const void *> ptrs;
+    hipStream_t stream;
+    hipGraphExec_t exec;
+};
+
+}  // namespace
+
+struct shtk_batch {
+    int ngroups = 0, precision = SHTK_PREC_FP64, use_graph = 0;
+    std::vector<shtk_plan *> plans;   // in caller order
+    std::vector<int> nf;
+    std::vector<int> order;           // groups by descending lmax (work order of the persistent kernels)
+    int *counter = nullptr;           // device work counters (synthesis, analysis)
+    int persistent_blocks = 0;        // 0: 2 per CU
+    int fft_pairs = 0;                // FFT latitude pairs per block (0: automatic)
+    int device = 0, cus = 256;
+    std::vector<BatchGraph> graphs;
+    std::mutex mu;
+};
+
+namespace {
+
+int batch_blocks(const shtk_batch *b) {
+    return b->persistent_blocks > 0 ? b->persistent_blocks : 2 * b->cus;
+}
+
+// The grouped split launches of one direction (tables and workspaces ready).
+int batch_split(shtk_batch *b, int dir, const void *const *in, void *const *out, int dtype, hipStream_t st) {
+    shtk::OzGroups g1{}, g2{}, g3{};
+    const int n = b->ngroups;
+    g1.ngroups = g2.ngroups = g3.ngroups = n;
+    g3.counter = b->counter + dir;
+    std::vector<double *> accs(n);
+    if (dir == 0) {
+        for (int q = 0; q < n; ++q) {
+            const int gi = b->order[q];
+            shtk_plan *p = b->plans[gi];
+            const int nf = b->nf[gi];
+            const WsLayout L = ws_layout(p, nf, 0);
+            double2 *F0 = (double2 *)p->ws;
+            const void *spec = in[gi];
+            int in_f32 = dtype == SHTK_F32;
+            if (!p->canonical) {
+                void *dst = (char *)p->ws + L.f_bytes;
+                HIP_TRY(shtk::launch_spec_scale(spec, dst, in_f32, 0, p->convscale, 0, p->d.lmax, p->d.mmax,
+                                                p->d.mres, (long long)(p->d.lmax + 1) * (p->d.mmax + 1), nf, st),
+                        "coefficient conversion");
+                spec = dst;
+                in_f32 = 0;
+            }
+            const shtk::OzArgs o = oz_synth_args(p, spec, in_f32, nf, p->d.lmax, p->inv_scale / p->d.nlon, F0);
+            g1.args[q] = g2.args[q] = g3.args[q] = o;
+            g1.prefix[q + 1] = g1.prefix[q] + o.nm * o.nfg;
+            g2.prefix[q + 1] = g2.prefix[q] + o.ch_total * o.nfg;
+            g3.prefix[q + 1] = g3.prefix[q] + ((o.np + 63) / 64) * o.nm * o.nfg;
+        }
+        // input dtype is the batch's for every group; converted groups read FP64
+        bool any_f32 = false;
+        for (int q = 0; q < n; ++q) any_f32 |= g1.args[q].in_f32 != 0;
+        if (any_f32)
+            for (int q = 0; q < n; ++q)
+                if (!g1.args[q].in_f32) return fail(SHTK_EUNSUPPORTED, "split batch: mixed input precisions");
+        HIP_TRY(shtk::launch_oz_synth_grp(g1, g2, g3, any_f32, batch_blocks(b), st), "split batch synthesis");
+        for (int gi = 0; gi < n; ++gi) {
+            shtk_plan *p = b->plans[gi];
+            shtk::FftArgs fa = fft_args(p);
+            fa.F = (double2 *)p->ws;
+            fa.grid = out[gi];
+            fa.io_f32 = dtype == SHTK_F32;
+            if (b->fft_pairs) fa.fft_pairs = b->fft_pairs;
+            HIP_TRY(shtk::launch_fft(fa, b->nf[gi], 1, st), "phi synthesis launch");
+        }
+        return SHTK_OK;
+    }
+    for (int gi = 0; gi < n; ++gi) {
+        shtk_plan *p = b->plans[gi];
+        shtk::FftArgs fa = fft_args(p);
+        fa.F = (double2 *)p->ws;
+        fa.grid = (void *)in[gi];
+        fa.io_f32 = dtype == SHTK_F32;
+        if (b->fft_pairs) fa.fft_pairs = b->fft_pairs;
+        HIP_TRY(shtk::launch_fft(fa, b->nf[gi], 0, st), "phi analysis launch");
+    }
+    for (int q = 0; q < n; ++q) {
+        const int gi = b->order[q];
+        shtk_plan *p = b->plans[gi];
+        const int nf = b->nf[gi];
+        const int nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
+        const bool direct = p->canonical && dtype == SHTK_F64;
+        double *acc = direct ? (double *)out[gi] : (double *)((char *)p->oz_ws + oz_ws_layout(p, nfg, true).total);
+        accs[gi] = acc;
+        HIP_TRY(shtk::launch_zero(acc, (size_t)nf * (p->d.lmax + 1) * (p->d.mmax + 1) * 16, st), "output clear");
+        const shtk::OzArgs o = oz_anal_args(p, (double2 *)p->ws, acc, nf, p->d.lmax, p->wscaled, 1.0);
+        g1.args[q] = g2.args[q] = g3.args[q] = o;
+        g1.prefix[q + 1] = g1.prefix[q] + o.nm * o.nfg;
+        g2.prefix[q + 1] = g2.prefix[q] + o.nks * o.nm * o.nfg;
+        g3.prefix[q + 1] = g3.prefix[q] + o.nhalf * o.nseg_total * o.nfg;
+    }
+    HIP_TRY(shtk::launch_oz_anal_grp(g1, g2, g3, batch_blocks(b), st), "split batch analysis");
+    for (int gi = 0; gi < n; ++gi) {
+        shtk_plan *p = b->plans[gi];
+        if (p->canonical && dtype == SHTK_F64) continue;
+        HIP_TRY(shtk::launch_spec_scale(accs[gi], out[gi], 0, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale,
+                                        1, p->d.lmax, p->d.mmax, p->d.mres,
+                                        (long long)(p->d.lmax + 1) * (p->d.mmax + 1), b->nf[gi], st),
+                "coefficient conversion");
+    }
+    return SHTK_OK;
+}
+
+// One direction of a batch, eagerly (no graph).
+int batch_issue(shtk_batch *b, int dir, const void *const *in, void *const *out, int dtype, hipStream_t st) {
+    if (b->precision == SHTK_PREC_SPLIT) return batch_split(b, dir, in, out, dtype, st);
+    for (int gi = 0; gi < b->ngroups; ++gi) {
+        shtk_plan *p = b->plans[gi];
+        const int saved = p->fft_pairs;
+        if (b->fft_pairs) p->fft_pairs = b->fft_pairs;
+        int rc;
+        if (dir == 0)
+            rc = shtk_synthesis(p, in[gi], out[gi], b->nf[gi], dtype, 1, -1, st);
+        else
+            rc = shtk_analysis(p, in[gi], out[gi], b->nf[gi], dtype, -1, st);
+        p->fft_pairs = saved;
+        if (rc) return rc;
+    }
+    return SHTK_OK;
+}
+
+// Everything a call may allocate or build, done before any capture.
+int batch_prepare(shtk_batch *b, int dtype, hipStream_t st) {
+    for (int gi = 0; gi < b->ngroups; ++gi) {
+        shtk_plan *p = b->plans[gi];
+        std::lock_guard<std::mutex> lk(p->mu);
+        RC_TRY(ensure_ws(p, ws_layout(p, b->nf[gi], 0).total));
+        if (b->precision == SHTK_PREC_SPLIT) {
+            RC_TRY(ensure_oz_tables(p, st));
+            RC_TRY(ensure_oz_ws(p, oz_ws_bytes(p, b->nf[gi], !p->canonical || dtype != SHTK_F64)));
+        }
+    }
+    return SHTK_OK;
+}
+
+int batch_run(shtk_batch *b, int dir, const void *const *in, void *const *out, int dtype, void *stream) {
+    if (!b) return fail(SHTK_EINVAL, "batch is NULL");
+    if (!in || !out) return fail(SHTK_EINVAL, "array list is NULL");
+    if (dtype != SHTK_F64 && dtype != SHTK_F32) return fail(SHTK_EINVAL, "dtype must be SHTK_F64 or SHTK_F32");
+    for (int gi = 0; gi < b->ngroups; ++gi)
+        if (!in[gi] || !out[gi]) return fail(SHTK_EINVAL, "array pointer is NULL");
+    std::lock_guard<std::mutex> lk(b->mu);
+    DeviceGuard dg(b->device);
+    hipStream_t st = (hipStream_t)stream;
+    if (b->use_graph && !st) return fail(SHTK_EINVAL, "graph capture needs a non-default stream");
+    RC_TRY(batch_prepare(b, dtype, st));
+    if (!b->use_graph) return batch_issue(b, dir, in, out, dtype, st);
+    std::vector<const void *> key;
+    for (int gi = 0; gi < b->ngroups; ++gi) {
+        key.push_back(in[gi]);
+        key.push_back(out[gi]);
+    }
+    for (const BatchGraph &gr : b->graphs)
+        if (gr.dir == dir && gr.dtype == dtype && gr.stream == st && gr.ptrs == key) {
+            HIP_TRY(hipGraphLaunch(gr.exec, st), "graph launch");
+            return SHTK_OK;
+        }
+    for (shtk_plan *p : b->plans) p->profiling = false;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "graph capture begin");
+    const int rc = batch_issue(b, dir, in, out, dtype, st);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    HIP_TRY(ec, "graph capture end");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(ei, "graph instantiation");
+    b->graphs.push_back({dir, dtype, key, st, exec});
+    HIP_TRY(hipGraphLaunch(exec, st), "graph launch");
+    return SHTK_OK;
+}
+
+void batch_drop_graphs(shtk_batch *b) {
+    for (BatchGraph &gr : b->graphs) (void)hipGraphExecDestroy(gr.exec);
+    b->graphs.clear();
+}
+
 }  // namespace
 
 extern "C" {
+
+int shtk_batch_create(const shtk_desc *descs, const int *nfields, int ngroups, int precision, shtk_batch **out) {
+    if (!out) return fail(SHTK_EINVAL, "batch output pointer is NULL");
+    *out = nullptr;
+    if (!descs || !nfields) return fail(SHTK_EINVAL, "descs and nfields must not be NULL");
+    if (ngroups < 1 || ngroups > shtk::OZ_MAX_GROUPS)
+        return fail(SHTK_EINVAL, "ngroups must be between 1 and " + std::to_string(shtk::OZ_MAX_GROUPS));
+    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT)
+        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
+    for (int g = 0; g < ngroups; ++g) {
+        if (nfields[g] < 1) return fail(SHTK_EINVAL, "every group needs at least one field");
+        if (precision == SHTK_PREC_SPLIT && descs[g].lmax > 4095)
+            return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
+    }
+    shtk_batch *b = new (std::nothrow) shtk_batch();
+    if (!b) return fail(SHTK_ENOMEM, "host allocation failed");
+    b->ngroups = ngroups;
+    b->precision = precision;
+    (void)hipGetDevice(&b->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
+        b->cus = prop.multiProcessorCount;
+    for (int g = 0; g < ngroups; ++g) {
+        shtk_plan *p = nullptr;
+        const int rc = shtk_plan_create(&descs[g], &p);
+        if (rc) {
+            shtk_batch_destroy(b);
+            return rc;
+        }
+        b->plans.push_back(p);
+        b->nf.push_back(nfields[g]);
+        b->order.push_back(g);
+    }
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](int x, int y) { return b->plans[x]->d.lmax > b->plans[y]->d.lmax; });
+    if (hipMalloc((void **)&b->counter, 2 * sizeof(int)) != hipSuccess) {
+        shtk_batch_destroy(b);
+        return fail(SHTK_ENOMEM, "batch counter allocation failed");
+    }
+    *out = b;
+    return SHTK_OK;
+}
+
+int shtk_batch_destroy(shtk_batch *b) {
+    if (!b) return SHTK_OK;
+    {
+        DeviceGuard dg(b->device);
+        (void)hipDeviceSynchronize();
+        batch_drop_graphs(b);
+        if (b->counter) (void)hipFree(b->counter);
+    }
+    for (shtk_plan *p : b->plans) shtk_plan_destroy(p);
+    delete b;
+    return SHTK_OK;
+}
+
+int shtk_batch_plan(shtk_batch *b, int group, shtk_plan **plan) {
+    if (!b || !plan || group < 0 || group >= b->ngroups) return fail(SHTK_EINVAL, "bad batch group");
+    *plan = b->plans[group];
+    return SHTK_OK;
+}
+
+int shtk_batch_set_graph(shtk_batch *b, int on) {
+    if (!b) return fail(SHTK_EINVAL, "batch is NULL");
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (!on) batch_drop_graphs(b);
+    b->use_graph = on ? 1 : 0;
+    return SHTK_OK;
+}
+
+int shtk_batch_set_tiles(shtk_batch *b, int persistent_blocks, int fft_pairs) {
+    if (!b) return fail(SHTK_EINVAL, "batch is NULL");
+    if (persistent_blocks < 0 || (fft_pairs != 0 && fft_pairs != 1 && fft_pairs != 2 && fft_pairs != 4))
+        return fail(SHTK_EINVAL, "persistent_blocks must be >= 0 and fft_pairs 0, 1, 2 or 4");
+    std::lock_guard<std::mutex> lk(b->mu);
+    batch_drop_graphs(b);
+    b->persistent_blocks = persistent_blocks;
+    b->fft_pairs = fft_pairs;
+    return SHTK_OK;
+}
+
+int shtk_batch_synthesis(shtk_batch *b, const void *const *alm, void *const *f, int dtype, void *stream) {
+    return batch_run(b, 0, alm, f, dtype, stream);
+}
+
+int shtk_batch_analysis(shtk_batch *b, const void *const *f, void *const *alm, int dtype, void *stream) {
+    return batch_run(b, 1, f, alm, dtype, stream);
+}
+
+// Autotuner: times every candidate launch shape on the caller's arrays (round
+// trips: synthesis into f, analysis back into alm) and keeps the fastest.
+int shtk_batch_autotune(shtk_batch *b, void *const *alm, void *const *f, int dtype, int reps, void *stream,
+                        int *best_blocks, int *best_pairs, double *best_ms) {
+    if (!b || !alm || !f) return fail(SHTK_EINVAL, "batch and arrays must not be NULL");
+    if (reps < 1) reps = 1;
+    hipStream_t st = (hipStream_t)stream;
+    const int graph_was = b->use_graph;
+    RC_TRY(shtk_batch_set_graph(b, 0));
+    std::vector<int> blocks = {0};
+    if (b->precision == SHTK_PREC_SPLIT) blocks = {b->cus, 2 * b->cus, 3 * b->cus, 4 * b->cus};
+    const int pairs[] = {0, 1, 2, 4};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0), "autotune events");
+    HIP_TRY(hipEventCreate(&e1), "autotune events");
+    double best = 1e300;
+    int bb = 0, bp = 0;
+    for (int nb : blocks)
+        for (int fp : pairs) {
+            RC_TRY(shtk_batch_set_tiles(b, nb, fp));
+            // one untimed round trip, then reps timed ones
+            RC_TRY(shtk_batch_synthesis(b, (const void *const *)alm, f, dtype, st));
+            RC_TRY(shtk_batch_analysis(b, (const void *const *)f, alm, dtype, st));
+            HIP_TRY(hipEventRecord(e0, st), "autotune timing");
+            for (int r = 0; r < reps; ++r) {
+                RC_TRY(shtk_batch_synthesis(b, (const void *const *)alm, f, dtype, st));
+                RC_TRY(shtk_batch_analysis(b, (const void *const *)f, alm, dtype, st));
+            }
+            HIP_TRY(hipEventRecord(e1, st), "autotune timing");
+            HIP_TRY(hipEventSynchronize(e1), "autotune timing");
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1), "autotune timing");
+            if (ms / reps < best) {
+                best = ms / reps;
+                bb = nb;
+                bp = fp;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    RC_TRY(shtk_batch_set_tiles(b, bb, bp));
+    RC_TRY(shtk_batch_set_graph(b, graph_was));
+    if (best_blocks) *best_blocks = bb;
+    if (best_pairs) *best_pairs = bp;
+    if (best_ms) *best_ms = best;
+    return SHTK_OK;
+}
+
 
 const char *shtk_version(void) { return "0.2.0"; }
 
@@ -766,6 +1121,8 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     p->seg = segment_length();
     p->xcd_map = env_int("SHTK_XCD", 0) != 0;
     p->synth_chains = env_int("SHTK_SYNTH_CHAINS", 2);
+    p->fft_pairs = env_int("SHTK_FFT_PAIRS", 0);
+    if (p->fft_pairs != 1 && p->fft_pairs != 2 && p->fft_pairs != 4) p->fft_pairs = 0;
     p->canonical = d.norm == SHTK_NORM_ORTHONORMAL && !d.real_norm && d.cs_phase;
     p->inv_scale = d.phi_scale == SHTK_PHI_QUAD ? d.nlon / (2.0 * M_PI) : (double)d.nlon;
     p->scale_phi = (2.0 * M_PI / d.nlon) * d.nlon / p->inv_scale;

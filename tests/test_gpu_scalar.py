@@ -128,3 +128,26 @@ def test_errors(shtk, cuda):
         shtk.analysis(cfg, torch.zeros((8, 16), dtype=torch.float64))
     with pytest.raises(ValueError):
         shtk.create_gauss_config(7, 7)
+
+
+@pytest.mark.parametrize("pairs", ["1", "2", "4"])
+def test_fft_pairs_per_block(shtk, oracle, cuda, monkeypatch, pairs):
+    """Every FFT block shape (1, 2 or 4 latitude pairs per block) gives the same
+    transforms, including an odd nlat (equator row without a partner) and a
+    pair count that is not a multiple of the block shape."""
+    monkeypatch.setenv("SHTK_FFT_PAIRS", pairs)
+    lmax, nlat, nlon, nf = 40, 43, 90, 3
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+    ocfg = oracle.Config(lmax, nlat, nlon=nlon)
+    alm = oracle.random_alm(lmax, lmax, seed=17, power=1.0, nf=nf)
+    f = shtk.synthesis_batch(cfg, to_dev(alm))
+    assert rel_l2(to_host(f), ocfg.synthesis(alm)) < 1e-12
+    rng = np.random.default_rng(5)
+    g = np.asfortranarray(rng.standard_normal((nlat, nlon, nf)))
+    assert rel_l2(to_host(shtk.analysis_batch(cfg, to_dev(g))), ocfg.analysis(g)) < 1e-12
+    S = oracle.random_alm(lmax, lmax, 18, 2.0, nf=nf, lmin=1)
+    T = oracle.random_alm(lmax, lmax, 19, 2.0, nf=nf, lmin=1)
+    Vt, Vp = shtk.synthesis_sphtor_batch(cfg, to_dev(S), to_dev(T))
+    Vt_r, Vp_r = ocfg.synthesis_sphtor(S, T)
+    assert rel_l2(to_host(Vt), Vt_r) < 1e-12 and rel_l2(to_host(Vp), Vp_r) < 1e-12
+    cfg.close()

@@ -205,7 +205,7 @@ def measure_uneven(torch, np, shtk, dev, steps):
     b.close()
     cfgs3 = [shtk.create_gauss_config(L, L + 1, nlon=2 * L + 2) for L, _ in groups]
     g = shtk.UnevenBatch(cfgs3, nfs, precision="split", graph=True)
-    g.set_tiles(tune["persistent_blocks"], tune["fft_pairs"])
+    g.set_tiles(tune["synth_blocks"], tune["anal_blocks"])
     s = torch.cuda.Stream()
     torch.cuda.synchronize()
     with torch.cuda.stream(s):

@@ -169,16 +169,16 @@ int shtk_batch_plan(shtk_batch *batch, int group, shtk_plan **plan);
  * (per direction, arrays, dtype and stream; the stream must not be the
  * default stream) and replays it with a single graph launch afterwards. */
 int shtk_batch_set_graph(shtk_batch *batch, int on);
-/* Launch shapes: blocks of the persistent kernels (0: two per CU) and latitude
- * pairs per FFT block (0: automatic, or 1, 2, 4). */
-int shtk_batch_set_tiles(shtk_batch *batch, int persistent_blocks, int fft_pairs);
+/* Launch shapes: grid sizes of the persistent synthesis and analysis kernels
+ * (0: two blocks per CU). */
+int shtk_batch_set_tiles(shtk_batch *batch, int synth_blocks, int anal_blocks);
 int shtk_batch_synthesis(shtk_batch *batch, const void *const *alm, void *const *f, int dtype, void *stream);
 int shtk_batch_analysis(shtk_batch *batch, const void *const *f, void *const *alm, int dtype, void *stream);
 /* Autotuner: times round trips (synthesis into f, analysis back into alm, which
  * therefore ends up holding its own projection) for every candidate launch
  * shape and keeps the fastest; reports it if the pointers are not NULL. */
 int shtk_batch_autotune(shtk_batch *batch, void *const *alm, void *const *f, int dtype, int reps, void *stream,
-                        int *best_blocks, int *best_pairs, double *best_ms);
+                        int *best_synth_blocks, int *best_anal_blocks, double *best_ms);
 
 int shtk_profile_enable(shtk_plan *plan, int on);
 int shtk_profile_read(shtk_plan *plan, int max, char *names, double *total_ms, int *count, int *n_out);

@@ -81,9 +81,10 @@ class UnevenBatch:
                                                    _stream(fields[0].device)), "batch analysis")
         return outs
 
-    def set_tiles(self, persistent_blocks: int = 0, fft_pairs: int = 0) -> None:
-        _lib.check(_lib.load().shtk_batch_set_tiles(ctypes.c_void_p(self._b), int(persistent_blocks),
-                                                    int(fft_pairs)), "set_tiles")
+    def set_tiles(self, synth_blocks: int = 0, anal_blocks: int = 0) -> None:
+        """Grid sizes of the persistent synthesis / analysis kernels (0: two blocks per CU)."""
+        _lib.check(_lib.load().shtk_batch_set_tiles(ctypes.c_void_p(self._b), int(synth_blocks),
+                                                    int(anal_blocks)), "set_tiles")
 
     def autotune(self, alms: Sequence[torch.Tensor], fields: Sequence[torch.Tensor], reps: int = 3) -> dict:
         """Times every candidate launch shape on these arrays (round trips that
@@ -97,7 +98,7 @@ class UnevenBatch:
         _lib.check(_lib.load().shtk_batch_autotune(ctypes.c_void_p(self._b), ins, outs, _code(rdt), int(reps),
                                                    _stream(alms[0].device), ctypes.byref(nb), ctypes.byref(fp),
                                                    ctypes.byref(ms)), "autotune")
-        return {"persistent_blocks": nb.value, "fft_pairs": fp.value, "round_trip_ms": ms.value}
+        return {"synth_blocks": nb.value, "anal_blocks": fp.value, "round_trip_ms": ms.value}
 
     def close(self) -> None:
         if getattr(self, "_b", None):

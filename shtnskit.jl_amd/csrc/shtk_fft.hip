@@ -6,13 +6,13 @@
 // DFT.  The inverse here applies no 1/N: the Legendre stage folds
 // phi_inv_scale/nlon into its output factor.
 //
-// Design: one workgroup per (field, G consecutive latitude pairs), G = 1, 2
-// or 4.  The two real rows of a north/south pair travel as one complex row
-// z = f_N + i f_S, so a single complex FFT of length N serves both rows (no
-// separate real-FFT pass).  The G rows stay resident in LDS (16 G N bytes)
-// through a Stockham autosort FFT with radix 8/4/2/3/5 passes; twiddles come
-// from a per-plan table.  The grid is latitude-fastest, so G adjacent pairs
-// turn the longitude-major row accesses into runs of G elements.
+// Design: one workgroup per (field, latitude pair).  The two real rows of a
+// north/south pair travel as one complex row z = f_N + i f_S, so a single
+// complex FFT of length N serves both rows (no separate real-FFT pass).  The
+// row stays resident in LDS (16 N bytes) through a Stockham autosort FFT with
+// radix 8/4/2/3/5 passes; twiddles come from a per-plan table.  Consecutive
+// latitude pairs are mapped to the same XCD so the L2 merges the strided
+// longitude-major stores of neighbouring rows.
 #include "shtk_internal.h"
 
 namespace shtk {
@@ -89,30 +89,27 @@ __device__ __forceinline__ void dft_small(double2 *v) {
     }
 }
 
-// One Stockham pass of radix R over the G LDS rows z[g N + .] (length N
-// each).  Every thread stages all its inputs in registers before the barrier,
-// then writes.
-template <int R, int SIGN, int G>
+// One Stockham pass of radix R over the LDS row z (length N).  Every thread
+// stages all its inputs in registers before the barrier, then writes.
+template <int R, int SIGN>
 __device__ __forceinline__ void stockham_pass(double2 *z, int N, int Ns, const double2 *tw, int tid, int nt) {
-    constexpr int MAXB = (8 * G) / R + 1;  // butterflies per thread: G N / (R nt) with nt >= N / 8
+    constexpr int MAXB = 8 / R + 1;  // at most N/8 butterflies' worth of values per thread
     double2 v[MAXB][R];
-    const int nb = N / R, tot = G * nb;
+    const int nb = N / R;
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const int jj = tid + q * nt;
-        if (jj < tot) {
-            const int g = jj / nb, j = jj - g * nb;
+        const int j = tid + q * nt;
+        if (j < nb) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = z[g * N + j + r * nb];
+            for (int r = 0; r < R; ++r) v[q][r] = z[j + r * nb];
         }
     }
     __syncthreads();
     const int stride = N / (Ns * R);
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const int jj = tid + q * nt;
-        if (jj < tot) {
-            const int g = jj / nb, j = jj - g * nb;
+        const int j = tid + q * nt;
+        if (j < nb) {
             const int k = j % Ns;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
@@ -121,7 +118,7 @@ __device__ __forceinline__ void stockham_pass(double2 *z, int N, int Ns, const d
                 v[q][r] = cmul(v[q][r], w);
             }
             dft_small<R, SIGN>(v[q]);
-            const int base = g * N + (j / Ns) * Ns * R + k;
+            const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) z[base + r * Ns] = v[q][r];
         }
@@ -129,167 +126,113 @@ __device__ __forceinline__ void stockham_pass(double2 *z, int N, int Ns, const d
     __syncthreads();
 }
 
-template <int SIGN, int G>
+template <int SIGN>
 __device__ void fft_lds(double2 *z, const FftArgs &a, int tid, int nt) {
     const int N = a.nlon;
     int Ns = 1;
     for (int q = 0; q < a.nrad; ++q) {
         const int R = (int)((a.radix >> (4 * q)) & 15ull);
         switch (R) {
-            case 8: stockham_pass<8, SIGN, G>(z, N, Ns, a.tw, tid, nt); break;
-            case 4: stockham_pass<4, SIGN, G>(z, N, Ns, a.tw, tid, nt); break;
-            case 2: stockham_pass<2, SIGN, G>(z, N, Ns, a.tw, tid, nt); break;
-            case 3: stockham_pass<3, SIGN, G>(z, N, Ns, a.tw, tid, nt); break;
-            default: stockham_pass<5, SIGN, G>(z, N, Ns, a.tw, tid, nt); break;
+            case 8: stockham_pass<8, SIGN>(z, N, Ns, a.tw, tid, nt); break;
+            case 4: stockham_pass<4, SIGN>(z, N, Ns, a.tw, tid, nt); break;
+            case 2: stockham_pass<2, SIGN>(z, N, Ns, a.tw, tid, nt); break;
+            case 3: stockham_pass<3, SIGN>(z, N, Ns, a.tw, tid, nt); break;
+            default: stockham_pass<5, SIGN>(z, N, Ns, a.tw, tid, nt); break;
         }
         Ns *= R;
     }
 }
 
-__device__ __forceinline__ double row_factor(const FftArgs &a, int p) {
-    if (a.mode == 1) {
-        const double st = a.s[p];
-        return st > 0.0 ? 1.0 / st : 1.0;
-    }
-    return a.mode == 2 ? a.s[p] : 1.0;
+// Block index -> latitude pair, keeping runs of consecutive pairs on one XCD
+// (workgroups are dispatched round-robin over the 8 XCDs).
+__device__ __forceinline__ int pair_of_block(int b, int np) {
+    const int per = (np + 7) / 8;
+    const int xcd = b & 7, slot = b >> 3;
+    const int p = xcd * per + slot;
+    return p;
 }
 
-// Synthesis: one workgroup per (G consecutive latitude pairs, field).  North
-// rows p0..p0+G-1 and their south partners are adjacent in memory, so the
-// Fourier columns are read and the grid rows written in G-element runs.
-// Blocks run with at most NT threads: 256 (rows up to 2048 points, any G) or
-// 1024 (longer rows, G = 1).
-template <typename GT, int G, int NT>
-__global__ __launch_bounds__(NT) void k_fft_synth(const FftArgs a) {
+template <typename GT>
+__global__ __launch_bounds__(1024) void k_fft_synth(const FftArgs a) {
     extern __shared__ double2 z[];
-    const int p0 = blockIdx.x * G;
+    const int p = pair_of_block(blockIdx.x, a.np);
+    if (p >= a.np) return;
     const int f = blockIdx.y;
-    const int N = a.nlon, nlat = a.nlat, np = a.np;
+    const int N = a.nlon, nlat = a.nlat;
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int q = tid; q < G * N; q += nt) z[q] = make_double2(0.0, 0.0);
+    const int iN = p, iS = nlat - 1 - p;
+    const bool hasS = iS != iN;
+    for (int q = tid; q < N; q += nt) z[q] = make_double2(0.0, 0.0);
     __syncthreads();
     const double2 *Ff = a.F + (size_t)f * a.nm * nlat;
-    // z = f_N + i f_S has the spectrum Z_m = F_N + i F_S, Z_{N-m} = conj(F_N) + i conj(F_S):
-    // north terms first (unit (im, g), G consecutive latitudes per order) ...
-    for (int u = tid; u < a.nm * G; u += nt) {
-        const int g = u % G, im = u / G;
-        const int p = p0 + g;
-        if (p >= np) continue;
-        const double2 FN = Ff[(size_t)im * nlat + p];
+    for (int im = tid; im < a.nm; im += nt) {
         const int m = im * a.mres;
-        double2 *zr = z + g * N;
+        const double2 FN = Ff[(size_t)im * nlat + iN];
+        const double2 FS = hasS ? Ff[(size_t)im * nlat + iS] : make_double2(0.0, 0.0);
         if (m == 0) {
-            zr[0].x = FN.x;
+            z[0] = make_double2(FN.x, FS.x);
         } else {
-            zr[m] = FN;
-            zr[N - m] = make_double2(FN.x, -FN.y);
+            z[m] = make_double2(FN.x - FS.y, FN.y + FS.x);
+            z[N - m] = make_double2(FN.x + FS.y, FS.x - FN.y);
         }
     }
     __syncthreads();
-    // ... then south terms (the G partner latitudes, also consecutive)
-    for (int u = tid; u < a.nm * G; u += nt) {
-        const int g = u % G, im = u / G;
-        const int p = p0 + (G - 1 - g);
-        if (p >= np) continue;
-        const int lat = nlat - 1 - p;
-        const bool hasS = lat != p;
-        const double2 FS = hasS ? Ff[(size_t)im * nlat + lat] : make_double2(0.0, 0.0);
-        const int m = im * a.mres;
-        double2 *zr = z + (p - p0) * N;
-        if (m == 0) {
-            zr[0].y = FS.x;
-        } else {
-            zr[m].x -= FS.y;
-            zr[m].y += FS.x;
-            zr[N - m].x += FS.y;
-            zr[N - m].y += FS.x;
-        }
+    fft_lds<+1>(z, a, tid, nt);
+    GT *g = (GT *)a.grid + (size_t)f * nlat * N;
+    double sc = 1.0;
+    if (a.mode == 1) {
+        const double st = a.s[p];
+        sc = st > 0.0 ? 1.0 / st : 1.0;
+    } else if (a.mode == 2) {
+        sc = a.s[p];
     }
-    __syncthreads();
-    fft_lds<+1, G>(z, a, tid, nt);
-    GT *gr = (GT *)a.grid + (size_t)f * nlat * N;
-    // unit (j, h, g): longitude j, north (h = 0) or south row of pair p0 + g
-    for (int u = tid; u < N * 2 * G; u += nt) {
-        const int g = u % G, h = (u / G) & 1, j = u / (2 * G);
-        const int p = h == 0 ? p0 + g : p0 + (G - 1 - g);
-        if (p >= np) continue;
-        const int lat = h == 0 ? p : nlat - 1 - p;
-        if (h == 1 && lat == p) continue;
-        const double2 v = z[(p - p0) * N + j];
-        gr[(size_t)j * nlat + lat] = (GT)((h == 0 ? v.x : v.y) * row_factor(a, p));
+    for (int j = tid; j < N; j += nt) {
+        const double2 v = z[j];
+        g[(size_t)j * nlat + iN] = (GT)(v.x * sc);
+        if (hasS) g[(size_t)j * nlat + iS] = (GT)(v.y * sc);
     }
 }
 
-// Analysis: the mirror of the synthesis; z = f_N + i f_S per pair, forward
-// FFT, then F_N = (Z_m + conj Z_{N-m}) / 2, F_S = (Z_m - conj Z_{N-m}) / (2i).
-template <typename GT, int G, int NT>
-__global__ __launch_bounds__(NT) void k_fft_anal(const FftArgs a) {
+template <typename GT>
+__global__ __launch_bounds__(1024) void k_fft_anal(const FftArgs a) {
     extern __shared__ double2 z[];
-    const int p0 = blockIdx.x * G;
+    const int p = pair_of_block(blockIdx.x, a.np);
+    if (p >= a.np) return;
     const int f = blockIdx.y;
-    const int N = a.nlon, nlat = a.nlat, np = a.np;
+    const int N = a.nlon, nlat = a.nlat;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const GT *gr = (const GT *)a.grid + (size_t)f * nlat * N;
-    for (int u = tid; u < N * 2 * G; u += nt) {
-        const int g = u % G, h = (u / G) & 1, j = u / (2 * G);
-        const int p = h == 0 ? p0 + g : p0 + (G - 1 - g);
-        if (p >= np) continue;
-        const int lat = h == 0 ? p : nlat - 1 - p;
-        double v = 0.0;
-        if (h == 0 || lat != p) v = (double)gr[(size_t)j * nlat + lat] * row_factor(a, p);
-        double2 *zr = z + (p - p0) * N + j;
-        if (h == 0) zr->x = v;
-        else zr->y = v;
+    const int iN = p, iS = nlat - 1 - p;
+    const bool hasS = iS != iN;
+    const GT *g = (const GT *)a.grid + (size_t)f * nlat * N;
+    double sc = 1.0;
+    if (a.mode == 1) {
+        const double st = a.s[p];
+        sc = st > 0.0 ? 1.0 / st : 1.0;
+    } else if (a.mode == 2) {
+        sc = a.s[p];
+    }
+    for (int j = tid; j < N; j += nt) {
+        const double vN = (double)g[(size_t)j * nlat + iN];
+        const double vS = hasS ? (double)g[(size_t)j * nlat + iS] : 0.0;
+        z[j] = make_double2(vN * sc, vS * sc);
     }
     __syncthreads();
-    fft_lds<-1, G>(z, a, tid, nt);
+    fft_lds<-1>(z, a, tid, nt);
     double2 *Ff = a.F + (size_t)f * a.nm * nlat;
-    for (int u = tid; u < a.nm * 2 * G; u += nt) {
-        const int g = u % G, h = (u / G) & 1, im = u / (2 * G);
-        const int p = h == 0 ? p0 + g : p0 + (G - 1 - g);
-        if (p >= np) continue;
-        const int lat = h == 0 ? p : nlat - 1 - p;
-        if (h == 1 && lat == p) continue;
+    for (int im = tid; im < a.nm; im += nt) {
         const int m = im * a.mres;
-        const double2 *zr = z + (p - p0) * N;
-        const double2 Zm = zr[m];
-        const double2 Zc = zr[(N - m) % N];
-        Ff[(size_t)im * nlat + lat] = h == 0 ? make_double2(0.5 * (Zm.x + Zc.x), 0.5 * (Zm.y - Zc.y))
-                                             : make_double2(0.5 * (Zm.y + Zc.y), -0.5 * (Zm.x - Zc.x));
+        const double2 Zm = z[m];
+        const double2 Zc = z[(N - m) % N];  // conj taken below
+        // FN = (Zm + conj(Zc)) / 2, FS = (Zm - conj(Zc)) / (2i)
+        const double2 FN = make_double2(0.5 * (Zm.x + Zc.x), 0.5 * (Zm.y - Zc.y));
+        const double2 FS = make_double2(0.5 * (Zm.y + Zc.y), -0.5 * (Zm.x - Zc.x));
+        Ff[(size_t)im * nlat + iN] = FN;
+        if (hasS) Ff[(size_t)im * nlat + iS] = FS;
     }
-}
-
-template <int G, int NT>
-hipError_t launch_fft_g(const FftArgs &a, int nfields, int inverse, int nt, hipStream_t st) {
-    const size_t lds = sizeof(double2) * (size_t)G * a.nlon;
-    dim3 grid((a.np + G - 1) / G, nfields);
-    const void *fn = inverse ? (a.io_f32 ? (const void *)k_fft_synth<float, G, NT>
-                                         : (const void *)k_fft_synth<double, G, NT>)
-                             : (a.io_f32 ? (const void *)k_fft_anal<float, G, NT>
-                                         : (const void *)k_fft_anal<double, G, NT>);
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    if (inverse) {
-        if (a.io_f32)
-            hipLaunchKernelGGL((k_fft_synth<float, G, NT>), grid, dim3(nt), lds, st, a);
-        else
-            hipLaunchKernelGGL((k_fft_synth<double, G, NT>), grid, dim3(nt), lds, st, a);
-    } else {
-        if (a.io_f32)
-            hipLaunchKernelGGL((k_fft_anal<float, G, NT>), grid, dim3(nt), lds, st, a);
-        else
-            hipLaunchKernelGGL((k_fft_anal<double, G, NT>), grid, dim3(nt), lds, st, a);
-    }
-    return hipGetLastError();
 }
 
 }  // namespace
 
-// Pairs per block: 4 or 2 when the rows are short enough (N <= 2048, G N
-// complex values within 128 KB of LDS) and the grid keeps at least two blocks
-// per CU, else 1.
 hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
     if (a.nrad <= 0) return hipErrorNotSupported;
     const int N = a.nlon;
@@ -298,19 +241,28 @@ hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st
     if (nt > 1024) nt = 1024;
     nt = (nt + 63) / 64 * 64;
     if (N > 8 * nt) return hipErrorNotSupported;
-    if (sizeof(double2) * (size_t)N > 160 * 1024) return hipErrorNotSupported;
-    const long long pairs = (long long)a.np * nfields;
-    int G = 1;
-    if (a.fft_pairs > 0) G = a.fft_pairs;
-    else if (pairs / 4 >= 512) G = 4;
-    else if (pairs / 2 >= 512) G = 2;
-    if (nt > 256 || sizeof(double2) * (size_t)G * N > 128 * 1024) G = 1;
-    if (nt > 256) return launch_fft_g<1, 1024>(a, nfields, inverse, nt, st);
-    switch (G) {
-        case 4: return launch_fft_g<4, 256>(a, nfields, inverse, nt, st);
-        case 2: return launch_fft_g<2, 256>(a, nfields, inverse, nt, st);
-        default: return launch_fft_g<1, 256>(a, nfields, inverse, nt, st);
+    const size_t lds = sizeof(double2) * (size_t)N;
+    if (lds > 160 * 1024) return hipErrorNotSupported;
+    const int per = (a.np + 7) / 8;
+    dim3 grid(per * 8, nfields);
+    if (lds > 64 * 1024) {
+        const void *fn = inverse ? (a.io_f32 ? (const void *)k_fft_synth<float> : (const void *)k_fft_synth<double>)
+                                 : (a.io_f32 ? (const void *)k_fft_anal<float> : (const void *)k_fft_anal<double>);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
     }
+    if (inverse) {
+        if (a.io_f32)
+            hipLaunchKernelGGL(k_fft_synth<float>, grid, dim3(nt), lds, st, a);
+        else
+            hipLaunchKernelGGL(k_fft_synth<double>, grid, dim3(nt), lds, st, a);
+    } else {
+        if (a.io_f32)
+            hipLaunchKernelGGL(k_fft_anal<float>, grid, dim3(nt), lds, st, a);
+        else
+            hipLaunchKernelGGL(k_fft_anal<double>, grid, dim3(nt), lds, st, a);
+    }
+    return hipGetLastError();
 }
 
 }  // namespace shtk

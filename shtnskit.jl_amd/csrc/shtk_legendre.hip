@@ -203,25 +203,39 @@ __global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0.0;
     const int nseg = nseg_of(lmax, m, S);
-    for (int s = w; s < nseg; s += 4) {
-        const int l0 = m + s * S;
-        if (l0 > ltr) break;
-        const int gs = a.segbase[im] + s;
-        if (pb1 <= a.pfirst[gs]) continue;
-        const bool fast = pb0 >= a.pfast[gs];
-        const int lstop = min(l0 + S, ltr + 1);
-        const double c0 = s == 0 ? 1.0 : ab[l0].x;  // P_{l0} = a_{l0} Q_{l0}
-        double prev[K], cur[K];
-        int kk[K];
+    // segment seeds of the wave's next segment are loaded while the current one runs
+    double nq0[K], nq1[K];
+    int nqk[K];
+    auto load_seeds = [&](int s) {
+        const int gs = a.segbase[im] + min(s, nseg - 1);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const int p = pb0 + lane + 64 * j;
             const bool act = p < np;
             const size_t q = (size_t)gs * np + (act ? p : 0);
-            prev[j] = act ? a.seedq0[q] : 0.0;
-            cur[j] = act ? a.seedq1[q] * c0 : 0.0;
-            kk[j] = act ? a.seedk[q] : 0;
+            nq0[j] = act ? a.seedq0[q] : 0.0;
+            nq1[j] = act ? a.seedq1[q] : 0.0;
+            nqk[j] = act ? a.seedk[q] : 0;
         }
+    };
+    if (w < nseg) load_seeds(w);
+    for (int s = w; s < nseg; s += 4) {
+        const int l0 = m + s * S;
+        if (l0 > ltr) break;
+        const int gs = a.segbase[im] + s;
+        double prev[K], cur[K];
+        int kk[K];
+        const double c0 = s == 0 ? 1.0 : ab[l0].x;  // P_{l0} = a_{l0} Q_{l0}
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            prev[j] = nq0[j];
+            cur[j] = nq1[j] * c0;
+            kk[j] = nqk[j];
+        }
+        if (s + 4 < nseg) load_seeds(s + 4);
+        if (pb1 <= a.pfirst[gs]) continue;
+        const bool fast = pb0 >= a.pfast[gs];
+        const int lstop = min(l0 + S, ltr + 1);
         if (fast) synth_seg<K, true>(ab, al, l0, lstop, lmax, x, prev, cur, kk, acc);
         else synth_seg<K, false>(ab, al, l0, lstop, lmax, x, prev, cur, kk, acc);
     }

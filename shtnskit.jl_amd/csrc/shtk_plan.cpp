@@ -246,7 +246,6 @@ shtk::FftArgs fft_args(const shtk_plan *p) {
     for (size_t q = 0; q < p->radix.size(); ++q) f.radix |= (unsigned long long)p->radix[q] << (4 * q);
     f.tw = p->tw;
     f.s = p->s;
-    f.fft_pairs = p->fft_pairs;
     return f;
 }
 
@@ -576,15 +575,14 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
 
 int segment_length() {
     const char *e = std::getenv("SHTK_SEG");
-    int s = e ? std::atoi(e) : 128;
-    if (s < 8 || s > shtk::SEG_MAX || (s & 7)) s = 128;
+    int s = e ? std::atoi(e) : 64;
+    if (s < 8 || s > shtk::SEG_MAX || (s & 7)) s = 64;
     return s;
 }
 
 // Launch-shape knobs read once per plan: SHTK_XCD=1 keeps the pair blocks of
 // an order on one XCD in the synthesis kernels, SHTK_SYNTH_CHAINS (1, 2 or 4)
-// sets the latitude pairs per lane of the scalar synthesis, SHTK_FFT_PAIRS
-// (1, 2 or 4) the latitude pairs per FFT block (default: chosen per call).
+// sets the latitude pairs per lane of the scalar synthesis.
 int env_int(const char *name, int dflt) {
     const char *e = std::getenv(name);
     return e ? std::atoi(e) : dflt;
@@ -612,8 +610,7 @@ struct shtk_batch {
     std::vector<int> nf;
     std::vector<int> order;           // groups by descending lmax (work order of the persistent kernels)
     int *counter = nullptr;           // device work counters (synthesis, analysis)
-    int persistent_blocks = 0;        // 0: 2 per CU
-    int fft_pairs = 0;                // FFT latitude pairs per block (0: automatic)
+    int blocks_synth = 0, blocks_anal = 0;  // persistent kernels' grid sizes (0: two blocks per CU)
     int device = 0, cus = 256;
     std::vector<BatchGraph> graphs;
     std::mutex mu;
@@ -621,8 +618,9 @@ struct shtk_batch {
 
 namespace {
 
-int batch_blocks(const shtk_batch *b) {
-    return b->persistent_blocks > 0 ? b->persistent_blocks : 2 * b->cus;
+int batch_blocks(const shtk_batch *b, int dir) {
+    const int n = dir == 0 ? b->blocks_synth : b->blocks_anal;
+    return n > 0 ? n : 2 * b->cus;
 }
 
 // The grouped split launches of one direction (tables and workspaces ready).
@@ -661,14 +659,13 @@ int batch_split(shtk_batch *b, int dir, const void *const *in, void *const *out,
         if (any_f32)
             for (int q = 0; q < n; ++q)
                 if (!g1.args[q].in_f32) return fail(SHTK_EUNSUPPORTED, "split batch: mixed input precisions");
-        HIP_TRY(shtk::launch_oz_synth_grp(g1, g2, g3, any_f32, batch_blocks(b), st), "split batch synthesis");
+        HIP_TRY(shtk::launch_oz_synth_grp(g1, g2, g3, any_f32, batch_blocks(b, 0), st), "split batch synthesis");
         for (int gi = 0; gi < n; ++gi) {
             shtk_plan *p = b->plans[gi];
             shtk::FftArgs fa = fft_args(p);
             fa.F = (double2 *)p->ws;
             fa.grid = out[gi];
             fa.io_f32 = dtype == SHTK_F32;
-            if (b->fft_pairs) fa.fft_pairs = b->fft_pairs;
             HIP_TRY(shtk::launch_fft(fa, b->nf[gi], 1, st), "phi synthesis launch");
         }
         return SHTK_OK;
@@ -679,7 +676,6 @@ int batch_split(shtk_batch *b, int dir, const void *const *in, void *const *out,
         fa.F = (double2 *)p->ws;
         fa.grid = (void *)in[gi];
         fa.io_f32 = dtype == SHTK_F32;
-        if (b->fft_pairs) fa.fft_pairs = b->fft_pairs;
         HIP_TRY(shtk::launch_fft(fa, b->nf[gi], 0, st), "phi analysis launch");
     }
     for (int q = 0; q < n; ++q) {
@@ -697,7 +693,7 @@ int batch_split(shtk_batch *b, int dir, const void *const *in, void *const *out,
         g2.prefix[q + 1] = g2.prefix[q] + o.nks * o.nm * o.nfg;
         g3.prefix[q + 1] = g3.prefix[q] + o.nhalf * o.nseg_total * o.nfg;
     }
-    HIP_TRY(shtk::launch_oz_anal_grp(g1, g2, g3, batch_blocks(b), st), "split batch analysis");
+    HIP_TRY(shtk::launch_oz_anal_grp(g1, g2, g3, batch_blocks(b, 1), st), "split batch analysis");
     for (int gi = 0; gi < n; ++gi) {
         shtk_plan *p = b->plans[gi];
         if (p->canonical && dtype == SHTK_F64) continue;
@@ -713,15 +709,8 @@ int batch_split(shtk_batch *b, int dir, const void *const *in, void *const *out,
 int batch_issue(shtk_batch *b, int dir, const void *const *in, void *const *out, int dtype, hipStream_t st) {
     if (b->precision == SHTK_PREC_SPLIT) return batch_split(b, dir, in, out, dtype, st);
     for (int gi = 0; gi < b->ngroups; ++gi) {
-        shtk_plan *p = b->plans[gi];
-        const int saved = p->fft_pairs;
-        if (b->fft_pairs) p->fft_pairs = b->fft_pairs;
-        int rc;
-        if (dir == 0)
-            rc = shtk_synthesis(p, in[gi], out[gi], b->nf[gi], dtype, 1, -1, st);
-        else
-            rc = shtk_analysis(p, in[gi], out[gi], b->nf[gi], dtype, -1, st);
-        p->fft_pairs = saved;
+        const int rc = dir == 0 ? shtk_synthesis(b->plans[gi], in[gi], out[gi], b->nf[gi], dtype, 1, -1, st)
+                                : shtk_analysis(b->plans[gi], in[gi], out[gi], b->nf[gi], dtype, -1, st);
         if (rc) return rc;
     }
     return SHTK_OK;
@@ -860,14 +849,13 @@ int shtk_batch_set_graph(shtk_batch *b, int on) {
     return SHTK_OK;
 }
 
-int shtk_batch_set_tiles(shtk_batch *b, int persistent_blocks, int fft_pairs) {
+int shtk_batch_set_tiles(shtk_batch *b, int synth_blocks, int anal_blocks) {
     if (!b) return fail(SHTK_EINVAL, "batch is NULL");
-    if (persistent_blocks < 0 || (fft_pairs != 0 && fft_pairs != 1 && fft_pairs != 2 && fft_pairs != 4))
-        return fail(SHTK_EINVAL, "persistent_blocks must be >= 0 and fft_pairs 0, 1, 2 or 4");
+    if (synth_blocks < 0 || anal_blocks < 0) return fail(SHTK_EINVAL, "block counts must be >= 0");
     std::lock_guard<std::mutex> lk(b->mu);
     batch_drop_graphs(b);
-    b->persistent_blocks = persistent_blocks;
-    b->fft_pairs = fft_pairs;
+    b->blocks_synth = synth_blocks;
+    b->blocks_anal = anal_blocks;
     return SHTK_OK;
 }
 
@@ -882,22 +870,22 @@ int shtk_batch_analysis(shtk_batch *b, const void *const *f, void *const *alm, i
 // Autotuner: times every candidate launch shape on the caller's arrays (round
 // trips: synthesis into f, analysis back into alm) and keeps the fastest.
 int shtk_batch_autotune(shtk_batch *b, void *const *alm, void *const *f, int dtype, int reps, void *stream,
-                        int *best_blocks, int *best_pairs, double *best_ms) {
+                        int *best_synth_blocks, int *best_anal_blocks, double *best_ms) {
     if (!b || !alm || !f) return fail(SHTK_EINVAL, "batch and arrays must not be NULL");
     if (reps < 1) reps = 1;
     hipStream_t st = (hipStream_t)stream;
     const int graph_was = b->use_graph;
     RC_TRY(shtk_batch_set_graph(b, 0));
+    // candidate grids of the two persistent kernels: 1..4 blocks per CU each
     std::vector<int> blocks = {0};
     if (b->precision == SHTK_PREC_SPLIT) blocks = {b->cus, 2 * b->cus, 3 * b->cus, 4 * b->cus};
-    const int pairs[] = {0, 1, 2, 4};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0), "autotune events");
     HIP_TRY(hipEventCreate(&e1), "autotune events");
     double best = 1e300;
     int bb = 0, bp = 0;
     for (int nb : blocks)
-        for (int fp : pairs) {
+        for (int fp : blocks) {
             RC_TRY(shtk_batch_set_tiles(b, nb, fp));
             // one untimed round trip, then reps timed ones
             RC_TRY(shtk_batch_synthesis(b, (const void *const *)alm, f, dtype, st));
@@ -921,8 +909,8 @@ int shtk_batch_autotune(shtk_batch *b, void *const *alm, void *const *f, int dty
     (void)hipEventDestroy(e1);
     RC_TRY(shtk_batch_set_tiles(b, bb, bp));
     RC_TRY(shtk_batch_set_graph(b, graph_was));
-    if (best_blocks) *best_blocks = bb;
-    if (best_pairs) *best_pairs = bp;
+    if (best_synth_blocks) *best_synth_blocks = bb;
+    if (best_anal_blocks) *best_anal_blocks = bp;
     if (best_ms) *best_ms = best;
     return SHTK_OK;
 }
@@ -1120,9 +1108,7 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     p->nm = d.mmax / d.mres + 1;
     p->seg = segment_length();
     p->xcd_map = env_int("SHTK_XCD", 0) != 0;
-    p->synth_chains = env_int("SHTK_SYNTH_CHAINS", 2);
-    p->fft_pairs = env_int("SHTK_FFT_PAIRS", 0);
-    if (p->fft_pairs != 1 && p->fft_pairs != 2 && p->fft_pairs != 4) p->fft_pairs = 0;
+    p->synth_chains = env_int("SHTK_SYNTH_CHAINS", 4);
     p->canonical = d.norm == SHTK_NORM_ORTHONORMAL && !d.real_norm && d.cs_phase;
     p->inv_scale = d.phi_scale == SHTK_PHI_QUAD ? d.nlon / (2.0 * M_PI) : (double)d.nlon;
     p->scale_phi = (2.0 * M_PI / d.nlon) * d.nlon / p->inv_scale;

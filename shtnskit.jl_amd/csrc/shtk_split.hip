@@ -225,14 +225,27 @@ __device__ __forceinline__ void synth_oz_body(const OzArgs &a, int pbk, int im, 
 #pragma unroll
             for (int v = 0; v < OZ_SLICES; ++v) acc[par][ct][v] = v4i{0, 0, 0, 0};
 
-    for (int rho = 0; rho < nch; ++rho) {
-        __syncthreads();
+    // the next chunk's coefficients and B operands travel in registers while the
+    // current chunk computes (OZ_CHUNK_V4 / 256 = 6 operands per thread)
+    constexpr int NB = OZ_CHUNK_V4 / 256;
+    v4i bnext[NB];
+    double2 abnext = make_double2(0.0, 0.0);
+    auto fetch = [&](int rho) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) bnext[k] = bsrc[(size_t)rho * OZ_CHUNK_V4 + tid + 256 * k];
         if (tid < 128) {
             const int l = m + (tid >> 5) * Kq + 32 * rho + (tid & 31);
-            abch[tid] = ab[min(l, lmax) + 1];
+            abnext = ab[min(l, lmax) + 1];
         }
-        for (int q = tid; q < OZ_CHUNK_V4; q += 256) bch[q] = bsrc[(size_t)rho * OZ_CHUNK_V4 + q];
+    };
+    fetch(0);
+    for (int rho = 0; rho < nch; ++rho) {
         __syncthreads();
+        if (tid < 128) abch[tid] = abnext;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) bch[tid + 256 * k] = bnext[k];
+        __syncthreads();
+        if (rho + 1 < nch) fetch(rho + 1);
 
         v4i A[2][OZ_SLICES];
         // a lane still in the scaled range contributes zeros (scale 0 below);

@@ -95,7 +95,7 @@ def test_batch_autotune(shtk, oracle, cuda):
     choice = b.autotune(work_a, work_f, reps=2)
     print("autotune choice:", choice)
     assert choice["round_trip_ms"] > 0
-    assert choice["fft_pairs"] in (0, 1, 2, 4)
+    assert choice["synth_blocks"] >= 0 and choice["anal_blocks"] >= 0
     fs2 = b.synthesis(d)
     for oc, a, f in zip(ocfgs, alms, fs2):
         assert rel_l2(to_host(f), oc.synthesis(a)) < TOL_SPLIT

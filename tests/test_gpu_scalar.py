@@ -130,12 +130,9 @@ def test_errors(shtk, cuda):
         shtk.create_gauss_config(7, 7)
 
 
-@pytest.mark.parametrize("pairs", ["1", "2", "4"])
-def test_fft_pairs_per_block(shtk, oracle, cuda, monkeypatch, pairs):
-    """Every FFT block shape (1, 2 or 4 latitude pairs per block) gives the same
-    transforms, including an odd nlat (equator row without a partner) and a
-    pair count that is not a multiple of the block shape."""
-    monkeypatch.setenv("SHTK_FFT_PAIRS", pairs)
+def test_batch_odd_nlat_all_transforms(shtk, oracle, cuda):
+    """A batch on an odd nlat (equator row without a partner) through the
+    scalar and vector transforms."""
     lmax, nlat, nlon, nf = 40, 43, 90, 3
     cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
     ocfg = oracle.Config(lmax, nlat, nlon=nlon)

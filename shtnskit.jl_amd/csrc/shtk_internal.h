@@ -165,7 +165,9 @@ struct shtk_plan {
     double *oz_q0 = nullptr, *oz_q1 = nullptr;  // [nm][4][np]
     int *oz_qk = nullptr, *oz_ep = nullptr, *oz_chbase = nullptr, *oz_chm = nullptr;
     int *oz_emaxl = nullptr, *oz_epl = nullptr;  // per (order, degree), qc indexing
-    double *oz_asc = nullptr;
+    double *oz_asc = nullptr, *oz_qb = nullptr, *oz_qcc = nullptr;
+    int4 *oz_run = nullptr;  // analysis work runs: (order, first segment, segments, 0)
+    int oz_nrun = 0, oz_run_len = 8;
     size_t qc_len = 0;
     int oz_ch_total = 0;
     void *oz_ws = nullptr;
@@ -210,14 +212,18 @@ struct OzArgs {
     const double2 *qc;      // Q-form (beta_l, c_l) per order (offset coff[im])
     const double *asc;      // c_l 2^(46 - epl_l) per order: digit scale of degree l
     const int *epl;         // |P_l| < 2^epl over all pairs, per order
+    const double *qb, *qcc; // beta_l and c_l of qc (padded by OZ_TPAD)
     const double *seedq0, *seedq1;
     const int *seedk, *segbase, *segm, *pfirst;
-    int seg, nseg_total, nks, nhalf;
+    int seg, nseg_total, nks;
+    int nh, npb;            // analysis pair blocks: nh halves of 256 pairs each, npb blocks
     const double *wlat;
     double *spec_out;       // FP64 canonical coefficients, accumulated (pre-zeroed)
+    const int4 *run;        // [nrun] (order, first segment, segment count, 0)
+    int nrun;
     int4 *gdig;             // [nfg][nm][nks][2][6][2][64] B operands (pairs as k)
     double *gcolsum;        // [nfg][nm][nks][2][32]
-    int *eg;                // [nfg][nm][2][32]
+    int *eg;                // [nfg][nm][npb][2][32] column exponents per pair block
 };
 
 struct OzSeedArgs {
@@ -232,12 +238,14 @@ struct OzSeedArgs {
     int *emaxl;             // per order and degree: max over pairs of ilogb|P_l| (atomicMax)
 };
 
+constexpr int OZ_TPAD = 64;  // entries past the end of the per-degree split tables
 struct OzAscArgs {
     int n;                  // table length (all orders)
     const double2 *qc;
     const int *emaxl;
     double *asc;
     int *epl;
+    double *qb, *qcc;       // beta_l and c_l of qc as separate arrays
 };
 
 // Several configurations in one launch per stage, passed by value as kernel
@@ -255,8 +263,7 @@ struct OzGroups {
 hipError_t launch_oz_seed(const OzSeedArgs &a, hipStream_t st);
 hipError_t launch_oz_synth_grp(const OzGroups &bexp, const OzGroups &bdig, const OzGroups &units, int in_f32,
                                int persistent_blocks, hipStream_t st);
-hipError_t launch_oz_anal_grp(const OzGroups &gexp, const OzGroups &gdig, const OzGroups &units,
-                              int persistent_blocks, hipStream_t st);
+hipError_t launch_oz_anal_grp(const OzGroups *gprep, const OzGroups *units, int persistent_blocks, hipStream_t st);
 hipError_t launch_oz_synth(const OzArgs &a, hipStream_t st);
 hipError_t launch_oz_asc(const OzAscArgs &a, hipStream_t st);
 hipError_t launch_oz_anal(const OzArgs &a, hipStream_t st);

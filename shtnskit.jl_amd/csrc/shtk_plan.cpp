@@ -122,7 +122,7 @@ void free_plan(shtk_plan *p) {
                     p->qd,     p->coff,   p->segbase, p->seedq0, p->seedq1, p->seedk, p->pfirst,
                     p->pfast,  p->segm,   p->convscale, p->tw,   p->ws,     p->sectv, p->sectk, p->oz_q0,
                     p->oz_q1,  p->oz_qk,  p->oz_ep,     p->oz_chbase, p->oz_ws, p->oz_emaxl, p->oz_epl,
-                    p->oz_asc, p->oz_chm};
+                    p->oz_asc, p->oz_chm, p->oz_run, p->oz_qb, p->oz_qcc};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -290,8 +290,25 @@ int ensure_oz_tables(shtk_plan *p, hipStream_t st) {
     RC_TRY(upload(&p->oz_chm, chm, "split tables"));
     p->oz_ch_total = total;
     RC_TRY(upload(&p->oz_emaxl, std::vector<int>(p->qc_len, -100000), "split tables"));
-    HIP_TRY(hipMalloc((void **)&p->oz_epl, sizeof(int) * p->qc_len), "split tables");
-    HIP_TRY(hipMalloc((void **)&p->oz_asc, sizeof(double) * p->qc_len), "split tables");
+    const size_t npad = p->qc_len + shtk::OZ_TPAD;
+    HIP_TRY(hipMalloc((void **)&p->oz_epl, sizeof(int) * npad), "split tables");
+    HIP_TRY(hipMalloc((void **)&p->oz_asc, sizeof(double) * npad), "split tables");
+    HIP_TRY(hipMalloc((void **)&p->oz_qb, sizeof(double) * npad), "split tables");
+    HIP_TRY(hipMalloc((void **)&p->oz_qcc, sizeof(double) * npad), "split tables");
+    // analysis work runs: up to oz_run_len consecutive segments of one order,
+    // orders of equal run count split evenly, lowest order (longest) first
+    std::vector<int4> runs;
+    for (int im = 0, gs = 0; im < nm; ++im) {
+        const int nseg = (lmax - im * p->d.mres + p->seg) / p->seg;
+        const int nr = (nseg + p->oz_run_len - 1) / p->oz_run_len;
+        for (int r = 0; r < nr; ++r) {
+            const int s0 = r * nseg / nr, s1 = (r + 1) * nseg / nr;
+            runs.push_back(make_int4(im, gs + s0, s1 - s0, 0));
+        }
+        gs += nseg;
+    }
+    RC_TRY(upload(&p->oz_run, runs, "split tables"));
+    p->oz_nrun = (int)runs.size();
     shtk::OzSeedArgs sa{};
     sa.lmax = lmax;
     sa.mres = p->d.mres;
@@ -314,6 +331,8 @@ int ensure_oz_tables(shtk_plan *p, hipStream_t st) {
     aa.emaxl = p->oz_emaxl;
     aa.asc = p->oz_asc;
     aa.epl = p->oz_epl;
+    aa.qb = p->oz_qb;
+    aa.qcc = p->oz_qcc;
     HIP_TRY(shtk::launch_oz_asc(aa, st), "split scale kernel");
     HIP_TRY(hipStreamSynchronize(st), "split seed kernel");
     return SHTK_OK;
@@ -323,9 +342,13 @@ struct OzWs {
     size_t bdig, colsum, eb, total;
 };
 
-// ksteps of 64 pairs in the split analysis: whole halves of 256 pairs, so that
-// every LDS position of a half has a field operand (zero beyond np)
-int oz_nks(const shtk_plan *p) { return 4 * ((p->np + 255) / 256); }
+// Split analysis pair blocks: one workgroup contracts 256 nh pairs (nh = 2
+// when np > 256, so that lmax <= 1023 needs one block per order and no
+// atomics); ksteps of 64 pairs cover whole blocks, so every LDS position of a
+// block has a field operand (zero beyond np).
+int oz_nh(const shtk_plan *p) { return p->np > 256 ? 2 : 1; }
+int oz_npb(const shtk_plan *p) { return (p->np + 256 * oz_nh(p) - 1) / (256 * oz_nh(p)); }
+int oz_nks(const shtk_plan *p) { return 4 * oz_nh(p) * oz_npb(p); }
 
 // synthesis: B operands per (field group, chunk); analysis: per (field group,
 // order, kstep of 64 pairs) plus an FP64 canonical accumulator when the output
@@ -341,7 +364,7 @@ OzWs oz_ws_layout(const shtk_plan *p, int nfg, bool analysis) {
     } else {
         L.bdig = al((size_t)nfg * p->nm * nks * shtk::OZ_CHUNK_V4 * 16);
         L.colsum = al(sizeof(double) * (size_t)nfg * p->nm * nks * 2 * shtk::OZ_COLS);
-        L.eb = al(sizeof(int) * (size_t)nfg * p->nm * 2 * shtk::OZ_COLS);
+        L.eb = al(sizeof(int) * (size_t)nfg * p->nm * oz_npb(p) * 2 * shtk::OZ_COLS);
     }
     L.total = L.bdig + L.colsum + L.eb;
     return L;
@@ -409,6 +432,8 @@ shtk::OzArgs oz_anal_args(const shtk_plan *p, double2 *F0, double *acc, int nf, 
     o.qc = p->qc;
     o.asc = p->oz_asc;
     o.epl = p->oz_epl;
+    o.qb = p->oz_qb;
+    o.qcc = p->oz_qcc;
     o.seedq0 = p->seedq0;
     o.seedq1 = p->seedq1;
     o.seedk = p->seedk;
@@ -418,7 +443,10 @@ shtk::OzArgs oz_anal_args(const shtk_plan *p, double2 *F0, double *acc, int nf, 
     o.seg = p->seg;
     o.nseg_total = p->nseg_total;
     o.nks = oz_nks(p);
-    o.nhalf = o.nks / 4;
+    o.nh = oz_nh(p);
+    o.npb = oz_npb(p);
+    o.run = p->oz_run;
+    o.nrun = p->oz_nrun;
     o.wlat = wlat;
     o.spec_out = acc;
     const OzWs W = oz_ws_layout(p, o.nfg, true);
@@ -609,7 +637,7 @@ struct shtk_batch {
     std::vector<shtk_plan *> plans;   // in caller order
     std::vector<int> nf;
     std::vector<int> order;           // groups by descending lmax (work order of the persistent kernels)
-    int *counter = nullptr;           // device work counters (synthesis, analysis)
+    int *counter = nullptr;           // device work counters (synthesis, analysis x 2 pair-block sizes)
     int blocks_synth = 0, blocks_anal = 0;  // persistent kernels' grid sizes (0: two blocks per CU)
     int device = 0, cus = 256;
     std::vector<BatchGraph> graphs;
@@ -628,7 +656,10 @@ int batch_split(shtk_batch *b, int dir, const void *const *in, void *const *out,
     shtk::OzGroups g1{}, g2{}, g3{};
     const int n = b->ngroups;
     g1.ngroups = g2.ngroups = g3.ngroups = n;
-    g3.counter = b->counter + dir;
+    g3.counter = b->counter;
+    shtk::OzGroups gprep[2]{}, units[2]{};
+    units[0].counter = b->counter + 1;
+    units[1].counter = b->counter + 2;
     std::vector<double *> accs(n);
     if (dir == 0) {
         for (int q = 0; q < n; ++q) {
@@ -688,12 +719,16 @@ int batch_split(shtk_batch *b, int dir, const void *const *in, void *const *out,
         accs[gi] = acc;
         HIP_TRY(shtk::launch_zero(acc, (size_t)nf * (p->d.lmax + 1) * (p->d.mmax + 1) * 16, st), "output clear");
         const shtk::OzArgs o = oz_anal_args(p, (double2 *)p->ws, acc, nf, p->d.lmax, p->wscaled, 1.0);
-        g1.args[q] = g2.args[q] = g3.args[q] = o;
-        g1.prefix[q + 1] = g1.prefix[q] + o.nm * o.nfg;
-        g2.prefix[q + 1] = g2.prefix[q] + o.nks * o.nm * o.nfg;
-        g3.prefix[q + 1] = g3.prefix[q] + o.nhalf * o.nseg_total * o.nfg;
+        // one launch pair per pair-block size: [0] two halves, [1] one half
+        const int k = o.nh == 2 ? 0 : 1;
+        shtk::OzGroups &gp = gprep[k], &gu = units[k];
+        const int at = gu.ngroups++;
+        gp.ngroups = gu.ngroups;
+        gp.args[at] = gu.args[at] = o;
+        gp.prefix[at + 1] = gp.prefix[at] + o.npb * o.nm * o.nfg;
+        gu.prefix[at + 1] = gu.prefix[at] + o.npb * o.nrun * o.nfg;
     }
-    HIP_TRY(shtk::launch_oz_anal_grp(g1, g2, g3, batch_blocks(b, 1), st), "split batch analysis");
+    HIP_TRY(shtk::launch_oz_anal_grp(gprep, units, batch_blocks(b, 1), st), "split batch analysis");
     for (int gi = 0; gi < n; ++gi) {
         shtk_plan *p = b->plans[gi];
         if (p->canonical && dtype == SHTK_F64) continue;
@@ -814,7 +849,7 @@ int shtk_batch_create(const shtk_desc *descs, const int *nfields, int ngroups, i
     }
     std::stable_sort(b->order.begin(), b->order.end(),
                      [&](int x, int y) { return b->plans[x]->d.lmax > b->plans[y]->d.lmax; });
-    if (hipMalloc((void **)&b->counter, 2 * sizeof(int)) != hipSuccess) {
+    if (hipMalloc((void **)&b->counter, 3 * sizeof(int)) != hipSuccess) {
         shtk_batch_destroy(b);
         return fail(SHTK_ENOMEM, "batch counter allocation failed");
     }
@@ -1109,6 +1144,7 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     p->seg = segment_length();
     p->xcd_map = env_int("SHTK_XCD", 0) != 0;
     p->synth_chains = env_int("SHTK_SYNTH_CHAINS", 4);
+    p->oz_run_len = std::max(1, env_int("SHTK_OZ_RUN", 8));
     p->canonical = d.norm == SHTK_NORM_ORTHONORMAL && !d.real_norm && d.cs_phase;
     p->inv_scale = d.phi_scale == SHTK_PHI_QUAD ? d.nlon / (2.0 * M_PI) : (double)d.nlon;
     p->scale_phi = (2.0 * M_PI / d.nlon) * d.nlon / p->inv_scale;

@@ -32,9 +32,21 @@ constexpr double kBias = 551911719040.0;  // 0x8080808080
 // 2^52 + 2^47: adding it to |v| < 2^46 leaves the integer 2^47 + round(v) in the
 // low 48 mantissa bits (exact: the sum stays in [2^52, 2^53))
 constexpr double kMagic = 0x1p52 + 0x1p47;
-constexpr int OZ_ROWB = 272;               // analysis LDS row stride (256 pairs + 16 bytes)
+// 2^52 + 0x808080808080: the same for balanced base-256 digits (see k_oz_gprep)
+constexpr double kMagicB = 0x1p52 + 141289400074368.0;
 
 __device__ __forceinline__ int oz_nch(int lmax, int m) { return (lmax - m + 128) / 128; }
+
+// Read-only tables read through the constant address space: wave-uniform reads
+// become scalar loads (counted apart from the vector memory operations) even in
+// kernels that also write global memory, so waiting for them never waits for
+// the kernel's own output stores and atomics.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) {
+    return (cptr<T>)p;
+}
 
 // ---------------------------------------------------------------------------
 // Plan-time seeds at the quarter starts and per-pair exponents: one thread per
@@ -91,89 +103,110 @@ __global__ __launch_bounds__(256) void k_oz_seed(const OzSeedArgs a) {
     a.ep[sidx] = emax < -201 ? -200 : min(emax + 1, 64);
 }
 
+// byte b of v0..v3 packed into one word (v0 in the low byte)
+__device__ __forceinline__ unsigned pack4(unsigned v0, unsigned v1, unsigned v2, unsigned v3, int b) {
+    const unsigned sel = (unsigned)b | ((unsigned)(b + 4) << 8) | 0x0c0c0000u;
+    const unsigned t01 = __builtin_amdgcn_perm(v1, v0, sel);
+    const unsigned t23 = __builtin_amdgcn_perm(v3, v2, sel);
+    return __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+}
+
 // ---------------------------------------------------------------------------
 // Field operand of the synthesis.  k_oz_bexp: one block per (order, field
-// group) finds the column exponents eb (|a| < 2^eb over the order's degrees)
-// and clears the column sums.  k_oz_bdig: one block per (field group, chunk)
-// writes one 4-byte word of each of the 6 digit planes per thread-unit, in the
-// B-operand layout
+// group); wave w reads the coefficient columns of fields 4w..4w+3 with lanes
+// over consecutive degrees, finds the column exponents eb (|a| < 2^eb over the
+// order's degrees) and clears the column sums.  k_oz_bdig: one block per
+// (field group, chunk) stages the chunk's 16 fields x 4 quarters x 32 degrees
+// in LDS with coalesced reads, then writes one 4-byte word of each of the 6
+// digit planes per thread-unit, in the B-operand layout
 //   [chunk][parity][slice t][column tile ct][lane][16 bytes],
 // lane = c + 16 g (column c of the tile, quarter g), byte j <-> degree
 // l = m + g Kq + 32 chunk + 2 j + parity, and adds its integer column sums.
+// Digits come from the kMagicB FMA as in the analysis operand.
 // ---------------------------------------------------------------------------
 template <typename IT>
-__device__ __forceinline__ const IT *oz_col(const OzArgs &a, int fg, int m, int c, bool *ok) {
-    const int f = fg * OZ_FIELDS + (c >> 1);
-    *ok = f < a.nf;
-    return (const IT *)a.spec_in + 2 * ((size_t)(*ok ? f : 0) * a.spec_fstride + (size_t)(a.lmax + 1) * m) + (c & 1);
-}
-
-template <typename IT>
-__device__ __forceinline__ void oz_bexp_body(const OzArgs &a, int im, int fg, double *cmax) {
+__device__ __forceinline__ void oz_bexp_body(const OzArgs &a, int im, int fg) {
     const int m = im * a.mres;
     const int lend = min(a.lmax, a.ltr);
-    const int tid = threadIdx.x;
-    const int c = tid & (OZ_COLS - 1), sub = tid >> 5;
-    bool ok;
-    const IT *src = oz_col<IT>(a, fg, m, c, &ok);
-    double mx = 0.0;
-    if (ok)
-        for (int l = m + sub; l <= lend; l += 8) mx = fmax(mx, fabs((double)src[2 * l]));
-    cmax[tid] = mx;
-    __syncthreads();
-    if (tid < OZ_COLS) {
-        double v = 0.0;
-        for (int s = 0; s < 8; ++s) v = fmax(v, cmax[s * OZ_COLS + tid]);
-        a.eb[((size_t)fg * a.nm + im) * OZ_COLS + tid] = v > 0.0 ? ilogb(v) + 1 : 0;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int fl = 4 * w + k, f = fg * OZ_FIELDS + fl;
+        double mr = 0.0, mi = 0.0;
+        if (f < a.nf) {
+            const IT *src = (const IT *)a.spec_in + 2 * ((size_t)f * a.spec_fstride + (size_t)(a.lmax + 1) * m);
+            for (int l = m + lane; l <= lend; l += 64) {
+                mr = fmax(mr, fabs((double)src[2 * l]));
+                mi = fmax(mi, fabs((double)src[2 * l + 1]));
+            }
+        }
+#pragma unroll
+        for (int msk = 32; msk >= 1; msk >>= 1) {
+            mr = fmax(mr, __shfl_xor(mr, msk));
+            mi = fmax(mi, __shfl_xor(mi, msk));
+        }
+        if (lane < 2) {
+            const double v = lane ? mi : mr;
+            a.eb[((size_t)fg * a.nm + im) * OZ_COLS + 2 * fl + lane] = v > 0.0 ? ilogb(v) + 1 : 0;
+        }
     }
     if (tid < 2 * OZ_COLS) a.colsum[((size_t)fg * a.nm + im) * 2 * OZ_COLS + tid] = 0ull;
 }
 
 template <typename IT>
 __global__ __launch_bounds__(256) void k_oz_bexp(const OzArgs a) {
-    __shared__ double cmax[256];
-    oz_bexp_body<IT>(a, blockIdx.x % a.nm, blockIdx.x / a.nm, cmax);
+    oz_bexp_body<IT>(a, blockIdx.x % a.nm, blockIdx.x / a.nm);
 }
 
+constexpr int OZ_BSTG = 129;  // LDS stride of one field's 4 x 32 staged values (+1 against bank conflicts)
+
 template <typename IT>
-__device__ __forceinline__ void oz_bdig_body(const OzArgs &a, int gc, int fg, unsigned long long (*csum)[OZ_COLS]) {
+__device__ __forceinline__ void oz_bdig_body(const OzArgs &a, int gc, int fg, double2 *stg,
+                                             unsigned long long (*csum)[OZ_COLS]) {
     const int im = a.chm[gc], rho = gc - a.chbase[im];
     const int m = im * a.mres, lmax = a.lmax;
     const int Kq = 32 * oz_nch(lmax, m);
     const int lend = min(lmax, a.ltr);
     const int tid = threadIdx.x;
     if (tid < 2 * OZ_COLS) csum[tid >> 5][tid & 31] = 0ull;
+    // stage: element e = (field, quarter, degree offset) with the degree fastest
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int e = tid + 256 * k;
+        const int l32 = e & 31, g = (e >> 5) & 3, fl = e >> 7;
+        const int f = fg * OZ_FIELDS + fl;
+        const int l = m + g * Kq + 32 * rho + l32;
+        double2 v = make_double2(0.0, 0.0);
+        if (f < a.nf && l <= lend) {
+            const IT *src = (const IT *)a.spec_in + 2 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m + l);
+            v = make_double2((double)src[0], (double)src[1]);
+        }
+        stg[fl * OZ_BSTG + g * 32 + l32] = v;
+    }
     __syncthreads();
     unsigned *dst = (unsigned *)(a.bdig + ((size_t)fg * a.ch_total + gc) * OZ_CHUNK_V4);
     const int *ebm = a.eb + ((size_t)fg * a.nm + im) * OZ_COLS;
     for (int u = tid; u < 1024; u += 256) {
         const int d = u & 3, lane = (u >> 2) & 63, ct = (u >> 8) & 1, par = u >> 9;
         const int g = lane >> 4, c = (lane & 15) + 16 * ct;
-        bool ok;
-        const IT *src = oz_col<IT>(a, fg, m, c, &ok);
         const double sc = ldexp(1.0, 46 - ebm[c]);
-        unsigned dig[OZ_SLICES];
-#pragma unroll
-        for (int t = 0; t < OZ_SLICES; ++t) dig[t] = 0u;
-        long long ysum = 0;
+        const double *sv = (const double *)(stg + (c >> 1) * OZ_BSTG + g * 32) + (c & 1);
+        unsigned hw[4], lw[4];
+        double ys = 0.0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int j = 4 * d + q;
-            const int l = m + g * Kq + 32 * rho + 2 * j + par;
-            long long Y = 0;
-            if (ok && l <= lend) Y = llrint((double)src[2 * l] * sc);
-            ysum += Y;
-#pragma unroll
-            for (int t = OZ_SLICES - 1; t >= 0; --t) {
-                const long long dd = ((Y + 128) & 255) - 128;
-                Y = (Y - dd) >> 8;
-                dig[t] |= ((unsigned)dd & 255u) << (8 * q);
-            }
+            const int l32 = 2 * (4 * d + q) + par;
+            const double r = fma(sv[2 * l32], sc, kMagicB);
+            ys += r - kMagicB;
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(r);
+            hw[q] = (unsigned)(bits >> 32) ^ 0x8080u;
+            lw[q] = (unsigned)bits ^ 0x80808080u;
         }
-        atomicAdd(&csum[par][c], (unsigned long long)ysum);
+        atomicAdd(&csum[par][c], (unsigned long long)(long long)ys);
 #pragma unroll
         for (int t = 0; t < OZ_SLICES; ++t)
-            dst[((size_t)((par * OZ_SLICES + t) * 2 + ct) * 64 + lane) * 4 + d] = dig[t];
+            dst[((size_t)((par * OZ_SLICES + t) * 2 + ct) * 64 + lane) * 4 + d] =
+                t < 2 ? pack4(hw[0], hw[1], hw[2], hw[3], 1 - t) : pack4(lw[0], lw[1], lw[2], lw[3], 5 - t);
     }
     __syncthreads();
     if (tid < 2 * OZ_COLS)
@@ -182,16 +215,9 @@ __device__ __forceinline__ void oz_bdig_body(const OzArgs &a, int gc, int fg, un
 
 template <typename IT>
 __global__ __launch_bounds__(256) void k_oz_bdig(const OzArgs a) {
+    __shared__ double2 stg[OZ_FIELDS * OZ_BSTG];
     __shared__ unsigned long long csum[2][OZ_COLS];
-    oz_bdig_body<IT>(a, blockIdx.x % a.ch_total, blockIdx.x / a.ch_total, csum);
-}
-
-// byte b of v0..v3 packed into one word (v0 in the low byte)
-__device__ __forceinline__ unsigned pack4(unsigned v0, unsigned v1, unsigned v2, unsigned v3, int b) {
-    const unsigned sel = (unsigned)b | ((unsigned)(b + 4) << 8) | 0x0c0c0000u;
-    const unsigned t01 = __builtin_amdgcn_perm(v1, v0, sel);
-    const unsigned t23 = __builtin_amdgcn_perm(v3, v2, sel);
-    return __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+    oz_bdig_body<IT>(a, blockIdx.x % a.ch_total, blockIdx.x / a.ch_total, stg, csum);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,244 +370,365 @@ __global__ __launch_bounds__(256) void k_synth_oz(const OzArgs a) {
 // Per-degree digit scales for the analysis: |P_l| < 2^epl over all pairs, and
 // asc = c_l 2^(46 - epl) so that one FMA turns Q_l into the digit value.
 // ---------------------------------------------------------------------------
+// The same kernel also splits the Q-form table into contiguous beta_l and c_l
+// arrays for the analysis, and fills OZ_TPAD entries past the end of every
+// table (so the analysis can read whole 8- and 32-degree blocks unclamped).
 __global__ __launch_bounds__(256) void k_oz_asc(const OzAscArgs a) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.n) return;
+    if (i >= a.n + OZ_TPAD) return;
+    if (i >= a.n) {
+        a.epl[i] = 0;
+        a.asc[i] = 0.0;
+        a.qb[i] = 0.0;
+        a.qcc[i] = 0.0;
+        return;
+    }
     const int e = a.emaxl[i];
     const int ep = e < -201 ? -200 : min(e + 1, 64);
+    const double2 q = a.qc[i];
     a.epl[i] = ep;
-    a.asc[i] = a.qc[i].y * ldexp(1.0, 46 - ep);
+    a.asc[i] = q.y * ldexp(1.0, 46 - ep);
+    a.qb[i] = q.x;
+    a.qcc[i] = q.y;
 }
 
 // ---------------------------------------------------------------------------
 // Analysis field operand.  G+- = w (F_N +- F_S) per pair (even degrees use G+,
-// odd G-).  k_oz_gexp: one block per (order, field group) finds the column
-// exponents eg per parity.  k_oz_gdig: one block per (kstep of 64 pairs,
-// order, field group) writes balanced digits in the B-operand layout
+// odd G-), written as balanced signed base-256 digits in the B-operand layout
 //   [kstep][parity][slice t][column tile ct][lane][16 bytes]
-// and the kstep's integer column sums.  Within a half of 256 pairs, LDS
-// position q (kstep 64 (ks & 3) + 16 g + byte j) holds local pair
-// (q & 3) * 64 + (q >> 2): the order in which the analysis kernel's lanes write
-// their digits without bank conflicts.
+// with one column exponent per (pair block, parity, column) and the integer
+// column sums of every kstep.  A pair block is the 256 NH latitude pairs that
+// one analysis workgroup contracts (NH = 1 or 2 halves of 256 pairs).  Within
+// a block, K slot q (kstep q >> 6, lane group (q >> 4) & 3, byte q & 15) holds
+// local pair 64 (q % NW) + q / NW, NW = 4 NH: the order in which the analysis
+// waves write their digit bytes to LDS without bank conflicts.
+//
+// k_oz_gprep: one workgroup of 256 NH threads per (pair block, order, field
+// group).  Thread (jj = tid % 16 NH, field tid / 16 NH) holds the 16 slots of
+// one B lane word (kstep jj >> 2 of the block, lane group jj & 3), so the
+// Fourier data is read once and each digit word is written once from
+// registers; the column maximum is a shuffle reduction over the 16 NH threads
+// of a field and the kstep column sums one over 4 threads.
+//
+// Digits: with kMagicB = 2^52 + 0x808080808080, fma(v, 2^(46-e), kMagicB)
+// leaves Y + 0x808080808080 in the low 48 mantissa bits (Y = round(v 2^(46-e)),
+// |Y| <= 2^46); its bytes are b_k + 128 for the balanced digits b_k of Y, so a
+// XOR with 0x80 per byte gives the int8 digits, and Y itself is the exact
+// difference of the FMA result and kMagicB.
 // ---------------------------------------------------------------------------
 constexpr int OZ_MAX_KS = 32;  // np <= 2048
 
-__device__ __forceinline__ double oz_gval(const OzArgs &a, int fg, int im, int c, int p, int par) {
-    const int f = fg * OZ_FIELDS + (c >> 1);
-    if (f >= a.nf || p >= a.np) return 0.0;
-    const double2 *Fi = a.F0 + ((size_t)f * a.nm + im) * a.nlat;
-    const double2 FN = Fi[p];
-    const int ps = a.nlat - 1 - p;
-    const double2 FS = ps != p ? Fi[ps] : make_double2(0.0, 0.0);
-    const double n = (c & 1) ? FN.y : FN.x, s = (c & 1) ? FS.y : FS.x;
-    const double wl = a.wlat[p];
-    return par ? wl * (n - s) : wl * (n + s);
+template <int NH>
+__device__ __forceinline__ int oz_pair_of_slot(int q) {
+    return 64 * (q % (4 * NH)) + q / (4 * NH);
 }
 
-__device__ __forceinline__ int oz_pair_of_slot(int ks, int slot) {
-    const int q = 64 * (ks & 3) + slot;
-    return (ks >> 2) * 256 + (q & 3) * 64 + (q >> 2);
-}
-
-__device__ __forceinline__ void oz_gexp_body(const OzArgs &a, int im, int fg, double (*cmax)[256]) {
-    const int tid = threadIdx.x;
-    const int c = tid & (OZ_COLS - 1), sub = tid >> 5;
-    double m0 = 0.0, m1 = 0.0;
-    for (int p = sub; p < a.np; p += 8) {
-        m0 = fmax(m0, fabs(oz_gval(a, fg, im, c, p, 0)));
-        m1 = fmax(m1, fabs(oz_gval(a, fg, im, c, p, 1)));
-    }
-    cmax[0][tid] = m0;
-    cmax[1][tid] = m1;
-    __syncthreads();
-    if (tid < 2 * OZ_COLS) {
-        const int par = tid >> 5, cc = tid & 31;
-        double v = 0.0;
-        for (int s = 0; s < 8; ++s) v = fmax(v, cmax[par][s * OZ_COLS + cc]);
-        a.eg[((size_t)fg * a.nm + im) * 2 * OZ_COLS + tid] = v > 0.0 ? ilogb(v) + 1 : 0;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_oz_gexp(const OzArgs a) {
-    __shared__ double cmax[2][256];
-    oz_gexp_body(a, blockIdx.x % a.nm, blockIdx.x / a.nm, cmax);
-}
-
-__device__ __forceinline__ void oz_gdig_body(const OzArgs &a, int ks, int im, int fg,
-                                             unsigned long long (*csum)[OZ_COLS]) {
-    const int tid = threadIdx.x;
-    if (tid < 2 * OZ_COLS) csum[tid >> 5][tid & 31] = 0ull;
-    __syncthreads();
+template <int NH>
+__device__ __forceinline__ void oz_gprep_body(const OzArgs &a, int pb, int im, int fg) {
+    constexpr int TPF = 16 * NH;  // threads per field
+    const int tid = threadIdx.x, jj = tid % TPF, fl = tid / TPF;
+    const int f = fg * OZ_FIELDS + fl;
+    const bool fok = f < a.nf;
+    const double2 *Fi = a.F0 + ((size_t)(fok ? f : 0) * a.nm + im) * a.nlat;
+    const int ksl = jj >> 2, g = jj & 3;
+    double v[2][2][16];  // [parity][re, im][4 j + d]: byte j of word d
+    double mx[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int p = pb * 256 * NH + oz_pair_of_slot<NH>(64 * ksl + 16 * g + 4 * d + j);
+            double2 FN = make_double2(0.0, 0.0), FS = make_double2(0.0, 0.0);
+            double wl = 0.0;
+            if (fok && p < a.np) {
+                FN = Fi[p];
+                const int ps = a.nlat - 1 - p;
+                if (ps != p) FS = Fi[ps];
+                wl = a.wlat[p];
+            }
+            const int i = 4 * j + d;
+            v[0][0][i] = wl * (FN.x + FS.x);
+            v[0][1][i] = wl * (FN.y + FS.y);
+            v[1][0][i] = wl * (FN.x - FS.x);
+            v[1][1][i] = wl * (FN.y - FS.y);
+#pragma unroll
+            for (int par = 0; par < 2; ++par)
+#pragma unroll
+                for (int ri = 0; ri < 2; ++ri) mx[par][ri] = fmax(mx[par][ri], fabs(v[par][ri][i]));
+        }
+    const int ks = pb * 4 * NH + ksl;
     const size_t ubase = ((size_t)fg * a.nm + im) * a.nks;
-    const int *egm = a.eg + ((size_t)fg * a.nm + im) * 2 * OZ_COLS;
-    unsigned *dst = (unsigned *)(a.gdig + (ubase + ks) * OZ_CHUNK_V4);
-    for (int u = tid; u < 1024; u += 256) {
-        const int d = u & 3, lane = (u >> 2) & 63, ct = (u >> 8) & 1, par = u >> 9;
-        const int g = lane >> 4, c = (lane & 15) + 16 * ct;
-        const double sc = ldexp(1.0, 46 - egm[par * OZ_COLS + c]);
-        unsigned dig[OZ_SLICES];
+    v4i *dst = (v4i *)a.gdig + (ubase + ks) * OZ_CHUNK_V4;
 #pragma unroll
-        for (int t = 0; t < OZ_SLICES; ++t) dig[t] = 0u;
-        long long ysum = 0;
+    for (int par = 0; par < 2; ++par)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int p = oz_pair_of_slot(ks, 16 * g + 4 * d + q);
-            long long Y = llrint(oz_gval(a, fg, im, c, p, par) * sc);
-            ysum += Y;
+        for (int ri = 0; ri < 2; ++ri) {
+            double m = mx[par][ri];
 #pragma unroll
-            for (int t = OZ_SLICES - 1; t >= 0; --t) {
-                const long long dd = ((Y + 128) & 255) - 128;
-                Y = (Y - dd) >> 8;
-                dig[t] |= ((unsigned)dd & 255u) << (8 * q);
+            for (int msk = 1; msk < TPF; msk <<= 1) m = fmax(m, __shfl_xor(m, msk));
+            const int e = m > 0.0 ? ilogb(m) + 1 : 0;
+            const int c = 2 * fl + ri;
+            if (jj == 0) a.eg[((((size_t)fg * a.nm + im) * a.npb + pb) * 2 + par) * OZ_COLS + c] = e;
+            const double sc = ldexp(1.0, 46 - e);
+            unsigned hw[16], lw[16];
+            double ys = 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const double r = fma(v[par][ri][i], sc, kMagicB);
+                ys += r - kMagicB;
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(r);
+                hw[i] = (unsigned)(bits >> 32) ^ 0x8080u;
+                lw[i] = (unsigned)bits ^ 0x80808080u;
+            }
+            ys += __shfl_xor(ys, 1);
+            ys += __shfl_xor(ys, 2);
+            if (g == 0) a.gcolsum[((ubase + ks) * 2 + par) * OZ_COLS + c] = ys;
+            // word d of the lane holds bytes j = 0..3 (j in byte j)
+            v4i *row = dst + ((par * OZ_SLICES) * 2 + (c >> 4)) * 64 + (c & 15) + 16 * g;
+#pragma unroll
+            for (int t = 0; t < OZ_SLICES; ++t) {
+                v4i wd;
+#pragma unroll
+                for (int d = 0; d < 4; ++d)
+                    wd[d] = (int)(t < 2 ? pack4(hw[d], hw[4 + d], hw[8 + d], hw[12 + d], 1 - t)
+                                        : pack4(lw[d], lw[4 + d], lw[8 + d], lw[12 + d], 5 - t));
+                row[t * 2 * 64] = wd;
             }
         }
-        atomicAdd(&csum[par][c], (unsigned long long)ysum);
-#pragma unroll
-        for (int t = 0; t < OZ_SLICES; ++t)
-            dst[((size_t)((par * OZ_SLICES + t) * 2 + ct) * 64 + lane) * 4 + d] = dig[t];
-    }
-    __syncthreads();
-    if (tid < 2 * OZ_COLS)
-        a.gcolsum[((ubase + ks) * 2) * OZ_COLS + tid] = (double)(long long)csum[tid >> 5][tid & 31];
 }
 
-__global__ __launch_bounds__(256) void k_oz_gdig(const OzArgs a) {
-    __shared__ unsigned long long csum[2][OZ_COLS];
-    oz_gdig_body(a, blockIdx.x % a.nks, (blockIdx.x / a.nks) % a.nm, blockIdx.x / (a.nks * a.nm), csum);
+template <int NH>
+__global__ __launch_bounds__(256 * NH) void k_oz_gprep(const OzArgs a) {
+    oz_gprep_body<NH>(a, blockIdx.x % a.npb, (blockIdx.x / a.npb) % a.nm, blockIdx.x / (a.npb * a.nm));
 }
 
 // ---------------------------------------------------------------------------
-// Analysis: one block of 4 waves per (degree segment, half of the pairs,
-// field group).  Every lane runs the Q-form recurrence of one pair from the
-// segment seeds and writes the 6 digit bytes of each degree to LDS as
+// Analysis: one workgroup of 4 NH waves per (pair block, run of degree
+// segments, field group).  Every lane runs the Q-form recurrence of one pair
+// and writes the 6 digit bytes of each degree to LDS as
 // [parity][slice][row][pair] (row = degree pair index within 32 degrees);
-// wave w then contracts parity w >> 1, column tile w & 1 over the block's 4
-// ksteps of 64 pairs (the field operand stays in registers for the whole
-// segment) and adds its rows to the FP64 output.
+// wave w then contracts parity (w >> 1) & 1, column tile w & 1 over the K half
+// w >> 2 of the block (the field operand of its 4 ksteps stays in registers
+// for the whole run).  With two halves the waves of a parity / column tile
+// exchange the integer sums of half their rows through LDS, so each output
+// has one writer: plain stores when one block covers all pairs, atomics only
+// when several pair blocks add to it (np > 512).
 // ---------------------------------------------------------------------------
+template <int NH>
 __device__ __forceinline__ void anal_oz_body(const OzArgs &a, const double *__restrict__ asc,
-                                             const double2 *__restrict__ qc, int h, int gs, int fg,
-                                             unsigned char *Alds) {
-    const int im = a.segm[gs], s = gs - a.segbase[im], m = im * a.mres;
-    const int l0 = m + s * a.seg;
-    const int lstop = min(min(l0 + a.seg, a.lmax + 1), a.ltr + 1);
-    const int nst = lstop - l0;
+                                             const double2 *__restrict__ qc, const int *__restrict__ epl,
+                                             const int *__restrict__ pfirst, int pb, int run, int fg,
+                                             unsigned char *Alds, int *Xch) {
+    constexpr int NW = 4 * NH, ROWB = 256 * NH + 16, RPW = NH == 2 ? 2 : 4;
+    const cptr<int> ru = as_const((const int *)a.run) + 4 * run;
+    const int im = ru[0], gs0 = ru[1], nsr = ru[2], m = im * a.mres;
+    const int s0 = gs0 - as_const(a.segbase)[im];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int pbase = h * 256;
-    if (nst <= 0 || pbase + 255 < a.pfirst[gs]) return;
+    const int pbase = pb * 256 * NH;
     const int p = pbase + tid;
     const bool act = p < a.np;
     const double x = act ? a.x[p] : 0.0;
-    const size_t q = (size_t)gs * a.np + (act ? p : 0);
-    double prev = act ? a.seedq0[q] : 0.0, cur = act ? a.seedq1[q] : 0.0;
-    int kk = act ? a.seedk[q] : 0;
-    double live = kk == 0 ? 1.0 : 0.0;  // lanes still in the scaled range contribute zeros
-    // read-only tables passed as restrict kernel arguments: per-degree
-    // constants become wave-uniform scalar loads
-    const double2 *qcm = qc + a.coff[im] - m;
-    const double *ascm = asc + a.coff[im] - m;
-    const int *eplm = a.epl + a.coff[im] - m;
+    const long long co = as_const(a.coff)[im] - m;
+    const cptr<double> ascm = as_const(asc) + co;
+    const cptr<double> qbm = as_const(a.qb) + co;
+    const cptr<double> qccm = as_const(a.qcc) + co;
+    const cptr<int> eplm = as_const(epl) + co;
+    const cptr<int> pfc = as_const(pfirst);
 
-    // field operand of this wave: parity wp, column tile wct, 4 ksteps x 6 slices
-    const int wp = w >> 1, wct = w & 1;
+    // contraction role of this wave: K half kh, parity wp, column tile wct
+    const int kh = w >> 2, wp = (w >> 1) & 1, wct = w & 1;
     v4i Bw[4][OZ_SLICES];
     const size_t ubase = ((size_t)fg * a.nm + im) * a.nks;
+    const int kb = pb * 4 * NH;
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-        const int kg = 4 * h + ks;
+    for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
         for (int t = 0; t < OZ_SLICES; ++t)
-            Bw[ks][t] = kg < a.nks ? ((const v4i *)a.gdig)[(ubase + kg) * OZ_CHUNK_V4 +
-                                                           ((wp * OZ_SLICES + t) * 2 + wct) * 64 + lane]
-                                   : v4i{0, 0, 0, 0};
-    }
+            Bw[ks][t] = ((const v4i *)a.gdig)[(ubase + kb + 4 * kh + ks) * OZ_CHUNK_V4 +
+                                              ((wp * OZ_SLICES + t) * 2 + wct) * 64 + lane];
     const int c = (lane & 15) + 16 * wct;
     double csum = 0.0;
-    for (int ks = 0; ks < 4; ++ks) {
-        const int kg = 4 * h + ks;
-        if (kg < a.nks) csum += a.gcolsum[((ubase + kg) * 2 + wp) * OZ_COLS + c];
-    }
-    const int egc = a.eg[(((size_t)fg * a.nm + im) * 2 + wp) * OZ_COLS + c];
+    for (int k = 0; k < 4 * NH; ++k) csum += a.gcolsum[((ubase + kb + k) * 2 + wp) * OZ_COLS + c];
+    const int egc = a.eg[((((size_t)fg * a.nm + im) * a.npb + pb) * 2 + wp) * OZ_COLS + c];
     const int f = fg * OZ_FIELDS + (c >> 1);
-    const int wpos = 4 * (tid & 63) + (tid >> 6);  // LDS position of this lane's pair (see k_oz_gdig)
+    const bool fok = f < a.nf;
+    double *outc = a.spec_out + 2 * ((size_t)(fok ? f : 0) * a.spec_fstride + (size_t)(a.lmax + 1) * m) + (c & 1);
+    const bool atom = a.npb > 1;
+    const int wpos = NW * (tid & 63) + (tid >> 6);  // LDS position of this lane's pair (as k_oz_gprep)
 
-    for (int t0 = 0; t0 < nst; t0 += 32) {
-        // recurrence and digits of 32 degrees
-#pragma unroll
-        for (int blk = 0; blk < 4; ++blk) {
-#pragma unroll
-            for (int qq = 0; qq < 8; ++qq) {
-                const int i = blk * 8 + qq;
-                const int t = t0 + i;
-                // wave-uniform degree: scalar loads of its per-degree constants; rows
-                // beyond the segment are never stored, so they need no mask
-                const int l = __builtin_amdgcn_readfirstlane(l0 + min(t, nst - 1));
-                // U = 2^47 + round(P 2^(46-epl)) lands in the low 48 bits of the mantissa
-                const unsigned long long bits =
-                    (unsigned long long)__double_as_longlong(fma(cur * live, ascm[l], kMagic));
-                const unsigned uh = (unsigned)(bits >> 32) ^ 0x8080u, ul = (unsigned)bits ^ 0x80808080u;
-                unsigned char *row = Alds + ((i & 1) * OZ_SLICES * 16 + (i >> 1)) * OZ_ROWB + wpos;
-                row[0 * 16 * OZ_ROWB] = (unsigned char)(uh >> 8);
-                row[1 * 16 * OZ_ROWB] = (unsigned char)uh;
-                row[2 * 16 * OZ_ROWB] = (unsigned char)(ul >> 24);
-                row[3 * 16 * OZ_ROWB] = (unsigned char)(ul >> 16);
-                row[4 * 16 * OZ_ROWB] = (unsigned char)(ul >> 8);
-                row[5 * 16 * OZ_ROWB] = (unsigned char)ul;
-                const double beta = qcm[l + 1].x;
-                const double nx = fma(x, cur, -(beta * prev));
-                prev = cur;
-                cur = nx;
-            }
-            const int lchk = __builtin_amdgcn_readfirstlane(l0 + min(t0 + blk * 8 + 7, nst - 1));
-            if (kk > 0 && fmax(fabs(cur), fabs(prev)) * qcm[lchk].y > kHiS) {
-                cur *= kDownS;
-                prev *= kDownS;
-                --kk;
+    // Segments of the run in which every pair of this block is still in the
+    // scaled range contribute nothing (pfirst does not increase along an
+    // order); the recurrence starts from the seeds of the first live segment
+    // and continues through the rest of the run: at a segment boundary L the
+    // Q-form of the next segment is the current one times c_{L-1} (the last
+    // scale of the segment ending), so no further per-lane loads are needed.
+    int sr0 = 0;
+    while (sr0 < nsr && pbase + 256 * NH - 1 < pfc[gs0 + sr0]) ++sr0;
+    if (sr0 == nsr) return;
+    double prev, cur;
+    int kk;
+    {
+        const size_t q = (size_t)(gs0 + sr0) * a.np + (act ? p : 0);
+        prev = act ? a.seedq0[q] : 0.0;
+        cur = act ? a.seedq1[q] : 0.0;
+        kk = act ? a.seedk[q] : 0;
+    }
+    double live = kk == 0 ? 1.0 : 0.0;  // lanes still in the scaled range contribute zeros
+    const int wpu = __builtin_amdgcn_readfirstlane(wp);
+    const int r0u = __builtin_amdgcn_readfirstlane(NH == 2 ? 2 * kh : 0);  // first row index i of this wave
+    const int grp = lane >> 4;
+    for (int sr = sr0; sr < nsr; ++sr) {
+        const int l0 = m + (s0 + sr) * a.seg;
+        const int lstop = min(min(l0 + a.seg, a.lmax + 1), a.ltr + 1);
+        const int nst = lstop - l0;
+        if (nst <= 0) break;
+        if (sr > sr0) {
+            if ((a.seg & 31) == 0) {
+                const double cb = as_const(qc)[co + l0 - 1].y;
+                prev *= cb;
+                cur *= cb;
+            } else {
+                // segments that end inside a 32-degree chunk ran the recurrence
+                // past their end: start the next one from its seeds
+                const size_t q = (size_t)(gs0 + sr) * a.np + (act ? p : 0);
+                prev = act ? a.seedq0[q] : 0.0;
+                cur = act ? a.seedq1[q] : 0.0;
+                kk = act ? a.seedk[q] : 0;
                 live = kk == 0 ? 1.0 : 0.0;
             }
         }
-        __syncthreads();
-        v4i acc[OZ_SLICES];
+        for (int t0 = 0; t0 < nst; t0 += 32) {
+            const int lc = __builtin_amdgcn_readfirstlane(l0 + t0);
+            // per-degree exponents of the rows 4 g + r0 + r this wave emits; rows
+            // past the segment read valid (padded) table entries and are never stored
+            int ev[4][RPW];
 #pragma unroll
-        for (int v = 0; v < OZ_SLICES; ++v) acc[v] = v4i{0, 0, 0, 0};
+            for (int gg = 0; gg < 4; ++gg)
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            v4i Aop[OZ_SLICES];
+                for (int r = 0; r < RPW; ++r) ev[gg][r] = eplm[lc + wpu + 2 * (4 * gg + r0u + r)];
+            // Recurrence and digits of 32 degrees.  The per-degree constants of a
+            // block of 8 degrees arrive in one batch of scalar loads, issued
+            // during the previous block (after its own wait), so the steps of a
+            // block never wait for memory.
+            double aq[8], bq[8], cq;
 #pragma unroll
-            for (int sl = 0; sl < OZ_SLICES; ++sl)
-                Aop[sl] = *(const v4i *)(Alds + ((wp * OZ_SLICES + sl) * 16 + (lane & 15)) * OZ_ROWB + 64 * ks +
-                                         16 * (lane >> 4));
+            for (int j = 0; j < 8; ++j) {
+                aq[j] = ascm[lc + j];
+                bq[j] = qbm[lc + j + 1];
+            }
+            cq = qccm[lc + 7];
 #pragma unroll
-            for (int t = 0; t < OZ_SLICES; ++t)
+            for (int blk = 0; blk < 4; ++blk) {
+                double na[8], nb[8], nc = 0.0;
 #pragma unroll
-                for (int sl = 0; sl + t < OZ_SLICES; ++sl)
-                    acc[sl + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aop[sl], Bw[ks][t], acc[sl + t], 0, 0, 0);
-        }
-        // rows 4 (lane >> 4) + i are degree pairs, column c
+                for (int qq = 0; qq < 8; ++qq) {
+                    const int i = blk * 8 + qq;
+                    // U = 2^47 + round(P 2^(46-epl)) lands in the low 48 bits of the mantissa
+                    const unsigned long long bits =
+                        (unsigned long long)__double_as_longlong(fma(cur * live, aq[qq], kMagic));
+                    const unsigned uh = (unsigned)(bits >> 32) ^ 0x8080u, ul = (unsigned)bits ^ 0x80808080u;
+                    unsigned char *row = Alds + ((i & 1) * OZ_SLICES * 16 + (i >> 1)) * ROWB + wpos;
+                    row[0 * 16 * ROWB] = (unsigned char)(uh >> 8);
+                    row[1 * 16 * ROWB] = (unsigned char)uh;
+                    row[2 * 16 * ROWB] = (unsigned char)(ul >> 24);
+                    row[3 * 16 * ROWB] = (unsigned char)(ul >> 16);
+                    row[4 * 16 * ROWB] = (unsigned char)(ul >> 8);
+                    row[5 * 16 * ROWB] = (unsigned char)ul;
+                    const double nx = fma(x, cur, -(bq[qq] * prev));
+                    prev = cur;
+                    cur = nx;
+                    if (qq == 0 && blk < 3) {
+                        // keeps the next batch below this block's wait
+                        __asm__ volatile("" ::: "memory");
+                        const int ln = lc + 8 * (blk + 1);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 4 * (lane >> 4) + i;
-            const int t = t0 + 2 * row + wp;
-            if (t < nst && f < a.nf) {
-                const int l = l0 + t;
-                double E = kBias * csum;
+                        for (int j = 0; j < 8; ++j) {
+                            na[j] = ascm[ln + j];
+                            nb[j] = qbm[ln + j + 1];
+                        }
+                        nc = qccm[ln + 7];
+                    }
+                }
+                if (kk > 0 && fmax(fabs(cur), fabs(prev)) * cq > kHiS) {
+                    cur *= kDownS;
+                    prev *= kDownS;
+                    --kk;
+                    live = kk == 0 ? 1.0 : 0.0;
+                }
+                if (blk < 3) {
 #pragma unroll
-                for (int v = 0; v < OZ_SLICES; ++v) E = fma((double)acc[v][i], ldexp(1.0, 80 - 8 * v), E);
-                const double val = E * a.g * ldexp(1.0, eplm[l] - 46 + egc - 46);
-                atomicAdd(a.spec_out + 2 * ((size_t)f * a.spec_fstride + (size_t)(a.lmax + 1) * m + l) + (c & 1),
-                          val);
+                    for (int j = 0; j < 8; ++j) {
+                        aq[j] = na[j];
+                        bq[j] = nb[j];
+                    }
+                    cq = nc;
+                }
+            }
+            __syncthreads();
+            v4i acc[OZ_SLICES];
+#pragma unroll
+            for (int v = 0; v < OZ_SLICES; ++v) acc[v] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                v4i Aop[OZ_SLICES];
+#pragma unroll
+                for (int sl = 0; sl < OZ_SLICES; ++sl)
+                    Aop[sl] = *(const v4i *)(Alds + ((wp * OZ_SLICES + sl) * 16 + (lane & 15)) * ROWB +
+                                             64 * (4 * kh + ks) + 16 * (lane >> 4));
+#pragma unroll
+                for (int t = 0; t < OZ_SLICES; ++t)
+#pragma unroll
+                    for (int sl = 0; sl + t < OZ_SLICES; ++sl)
+                        acc[sl + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aop[sl], Bw[ks][t], acc[sl + t], 0, 0, 0);
+            }
+            if constexpr (NH == 2) {
+                // rows i = 2 kh, 2 kh + 1 stay with this wave; its sums of the other
+                // two go to the partner of the other K half (wave w ^ 4)
+                int *xo = Xch + w * (2 * OZ_SLICES * 64) + lane;
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int v = 0; v < OZ_SLICES; ++v) xo[(r * OZ_SLICES + v) * 64] = acc[v][2 * (1 - kh) + r];
+            }
+            __syncthreads();
+            // rows 4 (lane >> 4) + i are degree pairs, column c
+            const int *xi = Xch + (w ^ 4) * (2 * OZ_SLICES * 64) + lane;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int i = r0u + r;
+                const int row = 4 * grp + i;
+                const int t = t0 + 2 * row + wp;
+                const int e = grp == 0 ? ev[0][r] : grp == 1 ? ev[1][r] : grp == 2 ? ev[2][r] : ev[3][r];
+                if (t < nst && fok) {
+                    double E = kBias * csum;
+#pragma unroll
+                    for (int v = 0; v < OZ_SLICES; ++v) {
+                        int s = acc[v][NH == 2 ? 2 * kh + r : r];
+                        if constexpr (NH == 2) s += xi[(r * OZ_SLICES + v) * 64];
+                        E = fma((double)s, ldexp(1.0, 80 - 8 * v), E);
+                    }
+                    const double val = E * a.g * ldexp(1.0, e - 46 + egc - 46);
+                    double *dst = outc + 2 * (l0 + t);
+                    if (atom)
+                        atomicAdd(dst, val);
+                    else
+                        *dst = val;
+                }
             }
         }
-        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(256) void k_anal_oz(const OzArgs a, const double *__restrict__ asc,
-                                                  const double2 *__restrict__ qc) {
-    // rows padded to 272 bytes: the 16 lanes of an A-operand read hit distinct banks
-    __shared__ __align__(16) unsigned char Alds[2 * OZ_SLICES * 16 * OZ_ROWB];
+template <int NH>
+__global__ __launch_bounds__(256 * NH) void k_anal_oz(const OzArgs a, const double *__restrict__ asc,
+                                                       const double2 *__restrict__ qc, const int *__restrict__ epl,
+                                                       const int *__restrict__ pfirst) {
+    // rows padded by 16 bytes: the 16 lanes of an A-operand read hit distinct banks
+    __shared__ __align__(16) unsigned char Alds[2 * OZ_SLICES * 16 * (256 * NH + 16)];
+    __shared__ int Xch[NH == 2 ? 8 * 2 * OZ_SLICES * 64 : 1];
     const int item = blockIdx.x;
-    anal_oz_body(a, asc, qc, item % a.nhalf, (item / a.nhalf) % a.nseg_total, item / (a.nhalf * a.nseg_total), Alds);
+    anal_oz_body<NH>(a, asc, qc, epl, pfirst, item % a.npb, (item / a.npb) % a.nrun, item / (a.npb * a.nrun), Alds,
+                     Xch);
 }
 
 // ---------------------------------------------------------------------------
@@ -599,36 +746,28 @@ __device__ __forceinline__ int oz_group_of(const OzGroups &G, int x) {
 
 template <typename IT>
 __global__ __launch_bounds__(256) void k_oz_bexp_grp(const OzGroups G) {
-    __shared__ double cmax[256];
     const int gi = oz_group_of(G, blockIdx.x);
     const OzArgs &a = G.args[gi];
     const int b = blockIdx.x - G.prefix[gi];
-    oz_bexp_body<IT>(a, b % a.nm, b / a.nm, cmax);
+    oz_bexp_body<IT>(a, b % a.nm, b / a.nm);
 }
 
 template <typename IT>
 __global__ __launch_bounds__(256) void k_oz_bdig_grp(const OzGroups G) {
+    __shared__ double2 stg[OZ_FIELDS * OZ_BSTG];
     __shared__ unsigned long long csum[2][OZ_COLS];
     const int gi = oz_group_of(G, blockIdx.x);
     const OzArgs &a = G.args[gi];
     const int b = blockIdx.x - G.prefix[gi];
-    oz_bdig_body<IT>(a, b % a.ch_total, b / a.ch_total, csum);
+    oz_bdig_body<IT>(a, b % a.ch_total, b / a.ch_total, stg, csum);
 }
 
-__global__ __launch_bounds__(256) void k_oz_gexp_grp(const OzGroups G) {
-    __shared__ double cmax[2][256];
+template <int NH>
+__global__ __launch_bounds__(256 * NH) void k_oz_gprep_grp(const OzGroups G) {
     const int gi = oz_group_of(G, blockIdx.x);
     const OzArgs &a = G.args[gi];
     const int b = blockIdx.x - G.prefix[gi];
-    oz_gexp_body(a, b % a.nm, b / a.nm, cmax);
-}
-
-__global__ __launch_bounds__(256) void k_oz_gdig_grp(const OzGroups G) {
-    __shared__ unsigned long long csum[2][OZ_COLS];
-    const int gi = oz_group_of(G, blockIdx.x);
-    const OzArgs &a = G.args[gi];
-    const int b = blockIdx.x - G.prefix[gi];
-    oz_gdig_body(a, b % a.nks, (b / a.nks) % a.nm, b / (a.nks * a.nm), csum);
+    oz_gprep_body<NH>(a, b % a.npb, (b / a.npb) % a.nm, b / (a.npb * a.nm));
 }
 
 __global__ void k_oz_reset(int *counter) {
@@ -656,9 +795,12 @@ __global__ __launch_bounds__(256) void k_synth_oz_grp(const OzGroups G) {
     }
 }
 
-// Persistent analysis: unit u of group gi is (pair half, segment, field group).
-__global__ __launch_bounds__(256) void k_anal_oz_grp(const OzGroups G) {
-    __shared__ __align__(16) unsigned char Alds[2 * OZ_SLICES * 16 * OZ_ROWB];
+// Persistent analysis: unit u of group gi is (pair block, run, field group);
+// every group of one launch has the same pair-block size NH.
+template <int NH>
+__global__ __launch_bounds__(256 * NH) void k_anal_oz_grp(const OzGroups G) {
+    __shared__ __align__(16) unsigned char Alds[2 * OZ_SLICES * 16 * (256 * NH + 16)];
+    __shared__ int Xch[NH == 2 ? 8 * 2 * OZ_SLICES * 64 : 1];
     __shared__ int s_unit;
     for (;;) {
         __syncthreads();
@@ -669,8 +811,8 @@ __global__ __launch_bounds__(256) void k_anal_oz_grp(const OzGroups G) {
         const int gi = oz_group_of(G, u);
         const OzArgs &a = G.args[gi];
         const int lu = u - G.prefix[gi];
-        anal_oz_body(a, a.asc, a.qc, lu % a.nhalf, (lu / a.nhalf) % a.nseg_total, lu / (a.nhalf * a.nseg_total),
-                     Alds);
+        anal_oz_body<NH>(a, a.asc, a.qc, a.epl, a.pfirst, lu % a.npb, (lu / a.npb) % a.nrun, lu / (a.npb * a.nrun),
+                         Alds, Xch);
     }
 }
 
@@ -693,27 +835,39 @@ hipError_t launch_oz_synth_grp(const OzGroups &bexp, const OzGroups &bdig, const
     return hipGetLastError();
 }
 
-hipError_t launch_oz_anal_grp(const OzGroups &gexp, const OzGroups &gdig, const OzGroups &units,
-                              int persistent_blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_oz_gexp_grp, dim3((unsigned)gexp.prefix[gexp.ngroups]), dim3(256), 0, st, gexp);
-    hipLaunchKernelGGL(k_oz_gdig_grp, dim3((unsigned)gdig.prefix[gdig.ngroups]), dim3(256), 0, st, gdig);
-    hipLaunchKernelGGL(k_oz_reset, dim3(1), dim3(64), 0, st, units.counter);
-    hipLaunchKernelGGL(k_anal_oz_grp, dim3(persistent_blocks), dim3(256), 0, st, units);
+hipError_t launch_oz_anal_grp(const OzGroups *gprep, const OzGroups *units, int persistent_blocks, hipStream_t st) {
+    // units[0]: groups with two-half pair blocks, units[1]: with one half
+    for (int k = 0; k < 2; ++k) {
+        if (units[k].ngroups == 0) continue;
+        const dim3 g1((unsigned)gprep[k].prefix[gprep[k].ngroups]);
+        if (k == 0)
+            hipLaunchKernelGGL(k_oz_gprep_grp<2>, g1, dim3(512), 0, st, gprep[k]);
+        else
+            hipLaunchKernelGGL(k_oz_gprep_grp<1>, g1, dim3(256), 0, st, gprep[k]);
+        hipLaunchKernelGGL(k_oz_reset, dim3(1), dim3(64), 0, st, units[k].counter);
+        if (k == 0)
+            hipLaunchKernelGGL(k_anal_oz_grp<2>, dim3(persistent_blocks), dim3(512), 0, st, units[k]);
+        else
+            hipLaunchKernelGGL(k_anal_oz_grp<1>, dim3(persistent_blocks), dim3(256), 0, st, units[k]);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_oz_asc(const OzAscArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL(k_oz_asc, dim3((a.n + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_oz_asc, dim3((a.n + OZ_TPAD + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_oz_anal(const OzArgs &a, hipStream_t st) {
-    if (a.nks > OZ_MAX_KS) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_oz_gexp, dim3((unsigned)a.nm * a.nfg), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_oz_gdig, dim3((unsigned)a.nks * a.nm * a.nfg), dim3(256), 0, st, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_anal_oz, dim3((unsigned)a.nhalf * a.nseg_total * a.nfg), dim3(256), 0, st, a, a.asc, a.qc);
+    if (a.nks > OZ_MAX_KS || a.nks != 4 * a.nh * a.npb) return hipErrorInvalidValue;
+    const dim3 g1((unsigned)a.npb * a.nm * a.nfg), g2((unsigned)a.npb * a.nrun * a.nfg);
+    if (a.nh == 2) {
+        hipLaunchKernelGGL(k_oz_gprep<2>, g1, dim3(512), 0, st, a);
+        hipLaunchKernelGGL(k_anal_oz<2>, g2, dim3(512), 0, st, a, a.asc, a.qc, a.epl, a.pfirst);
+    } else {
+        hipLaunchKernelGGL(k_oz_gprep<1>, g1, dim3(256), 0, st, a);
+        hipLaunchKernelGGL(k_anal_oz<1>, g2, dim3(256), 0, st, a, a.asc, a.qc, a.epl, a.pfirst);
+    }
     return hipGetLastError();
 }
 

@@ -13,6 +13,8 @@
 // radix 8/4/2/3/5 passes; twiddles come from a per-plan table.  Consecutive
 // latitude pairs are mapped to the same XCD so the L2 merges the strided
 // longitude-major stores of neighbouring rows.
+#include <algorithm>
+
 #include "shtk_internal.h"
 
 namespace shtk {
@@ -91,36 +93,57 @@ __device__ __forceinline__ void dft_small(double2 *v) {
 
 // One Stockham pass of radix R over the LDS row z (length N).  Every thread
 // stages all its inputs in registers before the barrier, then writes.
+// values per thread in one Stockham pass: the row length over the threads of a row
+constexpr int FFT_VPT = 8;
+
+// LDS position of row element i: one padding slot per 16 elements, which
+// spreads the strided 16-byte accesses of the Stockham passes (stride R or
+// R Ns elements) over all LDS banks.
+__device__ __forceinline__ int zp(int i) { return i + (i >> 4); }
+
 template <int R, int SIGN>
 __device__ __forceinline__ void stockham_pass(double2 *z, int N, int Ns, const double2 *tw, int tid, int nt) {
-    constexpr int MAXB = 8 / R + 1;  // at most N/8 butterflies' worth of values per thread
-    double2 v[MAXB][R];
+    // nt >= N / FFT_VPT (launch_fft), so a thread has at most ceil(FFT_VPT / R) butterflies
+    constexpr int MAXB = (FFT_VPT + R - 1) / R;
     const int nb = N / R;
+    const int stride = N / (Ns * R);
+    // base twiddle w = W^(k stride) of each butterfly (the others are its
+    // powers): independent of the LDS data, so its load overlaps the barrier
+    double2 w1[MAXB];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int j = tid + q * nt;
+        w1[q] = (Ns > 1 && j < nb) ? tw[(j % Ns) * stride] : make_double2(1.0, 0.0);
+    }
+    double2 v[MAXB][R];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
         const int j = tid + q * nt;
         if (j < nb) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = z[j + r * nb];
+            for (int r = 0; r < R; ++r) v[q][r] = z[zp(j + r * nb)];
         }
     }
     __syncthreads();
-    const int stride = N / (Ns * R);
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
         const int j = tid + q * nt;
         if (j < nb) {
             const int k = j % Ns;
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                double2 w = tw[r * k * stride];
+            if (Ns > 1) {
+                double2 w = w1[q];
                 if (SIGN > 0) w.y = -w.y;
-                v[q][r] = cmul(v[q][r], w);
+                double2 wr = w;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    v[q][r] = cmul(v[q][r], wr);
+                    if (r + 1 < R) wr = cmul(wr, w);
+                }
             }
             dft_small<R, SIGN>(v[q]);
             const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) z[base + r * Ns] = v[q][r];
+            for (int r = 0; r < R; ++r) z[zp(base + r * Ns)] = v[q][r];
         }
     }
     __syncthreads();
@@ -143,126 +166,213 @@ __device__ void fft_lds(double2 *z, const FftArgs &a, int tid, int nt) {
     }
 }
 
-// Block index -> latitude pair, keeping runs of consecutive pairs on one XCD
-// (workgroups are dispatched round-robin over the 8 XCDs).
-__device__ __forceinline__ int pair_of_block(int b, int np) {
-    const int per = (np + 7) / 8;
-    const int xcd = b & 7, slot = b >> 3;
-    const int p = xcd * per + slot;
-    return p;
+// ---------------------------------------------------------------------------
+// Tiles of G adjacent latitude pairs (G = 4 for nlon <= 2560, less for longer
+// rows: the G complex rows must fit in LDS).  The grid and the Fourier
+// workspace both keep the latitude index fastest, so a tile's G north and G
+// south latitudes are two contiguous runs at every longitude / order: lanes
+// are grouped by 2G over one longitude / order (lane k < G: north latitude
+// p0 + k; lane k >= G: south latitude of pair p0 + 2G-1-k), which turns the
+// per-element accesses of a one-pair block into 8G-byte runs.  The partner of
+// lane k (the other half of its pair) is lane k ^ (2G-1).
+//
+// Work distribution: the (field, tile) units are dealt to the 8 XCDs in
+// contiguous ranges and every workgroup of an XCD sweeps that range with a
+// stride of the XCD's workgroup count, so neighbouring tiles (which share
+// cache lines) run on one XCD at the same time.
+// ---------------------------------------------------------------------------
+struct FftTile {
+    int total, per, nslot, ntile;
+};
+
+__device__ __forceinline__ FftTile fft_tiles(const FftArgs &a, int G, int nfields) {
+    FftTile t;
+    t.ntile = (a.np + G - 1) / G;
+    t.total = t.ntile * nfields;
+    t.per = (t.total + 7) / 8;
+    t.nslot = gridDim.x >> 3;
+    return t;
 }
 
-template <typename GT>
-__global__ __launch_bounds__(1024) void k_fft_synth(const FftArgs a) {
-    extern __shared__ double2 z[];
-    const int p = pair_of_block(blockIdx.x, a.np);
-    if (p >= a.np) return;
-    const int f = blockIdx.y;
-    const int N = a.nlon, nlat = a.nlat;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int iN = p, iS = nlat - 1 - p;
-    const bool hasS = iS != iN;
-    for (int q = tid; q < N; q += nt) z[q] = make_double2(0.0, 0.0);
-    __syncthreads();
-    const double2 *Ff = a.F + (size_t)f * a.nm * nlat;
-    for (int im = tid; im < a.nm; im += nt) {
-        const int m = im * a.mres;
-        const double2 FN = Ff[(size_t)im * nlat + iN];
-        const double2 FS = hasS ? Ff[(size_t)im * nlat + iS] : make_double2(0.0, 0.0);
-        if (m == 0) {
-            z[0] = make_double2(FN.x, FS.x);
-        } else {
-            z[m] = make_double2(FN.x - FS.y, FN.y + FS.x);
-            z[N - m] = make_double2(FN.x + FS.y, FS.x - FN.y);
+__device__ __forceinline__ double fft_row_scale(const FftArgs &a, int p) {
+    if (a.mode == 1) {
+        const double st = a.s[p];
+        return st > 0.0 ? 1.0 / st : 1.0;
+    }
+    return a.mode == 2 ? a.s[p] : 1.0;
+}
+
+// Synthesis: Fourier data -> grid rows.  Row g of the tile holds the packed
+// pair z = f_N + i f_S of pair p0 + g at positions zp(j) (row stride RS = N + N/16 + 1
+// complex: the rows of one longitude fall in different LDS banks).
+template <int G, typename GT>
+__global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfields) {
+    extern __shared__ double2 zs[];
+    constexpr int L = 2 * G;
+    const int N = a.nlon, nlat = a.nlat, RS = N + (N >> 4) + 1;
+    const int tid = threadIdx.x, nt = blockDim.x, tr = nt / G;
+    const int row = tid / tr, lt = tid % tr;
+    const int k = tid % L;                // lane role within its group of 2G
+    const int gk = k < G ? k : L - 1 - k;  // pair of this lane within the tile
+    const FftTile T = fft_tiles(a, G, nfields);
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    for (int u = slot; u < T.per; u += T.nslot) {
+        const int ug = xcd * T.per + u;
+        if (ug >= T.total) break;
+        const int f = ug / T.ntile, p0 = (ug % T.ntile) * G;
+        const int p = p0 + gk;
+        const int lat = k < G ? p0 + k : nlat - L - p0 + k;
+        const bool live = p < a.np && (k < G || nlat - 1 - p != p);
+        for (int q = tid; q < G * RS; q += nt) zs[q] = make_double2(0.0, 0.0);
+        __syncthreads();
+        // packed rows: lane k < G writes z[m], its partner z[N - m]
+        const double2 *Ff = a.F + (size_t)f * a.nm * nlat + lat;
+        double2 *z = zs + gk * RS;
+        constexpr int PF = 8;  // loads in flight per thread
+        for (int e0 = 0; e0 < a.nm * L; e0 += PF * nt) {
+            double2 ld[PF];
+#pragma unroll
+            for (int q = 0; q < PF; ++q) {
+                const int im = (e0 + tid + q * nt) / L;
+                ld[q] = live && im < a.nm ? Ff[(size_t)im * nlat] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int q = 0; q < PF; ++q) {
+                const int im = (e0 + tid + q * nt) / L;
+                const double2 mine = ld[q];
+                const double2 other = make_double2(__shfl_xor(mine.x, L - 1), __shfl_xor(mine.y, L - 1));
+                const double2 FN = k < G ? mine : other, FS = k < G ? other : mine;
+                const int m = im * a.mres;
+                if (im < a.nm) {
+                    if (m == 0) {
+                        if (k < G) z[0] = make_double2(FN.x, FS.x);
+                    } else if (k < G) {
+                        z[zp(m)] = make_double2(FN.x - FS.y, FN.y + FS.x);
+                    } else {
+                        z[zp(N - m)] = make_double2(FN.x + FS.y, FS.x - FN.y);
+                    }
+                }
+            }
         }
-    }
-    __syncthreads();
-    fft_lds<+1>(z, a, tid, nt);
-    GT *g = (GT *)a.grid + (size_t)f * nlat * N;
-    double sc = 1.0;
-    if (a.mode == 1) {
-        const double st = a.s[p];
-        sc = st > 0.0 ? 1.0 / st : 1.0;
-    } else if (a.mode == 2) {
-        sc = a.s[p];
-    }
-    for (int j = tid; j < N; j += nt) {
-        const double2 v = z[j];
-        g[(size_t)j * nlat + iN] = (GT)(v.x * sc);
-        if (hasS) g[(size_t)j * nlat + iS] = (GT)(v.y * sc);
+        __syncthreads();
+        fft_lds<+1>(zs + row * RS, a, lt, tr);
+        GT *gf = (GT *)a.grid + (size_t)f * nlat * N + lat;
+        if (live) {
+            const double sc = fft_row_scale(a, p);
+            for (int e = tid; e < N * L; e += nt) {
+                const int j = e / L;
+                const double2 v = z[zp(j)];
+                gf[(size_t)j * nlat] = (GT)((k < G ? v.x : v.y) * sc);
+            }
+        }
+        __syncthreads();
     }
 }
 
-template <typename GT>
-__global__ __launch_bounds__(1024) void k_fft_anal(const FftArgs a) {
-    extern __shared__ double2 z[];
-    const int p = pair_of_block(blockIdx.x, a.np);
-    if (p >= a.np) return;
-    const int f = blockIdx.y;
-    const int N = a.nlon, nlat = a.nlat;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int iN = p, iS = nlat - 1 - p;
-    const bool hasS = iS != iN;
-    const GT *g = (const GT *)a.grid + (size_t)f * nlat * N;
-    double sc = 1.0;
-    if (a.mode == 1) {
-        const double st = a.s[p];
-        sc = st > 0.0 ? 1.0 / st : 1.0;
-    } else if (a.mode == 2) {
-        sc = a.s[p];
-    }
-    for (int j = tid; j < N; j += nt) {
-        const double vN = (double)g[(size_t)j * nlat + iN];
-        const double vS = hasS ? (double)g[(size_t)j * nlat + iS] : 0.0;
-        z[j] = make_double2(vN * sc, vS * sc);
-    }
-    __syncthreads();
-    fft_lds<-1>(z, a, tid, nt);
-    double2 *Ff = a.F + (size_t)f * a.nm * nlat;
-    for (int im = tid; im < a.nm; im += nt) {
-        const int m = im * a.mres;
-        const double2 Zm = z[m];
-        const double2 Zc = z[(N - m) % N];  // conj taken below
-        // FN = (Zm + conj(Zc)) / 2, FS = (Zm - conj(Zc)) / (2i)
-        const double2 FN = make_double2(0.5 * (Zm.x + Zc.x), 0.5 * (Zm.y - Zc.y));
-        const double2 FS = make_double2(0.5 * (Zm.y + Zc.y), -0.5 * (Zm.x - Zc.x));
-        Ff[(size_t)im * nlat + iN] = FN;
-        if (hasS) Ff[(size_t)im * nlat + iS] = FS;
+// Analysis: grid rows -> Fourier data, FN = (Z_m + conj Z_{N-m}) / 2,
+// FS = (Z_m - conj Z_{N-m}) / (2i) of the packed row Z.
+template <int G, typename GT>
+__global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfields) {
+    extern __shared__ double2 zs[];
+    constexpr int L = 2 * G;
+    const int N = a.nlon, nlat = a.nlat, RS = N + (N >> 4) + 1;
+    const int tid = threadIdx.x, nt = blockDim.x, tr = nt / G;
+    const int row = tid / tr, lt = tid % tr;
+    const int k = tid % L;
+    const int gk = k < G ? k : L - 1 - k;
+    const FftTile T = fft_tiles(a, G, nfields);
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    for (int u = slot; u < T.per; u += T.nslot) {
+        const int ug = xcd * T.per + u;
+        if (ug >= T.total) break;
+        const int f = ug / T.ntile, p0 = (ug % T.ntile) * G;
+        const int p = p0 + gk;
+        const int lat = k < G ? p0 + k : nlat - L - p0 + k;
+        const bool live = p < a.np && (k < G || nlat - 1 - p != p);
+        const double sc = p < a.np ? fft_row_scale(a, p) : 0.0;
+        const GT *gf = (const GT *)a.grid + (size_t)f * nlat * N + lat;
+        double2 *z = zs + gk * RS;
+        constexpr int PF = 8;  // loads in flight per thread
+        for (int e0 = 0; e0 < N * L; e0 += PF * nt) {
+            double ld[PF];
+#pragma unroll
+            for (int q = 0; q < PF; ++q) {
+                const int j = (e0 + tid + q * nt) / L;
+                ld[q] = live && j < N ? (double)gf[(size_t)j * nlat] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < PF; ++q) {
+                const int j = (e0 + tid + q * nt) / L;
+                const double mine = ld[q] * sc;
+                const double other = __shfl_xor(mine, L - 1);
+                if (k < G && j < N) z[zp(j)] = make_double2(mine, other);
+            }
+        }
+        __syncthreads();
+        fft_lds<-1>(zs + row * RS, a, lt, tr);
+        if (live) {
+            double2 *Ff = a.F + (size_t)f * a.nm * nlat + lat;
+            for (int e = tid; e < a.nm * L; e += nt) {
+                const int im = e / L;
+                const int m = im * a.mres;
+                const double2 Zm = z[zp(m)];
+                const double2 Zc = z[zp((N - m) % N)];
+                Ff[(size_t)im * nlat] = k < G ? make_double2(0.5 * (Zm.x + Zc.x), 0.5 * (Zm.y - Zc.y))
+                                              : make_double2(0.5 * (Zm.y + Zc.y), -0.5 * (Zm.x - Zc.x));
+            }
+        }
+        __syncthreads();
     }
 }
 
 }  // namespace
 
-hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
-    if (a.nrad <= 0) return hipErrorNotSupported;
-    const int N = a.nlon;
-    int nt = N / 8;
-    if (nt < 64) nt = 64;
-    if (nt > 1024) nt = 1024;
-    nt = (nt + 63) / 64 * 64;
-    if (N > 8 * nt) return hipErrorNotSupported;
-    const size_t lds = sizeof(double2) * (size_t)N;
-    if (lds > 160 * 1024) return hipErrorNotSupported;
-    const int per = (a.np + 7) / 8;
-    dim3 grid(per * 8, nfields);
+namespace {
+template <int G>
+hipError_t launch_fft_g(const FftArgs &a, int nfields, int inverse, int tr, hipStream_t st) {
+    const int nt = G * tr;
+    const size_t lds = sizeof(double2) * (size_t)G * (a.nlon + (a.nlon >> 4) + 1);
+    const void *fn = inverse ? (a.io_f32 ? (const void *)k_fft_synth_t<G, float> : (const void *)k_fft_synth_t<G, double>)
+                             : (a.io_f32 ? (const void *)k_fft_anal_t<G, float> : (const void *)k_fft_anal_t<G, double>);
     if (lds > 64 * 1024) {
-        const void *fn = inverse ? (a.io_f32 ? (const void *)k_fft_synth<float> : (const void *)k_fft_synth<double>)
-                                 : (a.io_f32 ? (const void *)k_fft_anal<float> : (const void *)k_fft_anal<double>);
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
+    const int ntile = (a.np + G - 1) / G;
+    const int total = ntile * nfields;
+    // persistent grid: as many workgroups as fit the chip (LDS-limited), a multiple of 8
+    const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
+    int grid = std::min(256 * per_cu, (total + 7) / 8 * 8);
+    grid = std::max(8, grid / 8 * 8);
     if (inverse) {
         if (a.io_f32)
-            hipLaunchKernelGGL(k_fft_synth<float>, grid, dim3(nt), lds, st, a);
+            hipLaunchKernelGGL((k_fft_synth_t<G, float>), dim3(grid), dim3(nt), lds, st, a, nfields);
         else
-            hipLaunchKernelGGL(k_fft_synth<double>, grid, dim3(nt), lds, st, a);
+            hipLaunchKernelGGL((k_fft_synth_t<G, double>), dim3(grid), dim3(nt), lds, st, a, nfields);
     } else {
         if (a.io_f32)
-            hipLaunchKernelGGL(k_fft_anal<float>, grid, dim3(nt), lds, st, a);
+            hipLaunchKernelGGL((k_fft_anal_t<G, float>), dim3(grid), dim3(nt), lds, st, a, nfields);
         else
-            hipLaunchKernelGGL(k_fft_anal<double>, grid, dim3(nt), lds, st, a);
+            hipLaunchKernelGGL((k_fft_anal_t<G, double>), dim3(grid), dim3(nt), lds, st, a, nfields);
     }
     return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
+    if (a.nrad <= 0) return hipErrorNotSupported;
+    const int N = a.nlon;
+    // threads per row: N / FFT_VPT (every Stockham pass has at most FFT_VPT values per thread)
+    const int tr = (std::max(64, (N + FFT_VPT - 1) / FFT_VPT) + 63) / 64 * 64;
+    const size_t row = sizeof(double2) * (size_t)(N + (N >> 4) + 1);
+    // widest tile that fits LDS and still gives every CU work (a.tile_pairs > 0 forces it)
+    const long long units4 = (long long)((a.np + 3) / 4) * nfields, units2 = (long long)((a.np + 1) / 2) * nfields;
+    int G = a.tile_pairs;
+    if (G <= 0) G = units4 >= 512 ? 4 : units2 >= 256 ? 2 : 1;
+    if (G >= 4 && 4 * row <= 160 * 1024 && 4 * tr <= 1024) return launch_fft_g<4>(a, nfields, inverse, tr, st);
+    if (G >= 2 && 2 * row <= 160 * 1024 && 2 * tr <= 1024) return launch_fft_g<2>(a, nfields, inverse, tr, st);
+    if (row <= 160 * 1024 && tr <= 1024) return launch_fft_g<1>(a, nfields, inverse, tr, st);
+    return hipErrorNotSupported;
 }
 
 }  // namespace shtk

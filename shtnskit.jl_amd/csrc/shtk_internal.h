@@ -116,6 +116,7 @@ struct FftArgs {
     int io_f32;
     int mode;          // grid row factor: 0 none, 1 1/sin(theta), 2 sin(theta) (Robert form)
     const double *s;   // [np] sin(theta) for Robert scaling
+    int tile_pairs;    // latitude pairs per workgroup (0: chosen from the sizes)
 };
 
 }  // namespace shtk
@@ -123,7 +124,7 @@ struct FftArgs {
 struct shtk_plan {
     shtk_desc d{};
     int np = 0, nm = 0, seg = 128, nseg_total = 0;
-    int xcd_map = 0, synth_chains = 4;  // launch-shape knobs
+    int xcd_map = 0, synth_chains = 4, fft_tile = 0;  // launch-shape knobs
     bool canonical = true;
     double inv_scale = 0, scale_phi = 0;
     std::vector<double> hx, hw;  // north-first nodes and weights (nlat)

@@ -246,6 +246,7 @@ shtk::FftArgs fft_args(const shtk_plan *p) {
     for (size_t q = 0; q < p->radix.size(); ++q) f.radix |= (unsigned long long)p->radix[q] << (4 * q);
     f.tw = p->tw;
     f.s = p->s;
+    f.tile_pairs = p->fft_tile;
     return f;
 }
 
@@ -1144,6 +1145,7 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     p->seg = segment_length();
     p->xcd_map = env_int("SHTK_XCD", 0) != 0;
     p->synth_chains = env_int("SHTK_SYNTH_CHAINS", 4);
+    p->fft_tile = env_int("SHTK_FFT_PAIRS", 0);
     p->oz_run_len = std::max(1, env_int("SHTK_OZ_RUN", 8));
     p->canonical = d.norm == SHTK_NORM_ORTHONORMAL && !d.real_norm && d.cs_phase;
     p->inv_scale = d.phi_scale == SHTK_PHI_QUAD ? d.nlon / (2.0 * M_PI) : (double)d.nlon;

@@ -37,12 +37,15 @@ constexpr int SEG_MAX = 256;
 //   qc[l] = (beta_l, c_l)                  scalar and vector recurrences
 //   qd[l] = (l c_l, g_l c_{l-1})           vector: s dP_l/dtheta = l x P_l - g_l P_{l-1},
 //                                          g_l = sqrt((2l+1)(l^2-m^2)/(2l-1))
+constexpr int OZ_TPAD = 64;  // entries past the end of the padded per-degree tables
+
 struct LegArgs {
     int lmax, mmax, mres, nlat, np, nm, ltr, seg;
     const double *x;    // [np] cos(theta) of the north node of each pair
     const double *wlat; // [np] per-latitude factor, or nullptr for 1
     const double2 *qc;
     const double2 *qd;
+    const double *qb, *qcc; // beta_l and c_l of qc as separate arrays, padded by OZ_TPAD entries
     const double2 *ab;    // P-form (a_l, b_l)
     const double *gtab;   // g_l = sqrt((2l+1)(l^2-m^2)/(2l-1))
     const long long *coff;
@@ -137,6 +140,7 @@ struct shtk_plan {
     double *winvs2 = nullptr;    // [np] w * scale_phi / sin(theta)^2   (Robert-form vector analysis)
     double *invs = nullptr;      // [np] 1 / sin(theta)
     double2 *qc = nullptr, *qd = nullptr, *ab = nullptr;
+    double *qb = nullptr, *qcc = nullptr;  // beta_l, c_l of qc, padded by OZ_TPAD zeros
     double *gtab = nullptr;
     long long *coff = nullptr;
     int *segbase = nullptr;
@@ -166,7 +170,7 @@ struct shtk_plan {
     double *oz_q0 = nullptr, *oz_q1 = nullptr;  // [nm][4][np]
     int *oz_qk = nullptr, *oz_ep = nullptr, *oz_chbase = nullptr, *oz_chm = nullptr;
     int *oz_emaxl = nullptr, *oz_epl = nullptr;  // per (order, degree), qc indexing
-    double *oz_asc = nullptr, *oz_qb = nullptr, *oz_qcc = nullptr;
+    double *oz_asc = nullptr;
     int4 *oz_run = nullptr;  // analysis work runs: (order, first segment, segments, 0)
     int oz_nrun = 0, oz_run_len = 8;
     size_t qc_len = 0;
@@ -239,14 +243,12 @@ struct OzSeedArgs {
     int *emaxl;             // per order and degree: max over pairs of ilogb|P_l| (atomicMax)
 };
 
-constexpr int OZ_TPAD = 64;  // entries past the end of the per-degree split tables
 struct OzAscArgs {
     int n;                  // table length (all orders)
     const double2 *qc;
     const int *emaxl;
     double *asc;
     int *epl;
-    double *qb, *qcc;       // beta_l and c_l of qc as separate arrays
 };
 
 // Several configurations in one launch per stage, passed by value as kernel

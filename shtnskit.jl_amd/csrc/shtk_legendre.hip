@@ -470,22 +470,52 @@ __device__ void zero_fill(const LegArgs &a, const Item &it, OT *col, bool all_de
     }
 }
 
-// Scalar analysis of one item: two waves, K <= 4 chains per lane (pairs
-// pstart + tid + 128 j).  The reduced-sum scale c_l of the segment sits in LDS;
-// beta_{l+1} streams through scalar loads one chunk ahead.
-constexpr int ANS_THREADS = 128;
+// Scalar analysis: one wavefront per item (segment, pair block of 512,
+// field), K <= 8 chains per lane (pairs pstart + lane + 64 j), Q-form
+// recurrence.  Each output coefficient has one writer, so there is no
+// inter-wave reduction: every 8 degrees the lanes' 16 partial sums (8 degrees
+// x re/im) go through a wave-private LDS tile, lane (o = lane & 15, q =
+// lane >> 4) adds the partials of output o from lanes 16q .. 16q+15, two
+// shuffle stages finish, and 16 lanes store 8 consecutive coefficients.
+// beta_{l+1} and c_l of each 8-degree block arrive in one batch of scalar
+// loads from the padded tables.
+constexpr int ANW_WAVES = 4;
+constexpr int ANW_RS = 65;  // LDS row stride (doubles) of the reduction tile
 
-template <int K, typename OT>
-__device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, const double2 *__restrict__ qcm,
-                                           double2 *tb, const double *cseg, OT *col, int tid) {
-    constexpr int CH = 8;
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) {
+    return (cptr<T>)p;
+}
+
+__device__ __forceinline__ Item make_item_at(const LegArgs &a, int item, int pblk) {
+    Item it;
+    it.gs = item % a.gx;
+    it.pbi = (item / a.gx) % a.gy;
+    it.im = a.segm[it.gs];
+    it.s = it.gs - a.segbase[it.im];
+    it.m = it.im * a.mres;
+    it.l0 = it.m + it.s * a.seg;
+    it.lend = min(it.l0 + a.seg, a.lmax + 1);
+    it.lstop = min(it.lend, a.ltr + 1);
+    it.pb0 = it.pbi * pblk;
+    it.pb1 = min(it.pb0 + pblk, a.np);
+    it.pstart = max(it.pb0, a.pfirst[it.gs]);
+    it.fast = it.pstart >= a.pfast[it.gs];
+    it.f = item / (a.gx * a.gy);
+    return it;
+}
+
+template <int K, bool FAST, typename OT>
+__device__ __forceinline__ void anal_w_run(const LegArgs &a, const Item &it, double *red, OT *col, int lane) {
     const int np = a.np, nlat = a.nlat;
     const double2 *Fi = a.F0 + ((size_t)it.f * a.nm + it.im) * nlat;
     double x[K], prev[K], cur[K], gpr[K], gpi[K], gmr[K], gmi[K];
     int kk[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        const int p = it.pstart + tid + ANS_THREADS * j;
+        const int p = it.pstart + lane + 64 * j;
         const bool act = p < it.pb1;
         const int pc = act ? p : 0;
         x[j] = a.x[pc];
@@ -508,60 +538,33 @@ __device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, con
     }
     const double gm = a.g * mfac_of(a.mfac_mode, it.m);
     const int nst = it.lstop - it.l0;
-    const int lmax = a.lmax;
-    auto ldb = [&](int t, double *bt) {
+    const long long co = as_const(a.coff)[it.im] - it.m;
+    const cptr<double> qbm = as_const(a.qb) + co;
+    const cptr<double> qcm = as_const(a.qcc) + co;
+    const int o = lane & 15, qg = lane >> 4;
+    for (int c0 = 0; c0 < nst; c0 += 8) {
+        const int lc = __builtin_amdgcn_readfirstlane(it.l0 + c0);
+        double bq[8];
 #pragma unroll
-        for (int st = 0; st < CH; ++st) bt[st] = qcm[min(it.l0 + t + st, lmax) + 1].x;
-    };
-    double bcur[CH];
-    ldb(0, bcur);
-    for (int c0 = 0; c0 < nst; c0 += CH) {
-        double bnx[CH];
-        ldb(c0 + CH, bnx);
-        if (it.fast) {
+        for (int st = 0; st < 8; ++st) bq[st] = qbm[lc + st + 1];
 #pragma unroll
-            for (int st = 0; st < CH; ++st) {
-                const int t = c0 + st;
-                if (t < nst) {
-                    double pr = 0.0, pi = 0.0;
+        for (int st = 0; st < 8; ++st) {
+            double pr = 0.0, pi = 0.0;
 #pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        pr = fma(cur[j], (st & 1) ? gmr[j] : gpr[j], pr);
-                        pi = fma(cur[j], (st & 1) ? gmi[j] : gpi[j], pi);
-                    }
-                    tb[st * ANS_THREADS + tid] = make_double2(pr, pi);
-                    const double beta = bcur[st];
-#pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        const double nx = fma(x[j], cur[j], -(beta * prev[j]));
-                        prev[j] = cur[j];
-                        cur[j] = nx;
-                    }
-                }
+            for (int j = 0; j < K; ++j) {
+                const double v = FAST ? cur[j] : (kk[j] == 0 ? cur[j] : 0.0);
+                pr = fma(v, (st & 1) ? gmr[j] : gpr[j], pr);
+                pi = fma(v, (st & 1) ? gmi[j] : gpi[j], pi);
+                const double nx = fma(x[j], cur[j], -(bq[st] * prev[j]));
+                prev[j] = cur[j];
+                cur[j] = nx;
             }
-        } else {
-#pragma unroll
-            for (int st = 0; st < CH; ++st) {
-                const int t = c0 + st;
-                if (t < nst) {
-                    double pr = 0.0, pi = 0.0;
-#pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        const double v = kk[j] == 0 ? cur[j] : 0.0;
-                        pr = fma(v, (st & 1) ? gmr[j] : gpr[j], pr);
-                        pi = fma(v, (st & 1) ? gmi[j] : gpi[j], pi);
-                    }
-                    tb[st * ANS_THREADS + tid] = make_double2(pr, pi);
-                    const double beta = bcur[st];
-#pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        const double nx = fma(x[j], cur[j], -(beta * prev[j]));
-                        prev[j] = cur[j];
-                        cur[j] = nx;
-                    }
-                }
-            }
-            const double cl = cseg[c0 + min(CH, nst - c0) - 1];
+            // partial sums go to the wave's LDS tile right away
+            red[(2 * st) * ANW_RS + lane] = pr;
+            red[(2 * st + 1) * ANW_RS + lane] = pi;
+        }
+        if (!FAST) {
+            const double cl = qcm[lc + 7];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
@@ -571,55 +574,56 @@ __device__ __forceinline__ void anal_s_run(const LegArgs &a, const Item &it, con
                 }
             }
         }
-        __syncthreads();
-        // 8 steps x 128 partials: 16 threads per step
-        const int li = tid >> 4, q = tid & 15;
-        double sr = 0.0, si = 0.0;
+        // wave-private transpose-reduce of the 16 partial sums
+        __builtin_amdgcn_wave_barrier();
+        double sum = 0.0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const double2 v = tb[li * ANS_THREADS + q + 16 * i];
-            sr += v.x;
-            si += v.y;
+        for (int i = 0; i < 16; ++i) sum += red[o * ANW_RS + 16 * qg + i];
+        __builtin_amdgcn_wave_barrier();
+        sum += __shfl_xor(sum, 16);
+        sum += __shfl_xor(sum, 32);
+        const int t = c0 + (o >> 1);
+        if (qg == 0 && t < nst) {
+            const double v = gm * qcm[it.l0 + t] * sum;
+            OT *dst = col + 2 * (size_t)(it.l0 + t) + (o & 1);
+            if (a.atomic)
+                atomicAdd(dst, (OT)v);
+            else
+                *dst = (OT)v;
         }
-#pragma unroll
-        for (int msk = 8; msk >= 1; msk >>= 1) {
-            sr += __shfl_xor(sr, msk, 16);
-            si += __shfl_xor(si, msk, 16);
-        }
-        const int t = c0 + li;
-        if (q == 0 && t < nst) {
-            const double sc = gm * cseg[t];
-            if (a.atomic) add_c(col, it.l0 + t, sc * sr, sc * si);
-            else st_c(col, it.l0 + t, sc * sr, sc * si);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int st = 0; st < CH; ++st) bcur[st] = bnx[st];
     }
 }
 
-// The read-only coefficient tables come in as restrict kernel arguments so
-// that their wave-uniform reads compile to scalar loads.
+template <int K, typename OT>
+__device__ __forceinline__ void anal_w_dispatch(const LegArgs &a, const Item &it, double *red, OT *col, int lane) {
+    if (it.fast)
+        anal_w_run<K, true>(a, it, red, col, lane);
+    else
+        anal_w_run<K, false>(a, it, red, col, lane);
+}
+
 template <typename OT>
-__global__ __launch_bounds__(ANS_THREADS) void k_anal_s(const LegArgs a, const double2 *__restrict__ qc) {
-    __shared__ double2 tb[8 * ANS_THREADS];
-    __shared__ double cseg[SEG_MAX + 8];
-    const Item it = make_item<512>(a);
-    const int tid = threadIdx.x;
+__global__ __launch_bounds__(64 * ANW_WAVES) void k_anal_w(const LegArgs a) {
+    __shared__ double red_all[ANW_WAVES][16 * ANW_RS];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int item = blockIdx.x * ANW_WAVES + w;
+    if (item >= a.gx * a.gy * a.gz) return;
+    const Item it = make_item_at(a, item, 512);
     OT *col = (OT *)a.out0 + 2 * ((size_t)it.f * a.spec_fstride + (size_t)(a.lmax + 1) * it.m);
     const bool dead = it.lstop <= it.l0 || it.pstart >= it.pb1;
-    if (tid < 64) zero_fill(a, it, col, dead, tid);
+    zero_fill(a, it, col, dead, lane);
     if (dead) return;
-    const double2 *__restrict__ qcm = qc + a.coff[it.im] - it.m;
-    // c_l of this segment (l0 <= l < lstop; the tail up to a multiple of 8 is padded)
-    for (int t = tid; t < SEG_MAX + 8; t += ANS_THREADS) cseg[t] = qcm[min(it.l0 + t, a.lmax)].y;
-    __syncthreads();
-    const int keff = (it.pb1 - it.pstart + ANS_THREADS - 1) / ANS_THREADS;
+    double *red = red_all[w];
+    const int keff = (it.pb1 - it.pstart + 63) / 64;
     switch (keff) {
-        case 1: anal_s_run<1>(a, it, qcm, tb, cseg, col, tid); break;
-        case 2: anal_s_run<2>(a, it, qcm, tb, cseg, col, tid); break;
-        case 3: anal_s_run<3>(a, it, qcm, tb, cseg, col, tid); break;
-        default: anal_s_run<4>(a, it, qcm, tb, cseg, col, tid); break;
+        case 1: anal_w_dispatch<1>(a, it, red, col, lane); break;
+        case 2: anal_w_dispatch<2>(a, it, red, col, lane); break;
+        case 3: anal_w_dispatch<3>(a, it, red, col, lane); break;
+        case 4: anal_w_dispatch<4>(a, it, red, col, lane); break;
+        case 5: anal_w_dispatch<5>(a, it, red, col, lane); break;
+        case 6: anal_w_dispatch<6>(a, it, red, col, lane); break;
+        case 7: anal_w_dispatch<7>(a, it, red, col, lane); break;
+        default: anal_w_dispatch<8>(a, it, red, col, lane); break;
     }
 }
 
@@ -894,10 +898,11 @@ hipError_t launch_leg_anal(const LegArgs &a0, int nfields, int vector, hipStream
         else
             hipLaunchKernelGGL(k_anal_v<double>, grid, dim3(ANV_THREADS), 0, st, a, a.qc, a.qd);
     } else {
+        const dim3 gw((unsigned)(((long long)a.gx * a.gy * nfields + ANW_WAVES - 1) / ANW_WAVES));
         if (a.io_f32)
-            hipLaunchKernelGGL(k_anal_s<float>, grid, dim3(ANS_THREADS), 0, st, a, a.qc);
+            hipLaunchKernelGGL(k_anal_w<float>, gw, dim3(64 * ANW_WAVES), 0, st, a);
         else
-            hipLaunchKernelGGL(k_anal_s<double>, grid, dim3(ANS_THREADS), 0, st, a, a.qc);
+            hipLaunchKernelGGL(k_anal_w<double>, gw, dim3(64 * ANW_WAVES), 0, st, a);
     }
     return hipGetLastError();
 }

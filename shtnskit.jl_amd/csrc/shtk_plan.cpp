@@ -122,7 +122,7 @@ void free_plan(shtk_plan *p) {
                     p->qd,     p->coff,   p->segbase, p->seedq0, p->seedq1, p->seedk, p->pfirst,
                     p->pfast,  p->segm,   p->convscale, p->tw,   p->ws,     p->sectv, p->sectk, p->oz_q0,
                     p->oz_q1,  p->oz_qk,  p->oz_ep,     p->oz_chbase, p->oz_ws, p->oz_emaxl, p->oz_epl,
-                    p->oz_asc, p->oz_chm, p->oz_run, p->oz_qb, p->oz_qcc};
+                    p->oz_asc, p->oz_chm, p->oz_run, p->qb, p->qcc};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -217,6 +217,8 @@ shtk::LegArgs base_args(const shtk_plan *p, int ltr) {
     a.x = p->x;
     a.qc = p->qc;
     a.qd = p->qd;
+    a.qb = p->qb;
+    a.qcc = p->qcc;
     a.ab = p->ab;
     a.gtab = p->gtab;
     a.coff = p->coff;
@@ -294,8 +296,6 @@ int ensure_oz_tables(shtk_plan *p, hipStream_t st) {
     const size_t npad = p->qc_len + shtk::OZ_TPAD;
     HIP_TRY(hipMalloc((void **)&p->oz_epl, sizeof(int) * npad), "split tables");
     HIP_TRY(hipMalloc((void **)&p->oz_asc, sizeof(double) * npad), "split tables");
-    HIP_TRY(hipMalloc((void **)&p->oz_qb, sizeof(double) * npad), "split tables");
-    HIP_TRY(hipMalloc((void **)&p->oz_qcc, sizeof(double) * npad), "split tables");
     // analysis work runs: up to oz_run_len consecutive segments of one order,
     // orders of equal run count split evenly, lowest order (longest) first
     std::vector<int4> runs;
@@ -332,8 +332,6 @@ int ensure_oz_tables(shtk_plan *p, hipStream_t st) {
     aa.emaxl = p->oz_emaxl;
     aa.asc = p->oz_asc;
     aa.epl = p->oz_epl;
-    aa.qb = p->oz_qb;
-    aa.qcc = p->oz_qcc;
     HIP_TRY(shtk::launch_oz_asc(aa, st), "split scale kernel");
     HIP_TRY(hipStreamSynchronize(st), "split seed kernel");
     return SHTK_OK;
@@ -433,8 +431,8 @@ shtk::OzArgs oz_anal_args(const shtk_plan *p, double2 *F0, double *acc, int nf, 
     o.qc = p->qc;
     o.asc = p->oz_asc;
     o.epl = p->oz_epl;
-    o.qb = p->oz_qb;
-    o.qcc = p->oz_qcc;
+    o.qb = p->qb;
+    o.qcc = p->qcc;
     o.seedq0 = p->seedq0;
     o.seedq1 = p->seedq1;
     o.seedk = p->seedk;
@@ -1078,6 +1076,17 @@ static int plan_tables(shtk_plan *p) {
     RC_TRY(upload(&p->invs, invs, "plan tables"));
     RC_TRY(upload(&p->qc, qc, "plan tables"));
     p->qc_len = qc.size();
+    {
+        // contiguous, padded copies of the Q-form columns: kernels read whole
+        // blocks of 8 (or 32) degrees through scalar loads without clamping
+        std::vector<double> qb(qc.size() + shtk::OZ_TPAD, 0.0), qcc(qc.size() + shtk::OZ_TPAD, 0.0);
+        for (size_t i = 0; i < qc.size(); ++i) {
+            qb[i] = qc[i].x;
+            qcc[i] = qc[i].y;
+        }
+        RC_TRY(upload(&p->qb, qb, "plan tables"));
+        RC_TRY(upload(&p->qcc, qcc, "plan tables"));
+    }
     RC_TRY(upload(&p->qd, qd, "plan tables"));
     RC_TRY(upload(&p->ab, ab, "plan tables"));
     RC_TRY(upload(&p->gtab, gt, "plan tables"));

@@ -370,26 +370,20 @@ __global__ __launch_bounds__(256) void k_synth_oz(const OzArgs a) {
 // Per-degree digit scales for the analysis: |P_l| < 2^epl over all pairs, and
 // asc = c_l 2^(46 - epl) so that one FMA turns Q_l into the digit value.
 // ---------------------------------------------------------------------------
-// The same kernel also splits the Q-form table into contiguous beta_l and c_l
-// arrays for the analysis, and fills OZ_TPAD entries past the end of every
-// table (so the analysis can read whole 8- and 32-degree blocks unclamped).
+// OZ_TPAD entries past the end of both tables are zero (the analysis reads
+// whole 8- and 32-degree blocks unclamped).
 __global__ __launch_bounds__(256) void k_oz_asc(const OzAscArgs a) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n + OZ_TPAD) return;
     if (i >= a.n) {
         a.epl[i] = 0;
         a.asc[i] = 0.0;
-        a.qb[i] = 0.0;
-        a.qcc[i] = 0.0;
         return;
     }
     const int e = a.emaxl[i];
     const int ep = e < -201 ? -200 : min(e + 1, 64);
-    const double2 q = a.qc[i];
     a.epl[i] = ep;
-    a.asc[i] = q.y * ldexp(1.0, 46 - ep);
-    a.qb[i] = q.x;
-    a.qcc[i] = q.y;
+    a.asc[i] = a.qc[i].y * ldexp(1.0, 46 - ep);
 }
 
 // ---------------------------------------------------------------------------

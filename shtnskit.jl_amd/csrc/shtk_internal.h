@@ -279,6 +279,9 @@ hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st
 hipError_t launch_spec_scale(const void *in, void *out, int in_f32, int out_f32, const double *scale, int divide,
                              int lmax, int mmax, int mres, long long nelem_per_field, int nfields,
                              hipStream_t st);
+hipError_t launch_spec_qscale(const void *in, double *out, int in_f32, const double *scale, int divide,
+                              const double *qcc, const long long *coff, int lmax, int mres, int nm,
+                              long long per_field, int nfields, hipStream_t st);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
 
 }  // namespace shtk

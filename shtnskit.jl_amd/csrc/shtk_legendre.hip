@@ -61,6 +61,15 @@ __device__ __forceinline__ void add_c(T *p, size_t idx, double re, double im) {
 
 __device__ __forceinline__ int nseg_of(int lmax, int m, int S) { return (lmax - m + S) / S; }
 
+// Read-only tables read through the constant address space: wave-uniform
+// reads become scalar loads, also in kernels that write global memory.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) {
+    return (cptr<T>)p;
+}
+
 
 // ---------------------------------------------------------------------------
 // Plan-time seeds: one thread per (order, latitude pair) runs the P-form
@@ -124,22 +133,26 @@ __global__ __launch_bounds__(256) void k_seed(const SeedArgs a) {
 template <int K, int PAR, bool FAST>
 __device__ __forceinline__ void sstep(const double4 &r, const double *x, double *prev, double *cur, const double *cm,
                                       double (*acc)[4]) {
+    // r = (beta_{l+1}, c_l, Re c_l a_l, Im c_l a_l): the coefficients arrive
+    // scaled by c_l (k_spec_qscale)
+    const double cr = r.z, ci = r.w;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const double v = FAST ? cur[j] : cur[j] * cm[j];
-        acc[j][2 * PAR] = fma(v, r.z, acc[j][2 * PAR]);
-        acc[j][2 * PAR + 1] = fma(v, r.w, acc[j][2 * PAR + 1]);
-        const double nx = fma(r.x * x[j], cur[j], -(r.y * prev[j]));
+        acc[j][2 * PAR] = fma(v, cr, acc[j][2 * PAR]);
+        acc[j][2 * PAR + 1] = fma(v, ci, acc[j][2 * PAR + 1]);
+        const double nx = fma(x[j], cur[j], -(r.x * prev[j]));
         prev[j] = cur[j];
         cur[j] = nx;
     }
 }
 
+// c: the Q-form scale of the current degree (1 for the P form)
 template <int K>
-__device__ __forceinline__ void rescale(double *prev, double *cur, int *kk, double *cm) {
+__device__ __forceinline__ void rescale(double *prev, double *cur, int *kk, double *cm, double c = 1.0) {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) > kHi) {
+        if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * c > kHi) {
             cur[j] *= kDown;
             prev[j] *= kDown;
             kk[j] -= 1;
@@ -149,17 +162,21 @@ __device__ __forceinline__ void rescale(double *prev, double *cur, int *kk, doub
 }
 
 // Degrees l0 .. lstop-1 of one segment; (l - l0) has the parity of (l - m).
+// Q form: Q_{l+1} = x Q_l - beta_{l+1} Q_{l-1}, P_l = c_l Q_l, so the
+// contraction weight of degree l is c_l a_l (pre-scaled by k_spec_qscale) and a
+// chain step costs one multiply and three FMAs.
 template <int K, bool FAST, typename IT>
-__device__ __forceinline__ void synth_seg(const double2 *__restrict__ ab, const IT *__restrict__ al, int l0,
+__device__ __forceinline__ void synth_seg(cptr<double> qbm, cptr<double> qcm, const IT *__restrict__ al, int l0,
                                           int lstop, int lmax, const double *x, double *prev, double *cur,
                                           int *kk, double (*acc)[4]) {
     double cm[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) cm[j] = kk[j] == 0 ? 1.0 : 0.0;
+    // beta_{l+1}, c_l, c_l a_l; the tables are padded, the coefficient column is not
     auto ld = [&](int l) {
         const int lc = min(l, lmax);
-        const double2 c = ab[lc + 1];
-        return make_double4(c.x, c.y, (double)al[2 * lc], (double)al[2 * lc + 1]);
+        const double c = qcm[l];
+        return make_double4(qbm[l + 1], c, (double)al[2 * lc], (double)al[2 * lc + 1]);
     };
     double4 r0 = ld(l0), r1 = ld(l0 + 1), r2 = ld(l0 + 2), r3 = ld(l0 + 3);
     int l = l0;
@@ -169,7 +186,7 @@ __device__ __forceinline__ void synth_seg(const double2 *__restrict__ ab, const 
         sstep<K, 1, FAST>(r1, x, prev, cur, cm, acc);
         sstep<K, 0, FAST>(r2, x, prev, cur, cm, acc);
         sstep<K, 1, FAST>(r3, x, prev, cur, cm, acc);
-        if (!FAST && ((l - l0) & (SCALE_CHECK - 1)) == SCALE_CHECK - 4) rescale<K>(prev, cur, kk, cm);
+        if (!FAST && ((l - l0) & (SCALE_CHECK - 1)) == SCALE_CHECK - 4) rescale<K>(prev, cur, kk, cm, r3.y);
         r0 = n0;
         r1 = n1;
         r2 = n2;
@@ -197,7 +214,8 @@ __global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
         const int p = pb0 + lane + 64 * j;
         x[j] = p < np ? a.x[p] : 0.0;
     }
-    const double2 *__restrict__ ab = a.ab + a.coff[im] - m;
+    const long long co = as_const(a.coff)[im] - m;
+    const cptr<double> qbm = as_const(a.qb) + co, qcm = as_const(a.qcc) + co;
     const IT *__restrict__ al = (const IT *)a.in0 + 2 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m);
     double acc[K][4];
 #pragma unroll
@@ -225,19 +243,18 @@ __global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
         const int gs = a.segbase[im] + s;
         double prev[K], cur[K];
         int kk[K];
-        const double c0 = s == 0 ? 1.0 : ab[l0].x;  // P_{l0} = a_{l0} Q_{l0}
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             prev[j] = nq0[j];
-            cur[j] = nq1[j] * c0;
+            cur[j] = nq1[j];
             kk[j] = nqk[j];
         }
         if (s + 4 < nseg) load_seeds(s + 4);
         if (pb1 <= a.pfirst[gs]) continue;
         const bool fast = pb0 >= a.pfast[gs];
         const int lstop = min(l0 + S, ltr + 1);
-        if (fast) synth_seg<K, true>(ab, al, l0, lstop, lmax, x, prev, cur, kk, acc);
-        else synth_seg<K, false>(ab, al, l0, lstop, lmax, x, prev, cur, kk, acc);
+        if (fast) synth_seg<K, true>(qbm, qcm, al, l0, lstop, lmax, x, prev, cur, kk, acc);
+        else synth_seg<K, false>(qbm, qcm, al, l0, lstop, lmax, x, prev, cur, kk, acc);
     }
 #pragma unroll
     for (int j = 0; j < K; ++j)
@@ -481,13 +498,6 @@ __device__ void zero_fill(const LegArgs &a, const Item &it, OT *col, bool all_de
 // loads from the padded tables.
 constexpr int ANW_WAVES = 4;
 constexpr int ANW_RS = 65;  // LDS row stride (doubles) of the reduction tile
-
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-template <typename T>
-__device__ __forceinline__ cptr<T> as_const(const T *p) {
-    return (cptr<T>)p;
-}
 
 __device__ __forceinline__ Item make_item_at(const LegArgs &a, int item, int pblk) {
     Item it;
@@ -844,6 +854,28 @@ __global__ void k_spec_scale(const IT *in, OT *out, const double *scale, int div
     out[2 * q + 1] = (OT)im;
 }
 
+// Coefficients of the scalar synthesis in the Q form: out = c_l(m) * conv(l, m)
+// * in for l >= m of every order of the plan (other entries are not read by the
+// synthesis and are left untouched).  One block per (order, field).
+template <typename IT>
+__global__ __launch_bounds__(256) void k_spec_qscale(const IT *in, double *out, const double *scale, int divide,
+                                                     const double *qcc, const long long *coff, int lmax, int mres,
+                                                     int nm, long long per_field) {
+    const int im = blockIdx.x % nm, f = blockIdx.x / nm, m = im * mres;
+    const double *cm = qcc + coff[im] - m;
+    const long long base = (long long)f * per_field + (long long)(lmax + 1) * m;
+    for (int l = m + (int)threadIdx.x; l <= lmax; l += 256) {
+        const long long q = base + l;
+        double s = cm[l];
+        if (scale) {
+            const double cs = scale[q - (long long)f * per_field];
+            s = divide ? s / cs : s * cs;
+        }
+        out[2 * q] = s * (double)in[2 * q];
+        out[2 * q + 1] = s * (double)in[2 * q + 1];
+    }
+}
+
 __global__ void k_zero(unsigned long long *p, size_t n8) {
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n8) p[q] = 0ull;
@@ -923,6 +955,19 @@ hipError_t launch_spec_scale(const void *in, void *out, int in_f32, int out_f32,
     else
         hipLaunchKernelGGL((k_spec_scale<double, double>), grid, dim3(256), 0, st, (const double *)in,
                            (double *)out, scale, divide, lmax, mmax, mres, per_field, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_spec_qscale(const void *in, double *out, int in_f32, const double *scale, int divide,
+                              const double *qcc, const long long *coff, int lmax, int mres, int nm,
+                              long long per_field, int nfields, hipStream_t st) {
+    const dim3 grid((unsigned)nm * nfields);
+    if (in_f32)
+        hipLaunchKernelGGL(k_spec_qscale<float>, grid, dim3(256), 0, st, (const float *)in, out, scale, divide, qcc,
+                           coff, lmax, mres, nm, per_field);
+    else
+        hipLaunchKernelGGL(k_spec_qscale<double>, grid, dim3(256), 0, st, (const double *)in, out, scale, divide,
+                           qcc, coff, lmax, mres, nm, per_field);
     return hipGetLastError();
 }
 

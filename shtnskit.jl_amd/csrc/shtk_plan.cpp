@@ -163,7 +163,9 @@ WsLayout ws_layout(const shtk_plan *p, int nf, int vector) {
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     WsLayout L;
     L.f_bytes = al(sizeof(double2) * (size_t)nf * p->nm * p->d.nlat);
-    L.spec_bytes = p->canonical ? 0 : al(sizeof(double2) * (size_t)nf * (p->d.lmax + 1) * (p->d.mmax + 1));
+    // spectral staging: canonical-convention copies (non-canonical plans) and the
+    // Q-form coefficients of the scalar synthesis
+    L.spec_bytes = al(sizeof(double2) * (size_t)nf * (p->d.lmax + 1) * (p->d.mmax + 1));
     L.total = (vector ? 2 : 1) * (L.f_bytes + L.spec_bytes);
     return L;
 }
@@ -490,7 +492,16 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
     a.in0 = s0;
     a.in1 = s1;
     a.io_f32 = dtype == SHTK_F32;
-    if (!p->canonical) {
+    if (!j.vector && j.prec != SHTK_PREC_SPLIT) {
+        // scalar FP64 synthesis: coefficients times c_l (Q form), converted to the
+        // canonical convention on the way
+        double *dst = (double *)spec;
+        HIP_TRY(shtk::launch_spec_qscale(s0, dst, a.io_f32, p->canonical ? nullptr : p->convscale, j.conv_div,
+                                         p->qcc, p->coff, p->d.lmax, p->d.mres, p->nm, a.spec_fstride, nf, st),
+                "coefficient scaling");
+        a.in0 = dst;
+        a.io_f32 = 0;
+    } else if (!p->canonical) {
         const void *src[2] = {s0, s1};
         for (int q = 0; q <= j.vector; ++q) {
             void *dst = spec + q * L.spec_bytes;

@@ -282,6 +282,9 @@ hipError_t launch_spec_scale(const void *in, void *out, int in_f32, int out_f32,
 hipError_t launch_spec_qscale(const void *in, double *out, int in_f32, const double *scale, int divide,
                               const double *qcc, const long long *coff, int lmax, int mres, int nm,
                               long long per_field, int nfields, hipStream_t st);
+hipError_t launch_vspec_prep(const void *S, const void *T, double *rec, int in_f32, const double *scale, int divide,
+                             int ll1, const double *qcc, const long long *coff, int lmax, int mres, int nm,
+                             long long per_field, int nfields, hipStream_t st);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
 
 }  // namespace shtk

@@ -286,79 +286,78 @@ __global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
 // Robert form).  Parity: P(-x) = (-1)^(l-m) P(x), D(-x) = -(-1)^(l-m) D(x).
 // Accumulators per parity: A = sum D S, B = sum P T, C = sum P S, E = sum D T.
 // ---------------------------------------------------------------------------
-struct VCoef {
-    double a, b, g, l;      // recurrence a_{l+1}, b_{l+1}; g_l; l
-    double sr, si, tr, ti;  // S_l, T_l (divided by l(l+1) when requested)
-};
-
 template <int K, int PAR, bool FAST>
-__device__ __forceinline__ void vstep(const VCoef &r, const double *x, double *prev, double *cur, const double *cm,
-                                      double (*acc)[2][8]) {
+__device__ __forceinline__ void vstep(double beta, double dl, double dg, const double *R, const double *x,
+                                      double *prev, double *cur, const double *cm, double (*acc)[2][8]) {
+    // Q form: D_l = l c_l x Q_l - g_l c_{l-1} Q_{l-1} = dl (x Q_l) - dg Q_{l-1},
+    // P_l = c_l Q_l; R = (S', T', c_l S', c_l T') of this degree
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        const double t = x[j] * cur[j];
-        double D = fma(r.l, t, -(r.g * prev[j]));
-        double P = cur[j];
+        const double xc = x[j] * cur[j];
+        double D = fma(dl, xc, -(dg * prev[j]));
+        double Q = cur[j];
         if (!FAST) {
             D *= cm[j];
-            P *= cm[j];
+            Q *= cm[j];
         }
         double *A = acc[j][PAR];
-        A[0] = fma(D, r.sr, A[0]);
-        A[1] = fma(D, r.si, A[1]);
-        A[2] = fma(P, r.tr, A[2]);
-        A[3] = fma(P, r.ti, A[3]);
-        A[4] = fma(P, r.sr, A[4]);
-        A[5] = fma(P, r.si, A[5]);
-        A[6] = fma(D, r.tr, A[6]);
-        A[7] = fma(D, r.ti, A[7]);
-        const double nx = fma(r.a, t, -(r.b * prev[j]));
+        A[0] = fma(D, R[0], A[0]);
+        A[1] = fma(D, R[1], A[1]);
+        A[2] = fma(Q, R[6], A[2]);
+        A[3] = fma(Q, R[7], A[3]);
+        A[4] = fma(Q, R[4], A[4]);
+        A[5] = fma(Q, R[5], A[5]);
+        A[6] = fma(D, R[2], A[6]);
+        A[7] = fma(D, R[3], A[7]);
+        const double nx = fma(-beta, prev[j], xc);
         prev[j] = cur[j];
         cur[j] = nx;
     }
 }
 
-template <int K, bool FAST, typename IT>
-__device__ __forceinline__ void vsynth_seg(const double2 *__restrict__ ab, const double *__restrict__ gt,
-                                           const IT *__restrict__ Sl, const IT *__restrict__ Tl, int ll1, int l0,
-                                           int lstop, int lmax, const double *x, double *prev, double *cur,
-                                           int *kk, double (*acc)[2][8]) {
+// One degree's uniform operands: beta_{l+1}, (l c_l, g_l c_{l-1}) and the
+// prologue record (S', T', c_l S', c_l T'), all through scalar loads.
+struct VDeg {
+    double beta, dl, dg, R[8];
+};
+
+template <int K, bool FAST>
+__device__ __forceinline__ void vsynth_seg(cptr<double> qbm, cptr<double> qdm, cptr<double> qcm, cptr<double> rec,
+                                           int l0, int lstop, int lmax, const double *x, double *prev,
+                                           double *cur, int *kk, double (*acc)[2][8]) {
     double cm[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) cm[j] = kk[j] == 0 ? 1.0 : 0.0;
     auto ld = [&](int l) {
+        VDeg d;
+        d.beta = qbm[l + 1];
+        const int ld2 = 2 * min(l, lmax + 1);
+        d.dl = qdm[ld2];
+        d.dg = qdm[ld2 + 1];
         const int lc = min(l, lmax);
-        const double2 c = ab[lc + 1];
-        VCoef r;
-        r.a = c.x;
-        r.b = c.y;
-        r.g = gt[lc];
-        r.l = (double)lc;
-        double sc = 1.0;
-        if (ll1) sc = lc > 0 ? 1.0 / ((double)lc * (lc + 1)) : 0.0;
-        r.sr = sc * (double)Sl[2 * lc];
-        r.si = sc * (double)Sl[2 * lc + 1];
-        r.tr = sc * (double)Tl[2 * lc];
-        r.ti = sc * (double)Tl[2 * lc + 1];
-        return r;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) d.R[k] = rec[8 * lc + k];
+        return d;
     };
-    VCoef r0 = ld(l0), r1 = ld(l0 + 1);
+    // two degrees (both parities) per iteration, the next two loaded meanwhile
+    VDeg r0 = ld(l0), r1 = ld(l0 + 1);
     int l = l0;
     for (; l + 2 <= lstop; l += 2) {
-        const VCoef n0 = ld(l + 2), n1 = ld(l + 3);
-        vstep<K, 0, FAST>(r0, x, prev, cur, cm, acc);
-        vstep<K, 1, FAST>(r1, x, prev, cur, cm, acc);
-        if (!FAST && ((l - l0) & (SCALE_CHECK - 1)) == SCALE_CHECK - 2) rescale<K>(prev, cur, kk, cm);
+        const VDeg n0 = ld(l + 2), n1 = ld(l + 3);
+        vstep<K, 0, FAST>(r0.beta, r0.dl, r0.dg, r0.R, x, prev, cur, cm, acc);
+        vstep<K, 1, FAST>(r1.beta, r1.dl, r1.dg, r1.R, x, prev, cur, cm, acc);
+        if (!FAST && ((l - l0) & (SCALE_CHECK - 1)) == SCALE_CHECK - 2) rescale<K>(prev, cur, kk, cm, qcm[l + 2]);
         r0 = n0;
         r1 = n1;
     }
-    if (l < lstop) vstep<K, 0, FAST>(r0, x, prev, cur, cm, acc);
+    if (l < lstop) vstep<K, 0, FAST>(r0.beta, r0.dl, r0.dg, r0.R, x, prev, cur, cm, acc);
 }
 
-template <int K, typename IT>
+constexpr int VS_CHAINS = 2;
+
+template <int K>
 __global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
     __shared__ double red[4][16][64];
-    static_assert(K == 1, "vector synthesis reduces one chain per lane");
     int b, unit;
     if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
     const int im = unit % a.gy, f = unit / a.gy;
@@ -373,11 +372,12 @@ __global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
         const int p = pb0 + lane + 64 * j;
         x[j] = p < np ? a.x[p] : 0.0;
     }
-    const double2 *__restrict__ ab = a.ab + a.coff[im] - m;
-    const double *__restrict__ gt = a.gtab + a.coff[im] - m;
-    const size_t co = 2 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m);
-    const IT *__restrict__ Sl = (const IT *)a.in0 + co;
-    const IT *__restrict__ Tl = (const IT *)a.in1 + co;
+    const long long co = as_const(a.coff)[im] - m;
+    const cptr<double> qbm = as_const(a.qb) + co, qcm = as_const(a.qcc) + co;
+    const cptr<double> qdm = as_const((const double *)a.qd) + 2 * co;
+    // prologue records (k_vspec_prep), same element offsets as the spectral arrays
+    const cptr<double> rec =
+        as_const((const double *)a.in0) + 8 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m);
     double acc[K][2][8];
 #pragma unroll
     for (int j = 0; j < K; ++j)
@@ -393,7 +393,6 @@ __global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
         if (pb1 <= a.pfirst[gs]) continue;
         const bool fast = pb0 >= a.pfast[gs];
         const int lstop = min(l0 + S, ltr + 1);
-        const double c0 = s == 0 ? 1.0 : ab[l0].x;
         double prev[K], cur[K];
         int kk[K];
 #pragma unroll
@@ -402,46 +401,77 @@ __global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
             const bool act = p < np;
             const size_t q = (size_t)gs * np + (act ? p : 0);
             prev[j] = act ? a.seedq0[q] : 0.0;
-            cur[j] = act ? a.seedq1[q] * c0 : 0.0;
+            cur[j] = act ? a.seedq1[q] : 0.0;
             kk[j] = act ? a.seedk[q] : 0;
         }
-        if (fast) vsynth_seg<K, true>(ab, gt, Sl, Tl, a.ll1_mode, l0, lstop, lmax, x, prev, cur, kk, acc);
-        else vsynth_seg<K, false>(ab, gt, Sl, Tl, a.ll1_mode, l0, lstop, lmax, x, prev, cur, kk, acc);
-    }
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int r = 0; r < 8; ++r) red[w][8 * q + r][lane] = acc[0][q][r];
-    __syncthreads();
-    if (w != 0) return;
-    const int p = pb0 + lane;
-    if (p >= np) return;
-    double e[8], o[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        e[r] = ((red[0][r][lane] + red[1][r][lane]) + red[2][r][lane]) + red[3][r][lane];
-        o[r] = ((red[0][8 + r][lane] + red[1][8 + r][lane]) + red[2][8 + r][lane]) + red[3][8 + r][lane];
+        if (fast) vsynth_seg<K, true>(qbm, qdm, qcm, rec, l0, lstop, lmax, x, prev, cur, kk, acc);
+        else vsynth_seg<K, false>(qbm, qdm, qcm, rec, l0, lstop, lmax, x, prev, cur, kk, acc);
     }
     const double mm = (double)m;
-#pragma unroll
-    for (int r = 2; r < 6; ++r) {
-        e[r] *= mm;
-        o[r] *= mm;
-    }
     const double gm = a.g * mfac_of(a.mfac_mode, m);
-    const double sc = gm * (a.wlat ? a.wlat[p] : 1.0);
-    // [0..1] = A, [2..3] = m B, [4..5] = m C, [6..7] = E
-    const double tNr = e[0] + o[0] + (e[3] + o[3]), tNi = e[1] + o[1] - (e[2] + o[2]);
-    const double tSr = -e[0] + o[0] + (e[3] - o[3]), tSi = -e[1] + o[1] - (e[2] - o[2]);
-    const double pNr = -(e[5] + o[5]) + e[6] + o[6], pNi = e[4] + o[4] + e[7] + o[7];
-    const double pSr = -(e[5] - o[5]) - e[6] + o[6], pSi = (e[4] - o[4]) - e[7] + o[7];
     const size_t fo = ((size_t)f * a.nm + im) * a.nlat;
-    a.F0[fo + p] = make_double2(tNr * sc, tNi * sc);
-    a.F1[fo + p] = make_double2(pNr * sc, pNi * sc);
-    const int ps = a.nlat - 1 - p;
-    if (ps != p) {
-        a.F0[fo + ps] = make_double2(tSr * sc, tSi * sc);
-        a.F1[fo + ps] = make_double2(pSr * sc, pSi * sc);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (j > 0) __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 8; ++r) red[w][8 * q + r][lane] = acc[j][q][r];
+        __syncthreads();
+        const int p = pb0 + lane + 64 * j;
+        if (w == 0 && p < np) {
+            double e[8], o[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                e[r] = ((red[0][r][lane] + red[1][r][lane]) + red[2][r][lane]) + red[3][r][lane];
+                o[r] = ((red[0][8 + r][lane] + red[1][8 + r][lane]) + red[2][8 + r][lane]) + red[3][8 + r][lane];
+            }
+#pragma unroll
+            for (int r = 2; r < 6; ++r) {
+                e[r] *= mm;
+                o[r] *= mm;
+            }
+            const double sc = gm * (a.wlat ? a.wlat[p] : 1.0);
+            // [0..1] = A, [2..3] = m B, [4..5] = m C, [6..7] = E
+            const double tNr = e[0] + o[0] + (e[3] + o[3]), tNi = e[1] + o[1] - (e[2] + o[2]);
+            const double tSr = -e[0] + o[0] + (e[3] - o[3]), tSi = -e[1] + o[1] - (e[2] - o[2]);
+            const double pNr = -(e[5] + o[5]) + e[6] + o[6], pNi = e[4] + o[4] + e[7] + o[7];
+            const double pSr = -(e[5] - o[5]) - e[6] + o[6], pSi = (e[4] - o[4]) - e[7] + o[7];
+            a.F0[fo + p] = make_double2(tNr * sc, tNi * sc);
+            a.F1[fo + p] = make_double2(pNr * sc, pNi * sc);
+            const int ps = a.nlat - 1 - p;
+            if (ps != p) {
+                a.F0[fo + ps] = make_double2(tSr * sc, tSi * sc);
+                a.F1[fo + ps] = make_double2(pSr * sc, pSi * sc);
+            }
+        }
+    }
+}
+
+// Prologue of the vector synthesis: per (order, field) and l >= m the record
+// (S', T', c_l S', c_l T') with S' = conv(l, m) S [/ l(l+1)], T' likewise.
+template <typename IT>
+__global__ __launch_bounds__(256) void k_vspec_prep(const IT *S, const IT *T, double *rec, const double *scale,
+                                                    int divide, int ll1, const double *qcc,
+                                                    const long long *coff, int lmax, int mres, int nm,
+                                                    long long per_field) {
+    const int im = blockIdx.x % nm, f = blockIdx.x / nm, m = im * mres;
+    const double *cm = qcc + coff[im] - m;
+    const long long base = (long long)f * per_field + (long long)(lmax + 1) * m;
+    for (int l = m + (int)threadIdx.x; l <= lmax; l += 256) {
+        const long long q = base + l;
+        double sc = 1.0;
+        if (scale) {
+            const double cs = scale[q - (long long)f * per_field];
+            sc = divide ? 1.0 / cs : cs;
+        }
+        if (ll1) sc = l > 0 ? sc / ((double)l * (l + 1)) : 0.0;
+        const double c = cm[l];
+        const double sr = sc * (double)S[2 * q], si = sc * (double)S[2 * q + 1];
+        const double tr = sc * (double)T[2 * q], ti = sc * (double)T[2 * q + 1];
+        double4 *r = (double4 *)(rec + 8 * q);
+        r[0] = make_double4(sr, si, tr, ti);
+        r[1] = make_double4(c * sr, c * si, c * tr, c * ti);
     }
 }
 
@@ -748,19 +778,20 @@ __device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, con
         }
         __syncthreads();
         {
-            // 8 (step, quantity) combos x 128 partials: 16 threads per combo
-            const int combo = tid >> 4, q = tid & 15;
+            // 8 (step, quantity) combos x ANV_THREADS partials: TPC threads per combo
+            constexpr int TPC = ANV_THREADS / 8;
+            const int combo = tid / TPC, q = tid % TPC;
             double sr = 0.0, si = 0.0;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const double2 v = tb[combo * ANV_THREADS + q + 16 * i];
+            for (int i = 0; i < ANV_THREADS / TPC; ++i) {
+                const double2 v = tb[combo * ANV_THREADS + q + TPC * i];
                 sr += v.x;
                 si += v.y;
             }
 #pragma unroll
-            for (int msk = 8; msk >= 1; msk >>= 1) {
-                sr += __shfl_xor(sr, msk, 16);
-                si += __shfl_xor(si, msk, 16);
+            for (int msk = TPC / 2; msk >= 1; msk >>= 1) {
+                sr += __shfl_xor(sr, msk, TPC);
+                si += __shfl_xor(si, msk, TPC);
             }
             if (q == 0) red[combo] = make_double2(sr, si);
         }
@@ -896,10 +927,8 @@ static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
     a.gz = nfields;
     const dim3 grid(grid_for_units(a.xcd, a.gx, a.gy * a.gz));
     if (vector) {
-        if (a.io_f32)
-            hipLaunchKernelGGL((k_synth_v<1, float>), grid, dim3(256), 0, st, a);
-        else
-            hipLaunchKernelGGL((k_synth_v<1, double>), grid, dim3(256), 0, st, a);
+        // inputs are the FP64 prologue records of k_vspec_prep
+        hipLaunchKernelGGL((k_synth_v<K>), grid, dim3(256), 0, st, a);
     } else {
         if (a.io_f32)
             hipLaunchKernelGGL((k_synth_s<K, float>), grid, dim3(256), 0, st, a);
@@ -910,15 +939,16 @@ static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
 
 hipError_t launch_leg_synth(const LegArgs &a, int nfields, int vector, int chains, hipStream_t st) {
     if (a.seg > SEG_MAX || (a.seg & 7)) return hipErrorInvalidValue;
-    if (vector || chains <= 1) launch_synth_k<1>(a, nfields, vector, st);
-    else if (chains == 2) launch_synth_k<2>(a, nfields, vector, st);
-    else launch_synth_k<4>(a, nfields, vector, st);
+    if (vector) launch_synth_k<VS_CHAINS>(a, nfields, 1, st);
+    else if (chains <= 1) launch_synth_k<1>(a, nfields, 0, st);
+    else if (chains == 2) launch_synth_k<2>(a, nfields, 0, st);
+    else launch_synth_k<4>(a, nfields, 0, st);
     return hipGetLastError();
 }
 
 hipError_t launch_leg_anal(const LegArgs &a0, int nfields, int vector, hipStream_t st) {
     if (a0.seg > SEG_MAX || (a0.seg & 7)) return hipErrorInvalidValue;
-    const int pblk = vector ? 256 : 512;
+    const int pblk = vector ? 256 : 512;  // pairs per work item
     LegArgs a = a0;
     a.gx = a.nseg_total;
     a.gy = (a.np + pblk - 1) / pblk;
@@ -968,6 +998,19 @@ hipError_t launch_spec_qscale(const void *in, double *out, int in_f32, const dou
     else
         hipLaunchKernelGGL(k_spec_qscale<double>, grid, dim3(256), 0, st, (const double *)in, out, scale, divide,
                            qcc, coff, lmax, mres, nm, per_field);
+    return hipGetLastError();
+}
+
+hipError_t launch_vspec_prep(const void *S, const void *T, double *rec, int in_f32, const double *scale, int divide,
+                             int ll1, const double *qcc, const long long *coff, int lmax, int mres, int nm,
+                             long long per_field, int nfields, hipStream_t st) {
+    const dim3 grid((unsigned)nm * nfields);
+    if (in_f32)
+        hipLaunchKernelGGL(k_vspec_prep<float>, grid, dim3(256), 0, st, (const float *)S, (const float *)T, rec, scale,
+                           divide, ll1, qcc, coff, lmax, mres, nm, per_field);
+    else
+        hipLaunchKernelGGL(k_vspec_prep<double>, grid, dim3(256), 0, st, (const double *)S, (const double *)T, rec,
+                           scale, divide, ll1, qcc, coff, lmax, mres, nm, per_field);
     return hipGetLastError();
 }
 

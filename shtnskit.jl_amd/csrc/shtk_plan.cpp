@@ -167,6 +167,8 @@ WsLayout ws_layout(const shtk_plan *p, int nf, int vector) {
     // Q-form coefficients of the scalar synthesis
     L.spec_bytes = al(sizeof(double2) * (size_t)nf * (p->d.lmax + 1) * (p->d.mmax + 1));
     L.total = (vector ? 2 : 1) * (L.f_bytes + L.spec_bytes);
+    // vector synthesis records: 8 doubles per coefficient (k_vspec_prep)
+    if (vector) L.total = std::max(L.total, 2 * L.f_bytes + 4 * L.spec_bytes);
     return L;
 }
 
@@ -501,6 +503,17 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
                 "coefficient scaling");
         a.in0 = dst;
         a.io_f32 = 0;
+    } else if (j.vector && j.prec != SHTK_PREC_SPLIT) {
+        // vector FP64 synthesis: records (S', T', c_l S', c_l T') with the convention
+        // map and the l(l+1) factor applied
+        double *dst = (double *)spec;
+        HIP_TRY(shtk::launch_vspec_prep(s0, s1, dst, a.io_f32, p->canonical ? nullptr : p->convscale, j.conv_div,
+                                        j.ll1_mode, p->qcc, p->coff, p->d.lmax, p->d.mres, p->nm, a.spec_fstride,
+                                        nf, st),
+                "coefficient scaling");
+        a.in0 = dst;
+        a.in1 = nullptr;
+        a.io_f32 = 0;
     } else if (!p->canonical) {
         const void *src[2] = {s0, s1};
         for (int q = 0; q <= j.vector; ++q) {
@@ -591,7 +604,7 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
                     "coefficient conversion");
         return SHTK_OK;
     }
-    const int pblk = j.vector ? 256 : 512;
+    const int pblk = j.vector ? 256 : 512;  // pairs per analysis work item (launch_leg_anal)
     a.atomic = p->np > pblk;
     if (a.atomic) {
         const size_t bytes = (size_t)nf * a.spec_fstride * (a.io_f32 ? 8 : 16);

@@ -824,6 +824,170 @@ __device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, con
     }
 }
 
+// Vector analysis, one wavefront per item (segment, pair block of 512,
+// field) with up to 8 chains per lane, so no coefficient has two writers.
+// Every 4 degrees the lanes' 32 partial sums (4 degrees x SD, SQ, TQ, TD x
+// re/im) go through a wave-private LDS tile; lane (o = lane & 31, h = lane >> 5)
+// adds 32 of them and one shuffle completes output o = 8 step + quantity; a
+// second shuffle pairs each quantity with the one S or T needs beside it.
+constexpr int AVW_WAVES = 4;
+constexpr int AVW_RS = 65;  // LDS row stride (doubles) of the reduction tile
+
+template <int K, bool FAST, typename OT>
+__device__ __forceinline__ void anal_vw_run(const LegArgs &a, const Item &it, double *red, OT *cs, OT *ct,
+                                            int lane) {
+    const int np = a.np, nlat = a.nlat;
+    const size_t fo = ((size_t)it.f * a.nm + it.im) * nlat;
+    double x[K], prev[K], cur[K];
+    double gt[K][4], gp[K][4];  // (+re, +im, -re, -im)
+    int kk[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int p = it.pstart + lane + 64 * j;
+        const bool act = p < it.pb1;
+        const int pc = act ? p : 0;
+        x[j] = a.x[pc];
+        const size_t q = (size_t)it.gs * np + pc;
+        prev[j] = act ? a.seedq0[q] : 0.0;
+        cur[j] = act ? a.seedq1[q] : 0.0;
+        kk[j] = act ? a.seedk[q] : 0;
+        double2 tN = make_double2(0.0, 0.0), tS = tN, pN = tN, pS = tN;
+        double wl = 0.0;
+        if (act) {
+            tN = a.F0[fo + p];
+            pN = a.F1[fo + p];
+            const int ps = nlat - 1 - p;
+            if (ps != p) {
+                tS = a.F0[fo + ps];
+                pS = a.F1[fo + ps];
+            }
+            wl = a.wlat ? a.wlat[p] : 1.0;
+        }
+        gt[j][0] = wl * (tN.x + tS.x);
+        gt[j][1] = wl * (tN.y + tS.y);
+        gt[j][2] = wl * (tN.x - tS.x);
+        gt[j][3] = wl * (tN.y - tS.y);
+        gp[j][0] = wl * (pN.x + pS.x);
+        gp[j][1] = wl * (pN.y + pS.y);
+        gp[j][2] = wl * (pN.x - pS.x);
+        gp[j][3] = wl * (pN.y - pS.y);
+    }
+    const double gm = a.g * mfac_of(a.mfac_mode, it.m);
+    const double mm = (double)it.m;
+    const int nst = it.lstop - it.l0;
+    const long long co = as_const(a.coff)[it.im] - it.m;
+    const cptr<double> qbm = as_const(a.qb) + co, qcm = as_const(a.qcc) + co;
+    const cptr<double> qdm = as_const((const double *)a.qd) + 2 * co;
+    const int o = lane & 31, hh = lane >> 5, kq = lane & 7, so = o >> 3;
+    for (int c0 = 0; c0 < nst; c0 += 4) {
+        const int lc = __builtin_amdgcn_readfirstlane(it.l0 + c0);
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            // beta_{l+1}, (l c_l, g_l c_{l-1}); rows past the segment are never stored
+            const double beta = qbm[lc + st + 1];
+            const int lq = 2 * min(lc + st, a.lmax + 1);
+            const double dl = qdm[lq], dg = qdm[lq + 1];
+            // even l - m: SD <- Gt-, SQ <- Gp+, TQ <- Gt+, TD <- Gp-; odd: the other signs
+            const int e = (st & 1) ? 0 : 2, od = (st & 1) ? 2 : 0;
+            double sdr = 0, sdi = 0, sqr = 0, sqi = 0, tqr = 0, tqi = 0, tdr = 0, tdi = 0;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const double xc = x[j] * cur[j];
+                double D = fma(dl, xc, -(dg * prev[j]));
+                double Q = cur[j];
+                if (!FAST) {
+                    const double cmj = kk[j] == 0 ? 1.0 : 0.0;
+                    D *= cmj;
+                    Q *= cmj;
+                }
+                sdr = fma(D, gt[j][e], sdr);
+                sdi = fma(D, gt[j][e + 1], sdi);
+                sqr = fma(Q, gp[j][od], sqr);
+                sqi = fma(Q, gp[j][od + 1], sqi);
+                tqr = fma(Q, gt[j][od], tqr);
+                tqi = fma(Q, gt[j][od + 1], tqi);
+                tdr = fma(D, gp[j][e], tdr);
+                tdi = fma(D, gp[j][e + 1], tdi);
+                const double nx = fma(-beta, prev[j], xc);
+                prev[j] = cur[j];
+                cur[j] = nx;
+            }
+            double *rw = red + (8 * st) * AVW_RS + lane;
+            rw[0 * AVW_RS] = sdr;
+            rw[1 * AVW_RS] = sdi;
+            rw[2 * AVW_RS] = sqr;
+            rw[3 * AVW_RS] = sqi;
+            rw[4 * AVW_RS] = tqr;
+            rw[5 * AVW_RS] = tqi;
+            rw[6 * AVW_RS] = tdr;
+            rw[7 * AVW_RS] = tdi;
+        }
+        if (!FAST && (c0 & 7) == 4) {
+            const double cl = qcm[lc + 3];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
+                    cur[j] *= kDown;
+                    prev[j] *= kDown;
+                    kk[j] -= 1;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        double sum = 0.0;
+#pragma unroll
+        for (int i = 0; i < 32; ++i) sum += red[o * AVW_RS + 32 * hh + i];
+        __builtin_amdgcn_wave_barrier();
+        sum += __shfl_xor(sum, 32);
+        // S = SD - i m c SQ, T = i m c TQ + TD: quantity k pairs with k ^ 3
+        const double other = __shfl_xor(sum, 3);
+        const int t = c0 + so;
+        if (hh == 0 && t < nst && (kq < 2 || kq >= 6)) {
+            const int l = it.l0 + t;
+            double sc = gm;
+            if (a.ll1_mode) sc *= l > 0 ? 1.0 / ((double)l * (l + 1)) : 0.0;
+            const double mc = mm * qcm[l];
+            const double sg = (kq == 0 || kq == 7) ? 1.0 : -1.0;
+            const double v = sc * fma(sg * mc, other, sum);
+            OT *dst = (kq < 2 ? cs : ct) + 2 * (size_t)l + (kq & 1);
+            if (a.atomic)
+                atomicAdd(dst, (OT)v);
+            else
+                *dst = (OT)v;
+        }
+    }
+}
+
+template <int K, typename OT>
+__device__ __forceinline__ void anal_vw_dispatch(const LegArgs &a, const Item &it, double *red, OT *cs, OT *ct,
+                                                 int lane) {
+    if (it.fast)
+        anal_vw_run<K, true>(a, it, red, cs, ct, lane);
+    else
+        anal_vw_run<K, false>(a, it, red, cs, ct, lane);
+}
+
+template <typename OT>
+__global__ __launch_bounds__(64 * AVW_WAVES) void k_anal_vw(const LegArgs a) {
+    __shared__ double red_all[AVW_WAVES][32 * AVW_RS];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int item = blockIdx.x * AVW_WAVES + w;
+    if (item >= a.gx * a.gy * a.gz) return;
+    const Item it = make_item_at(a, item, 512);
+    const size_t co = 2 * ((size_t)it.f * a.spec_fstride + (size_t)(a.lmax + 1) * it.m);
+    OT *cs = (OT *)a.out0 + co;
+    OT *ct = (OT *)a.out1 + co;
+    const bool dead = it.lstop <= it.l0 || it.pstart >= it.pb1;
+    zero_fill(a, it, cs, dead, lane);
+    zero_fill(a, it, ct, dead, lane);
+    if (dead) return;
+    double *red = red_all[w];
+    const int keff = (it.pb1 - it.pstart + 63) / 64;
+    if (keff <= 2) anal_vw_dispatch<2>(a, it, red, cs, ct, lane);
+    else if (keff <= 4) anal_vw_dispatch<4>(a, it, red, cs, ct, lane);
+    else anal_vw_dispatch<8>(a, it, red, cs, ct, lane);
+}
+
 template <typename OT>
 __global__ __launch_bounds__(ANV_THREADS) void k_anal_v(const LegArgs a, const double2 *__restrict__ qc,
                                                          const double2 *__restrict__ qd) {
@@ -948,17 +1112,18 @@ hipError_t launch_leg_synth(const LegArgs &a, int nfields, int vector, int chain
 
 hipError_t launch_leg_anal(const LegArgs &a0, int nfields, int vector, hipStream_t st) {
     if (a0.seg > SEG_MAX || (a0.seg & 7)) return hipErrorInvalidValue;
-    const int pblk = vector ? 256 : 512;  // pairs per work item
+    const int pblk = 512;  // pairs per work item
     LegArgs a = a0;
     a.gx = a.nseg_total;
     a.gy = (a.np + pblk - 1) / pblk;
     a.gz = nfields;
     const dim3 grid((unsigned)a.gx * a.gy * nfields);
     if (vector) {
+        const dim3 gw((unsigned)(((long long)a.gx * a.gy * nfields + AVW_WAVES - 1) / AVW_WAVES));
         if (a.io_f32)
-            hipLaunchKernelGGL(k_anal_v<float>, grid, dim3(ANV_THREADS), 0, st, a, a.qc, a.qd);
+            hipLaunchKernelGGL(k_anal_vw<float>, gw, dim3(64 * AVW_WAVES), 0, st, a);
         else
-            hipLaunchKernelGGL(k_anal_v<double>, grid, dim3(ANV_THREADS), 0, st, a, a.qc, a.qd);
+            hipLaunchKernelGGL(k_anal_vw<double>, gw, dim3(64 * AVW_WAVES), 0, st, a);
     } else {
         const dim3 gw((unsigned)(((long long)a.gx * a.gy * nfields + ANW_WAVES - 1) / ANW_WAVES));
         if (a.io_f32)

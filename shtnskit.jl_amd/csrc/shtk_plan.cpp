@@ -604,7 +604,7 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
                     "coefficient conversion");
         return SHTK_OK;
     }
-    const int pblk = j.vector ? 256 : 512;  // pairs per analysis work item (launch_leg_anal)
+    const int pblk = 512;  // pairs per analysis work item (launch_leg_anal)
     a.atomic = p->np > pblk;
     if (a.atomic) {
         const size_t bytes = (size_t)nf * a.spec_fstride * (a.io_f32 ? 8 : 16);

@@ -126,6 +126,21 @@ int shtk_synthesis_sphtor(shtk_plan *plan, const void *S, const void *T, void *V
 int shtk_analysis_sphtor(shtk_plan *plan, const void *Vt, const void *Vp, void *S, void *T, int nfields,
                          int dtype, int ltr, void *stream);
 
+/* Batched vector synthesis with a contraction precision: synthesis_sphtor_batch
+ * (src/batch_transforms.jl:745-815) and synthesis_sphtor (src/sphtor_transforms.jl:162-241).
+ * Same layouts and arguments as shtk_synthesis_sphtor.  SHTK_PREC_SPLIT runs
+ * sin(theta) V as a scalar split synthesis of degree lmax + 1 (the derivative
+ * rows written as l d_{l+1} P_{l+1} - (l+1) d_l P_{l-1}); needs lmax <= 4094. */
+int shtk_synthesis_sphtor_ex(shtk_plan *plan, const void *S, const void *T, void *Vt, void *Vp, int nfields,
+                             int dtype, int real_output, int ltr, int precision, void *stream);
+
+/* Batched vector analysis with a contraction precision: analysis_sphtor_batch
+ * (src/batch_transforms.jl:689-725) and analysis_sphtor (src/sphtor_transforms.jl:252-289).
+ * SHTK_PREC_SPLIT runs two scalar split analyses of degree lmax + 1 per field
+ * with the weights w/sin(theta) and combines them in FP64; needs lmax <= 4094. */
+int shtk_analysis_sphtor_ex(shtk_plan *plan, const void *Vt, const void *Vp, void *S, void *T, int nfields,
+                            int dtype, int ltr, int precision, void *stream);
+
 /* Adjoint of synthesis: real cotangent fbar (nlat, nlon, nfields) -> almbar.
  * _adjoint_synthesis, src/core_transforms.jl:477-519 (pullback of the
  * synthesis rrule, ext/SHTnsKitAdvancedADExt.jl:76-89, and of the batch rrule

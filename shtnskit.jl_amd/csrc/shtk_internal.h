@@ -68,6 +68,7 @@ struct LegArgs {
     double2 *F0, *F1;       // Fourier workspace [field][im][nlat]
     int gx, gy, gz;         // work items form a (gx, gy, gz = nfields) box, x fastest
     int xcd;                // synthesis: 1 keeps the pair blocks of an order on one XCD
+    int nm_run;             // synthesis: only orders im < nm_run (0: all)
 };
 
 // Work distribution over the 8 XCDs (each with its own 4 MB L2).  The
@@ -177,6 +178,8 @@ struct shtk_plan {
     int oz_ch_total = 0;
     void *oz_ws = nullptr;
     size_t oz_ws_bytes = 0;
+    // companion plan of degree lmax + 1 on the same nodes (split-precision vector transforms)
+    shtk_plan *vshadow = nullptr;
     std::mutex mu;
 };
 
@@ -286,5 +289,13 @@ hipError_t launch_vspec_prep(const void *S, const void *T, double *rec, int in_f
                              int ll1, const double *qcc, const long long *coff, int lmax, int mres, int nm,
                              long long per_field, int nfields, hipStream_t st);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
+// Split-precision vector transforms: (S, T) -> coefficients of 2 nf scalar
+// fields of degree lmax + 1, and the analysis sums of those back to (S, T).
+hipError_t launch_vsplit_coef(const void *S, const void *T, double *C, int in_f32, const double *scale, int divide,
+                              int lmax, int mres, int nm, int ltr, long long in_fstride, long long out_fstride,
+                              int nf, hipStream_t st);
+hipError_t launch_vsplit_out(const double *G, void *S, void *T, int out_f32, const double *scale, int divide,
+                             int lmax, int mmax, int mres, int ltr, long long g_fstride, long long out_fstride,
+                             int nf, hipStream_t st);
 
 }  // namespace shtk

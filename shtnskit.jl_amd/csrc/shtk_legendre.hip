@@ -1087,7 +1087,7 @@ hipError_t launch_seed(const SeedArgs &a, hipStream_t st) {
 template <int K>
 static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
     a.gx = (a.np + 64 * K - 1) / (64 * K);
-    a.gy = a.nm;
+    a.gy = a.nm_run > 0 ? min(a.nm_run, a.nm) : a.nm;
     a.gz = nfields;
     const dim3 grid(grid_for_units(a.xcd, a.gx, a.gy * a.gz));
     if (vector) {

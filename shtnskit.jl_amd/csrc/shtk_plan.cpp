@@ -20,6 +20,10 @@
 
 #include "shtk_internal.h"
 
+extern "C" {
+static int plan_tables(shtk_plan *p);  // defined with the entry points
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -125,6 +129,7 @@ void free_plan(shtk_plan *p) {
                     p->oz_asc, p->oz_chm, p->oz_run, p->qb, p->qcc};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    free_plan(p->vshadow);
     delete p;
 }
 
@@ -621,6 +626,132 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
                                             j.conv_div, p->d.lmax, p->d.mmax, p->d.mres, a.spec_fstride, nf, st),
                     "coefficient conversion");
     }
+    return SHTK_OK;
+}
+
+// Companion plan of degree lmax + 1 on the same nodes, canonical convention:
+// the split-precision vector transforms run as scalar split transforms on it.
+int ensure_vshadow(shtk_plan *p) {
+    if (p->vshadow) return SHTK_OK;
+    shtk_plan *s = new (std::nothrow) shtk_plan();
+    if (!s) return fail(SHTK_ENOMEM, "host allocation failed");
+    s->d = p->d;
+    s->d.lmax = p->d.lmax + 1;
+    s->d.norm = SHTK_NORM_ORTHONORMAL;
+    s->d.cs_phase = 1;
+    s->d.real_norm = 0;
+    s->d.robert_form = 0;
+    s->device = p->device;
+    s->np = p->np;
+    s->nm = p->nm;
+    s->seg = p->seg;
+    s->xcd_map = p->xcd_map;
+    s->synth_chains = p->synth_chains;
+    s->fft_tile = p->fft_tile;
+    s->oz_run_len = p->oz_run_len;
+    s->canonical = true;
+    s->inv_scale = p->inv_scale;
+    s->scale_phi = p->scale_phi;
+    s->hx = p->hx;
+    s->hw = p->hw;
+    const int rc = plan_tables(s);
+    if (rc) {
+        free_plan(s);
+        return rc;
+    }
+    p->vshadow = s;
+    return SHTK_OK;
+}
+
+// Split-precision vector transforms (launch_vsplit_coef / launch_vsplit_out):
+// sin(theta) V_theta and sin(theta) V_phi are scalar syntheses of degree
+// lmax + 1, and the analysis is two scalar analyses with the weights
+// w/sin(theta) followed by a three-term combination, so both directions run
+// the companion plan's split contraction on 2 nf fields whose Fourier data are
+// the contiguous [V_theta fields][V_phi fields] of the workspace.
+int run_vsplit_to_grid(shtk_plan *p, const Job &j, const void *S, const void *T, void *Vt, void *Vp, int nf,
+                       int dtype, void *stream) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const WsLayout L = ws_layout(p, nf, 1);
+    RC_TRY(ensure_ws(p, L.total));
+    RC_TRY(ensure_vshadow(p));
+    shtk_plan *s = p->vshadow;
+    RC_TRY(ensure_oz_tables(s, st));
+    RC_TRY(ensure_oz_ws(s, oz_ws_bytes(s, 2 * nf, false)));
+    char *ws = (char *)p->ws;
+    double2 *F0 = (double2 *)ws, *F1 = F0 + (size_t)nf * p->nm * p->d.nlat;
+    double *C = (double *)(ws + 2 * L.f_bytes);  // 2 nf (lmax+2)(mmax+1) complex <= the 4 spectral regions
+    const long long in_fs = (long long)(p->d.lmax + 1) * (p->d.mmax + 1);
+    const long long out_fs = (long long)(p->d.lmax + 2) * (p->d.mmax + 1);
+    HIP_TRY(shtk::launch_vsplit_coef(S, T, C, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale, j.conv_div,
+                                     p->d.lmax, p->d.mres, p->nm, j.ltr, in_fs, out_fs, nf, st),
+            "vector coefficient combination");
+    const shtk::OzArgs o = oz_synth_args(s, C, 0, 2 * nf, j.ltr + 1, j.g, F0);
+    HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_oz_synth(o, st); }), "split synthesis launch");
+    // Low orders: near the poles sin(theta) V is far smaller than its terms there
+    // (for m = 0 it vanishes like sin^2), so the division by sin(theta) would lift
+    // the split rounding of those rows above 1e-12; the FP64 kernel recomputes the
+    // orders m < max(16, (lmax+1)/32) (about 6 % of the work) over the split
+    // result, after the split prologues have consumed C.
+    const int m_lo = std::max(16, (p->d.lmax + 1) / 32);
+    const int nm_lo = std::min(p->nm, (m_lo + p->d.mres - 1) / p->d.mres);
+    HIP_TRY(shtk::launch_vspec_prep(S, T, C, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale, j.conv_div,
+                                    0, p->qcc, p->coff, p->d.lmax, p->d.mres, nm_lo, in_fs, nf, st),
+            "coefficient scaling");
+    shtk::LegArgs a = base_args(p, j.ltr);
+    a.in0 = C;
+    a.F0 = F0;
+    a.F1 = F1;
+    a.wlat = nullptr;  // sin(theta) V, as the split result
+    a.g = j.g;
+    a.nm_run = nm_lo;
+    HIP_TRY(timed(p, "leg_synth_sphtor_low", st, [&] { return shtk::launch_leg_synth(a, nf, 1, p->synth_chains, st); }),
+            "Legendre synthesis launch");
+    void *grids[2] = {Vt, Vp};
+    for (int q = 0; q < 2; ++q) {
+        shtk::FftArgs fa = fft_args(p);
+        fa.F = q == 0 ? F0 : F1;
+        fa.grid = grids[q];
+        fa.io_f32 = dtype == SHTK_F32;
+        fa.mode = j.fft_mode;
+        HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 1, st); }), "phi synthesis launch");
+    }
+    return SHTK_OK;
+}
+
+int run_vsplit_to_spec(shtk_plan *p, const Job &j, const void *Vt, const void *Vp, void *S, void *T, int nf,
+                       int dtype, void *stream) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const WsLayout L = ws_layout(p, nf, 1);
+    RC_TRY(ensure_ws(p, L.total));
+    RC_TRY(ensure_vshadow(p));
+    shtk_plan *s = p->vshadow;
+    RC_TRY(ensure_oz_tables(s, st));
+    RC_TRY(ensure_oz_ws(s, oz_ws_bytes(s, 2 * nf, false)));
+    char *ws = (char *)p->ws;
+    double2 *F0 = (double2 *)ws, *F1 = F0 + (size_t)nf * p->nm * p->d.nlat;
+    const void *grids[2] = {Vt, Vp};
+    for (int q = 0; q < 2; ++q) {
+        shtk::FftArgs fa = fft_args(p);
+        fa.F = q == 0 ? F0 : F1;
+        fa.grid = (void *)grids[q];
+        fa.io_f32 = dtype == SHTK_F32;
+        fa.mode = j.fft_mode;
+        HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 0, st); }), "phi analysis launch");
+    }
+    double *G = (double *)(ws + 2 * L.f_bytes);
+    const long long out_fs = (long long)(p->d.lmax + 1) * (p->d.mmax + 1);
+    const long long g_fs = (long long)(p->d.lmax + 2) * (p->d.mmax + 1);
+    HIP_TRY(shtk::launch_zero(G, (size_t)2 * nf * g_fs * 16, st), "analysis sums clear");
+    const shtk::OzArgs o = oz_anal_args(s, F0, G, 2 * nf, j.ltr + 1, j.wlat, j.g);
+    HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_oz_anal(o, st); }), "split analysis launch");
+    HIP_TRY(shtk::launch_vsplit_out(G, S, T, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale, j.conv_div,
+                                    p->d.lmax, p->d.mmax, p->d.mres, j.ltr, g_fs, out_fs, nf, st),
+            "vector coefficient combination");
     return SHTK_OK;
 }
 
@@ -1345,6 +1476,35 @@ int shtk_analysis_sphtor(shtk_plan *p, const void *Vt, const void *Vp, void *S, 
     RC_TRY(norm_ltr(p, ltr, &ltr));
     Job j{1, 1, p->d.robert_form ? p->winvs2 : p->winvs, 1.0, 0, 1, 0, ltr, "leg_anal_sphtor", "fft_anal"};
     return run_to_spec(p, j, Vt, Vp, S, T, nfields, dtype, stream);
+}
+
+int shtk_synthesis_sphtor_ex(shtk_plan *p, const void *S, const void *T, void *Vt, void *Vp, int nfields,
+                             int dtype, int real_output, int ltr, int precision, void *stream) {
+    if (precision == SHTK_PREC_FP64)
+        return shtk_synthesis_sphtor(p, S, T, Vt, Vp, nfields, dtype, real_output, ltr, stream);
+    if (precision != SHTK_PREC_SPLIT) return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
+    RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
+    if (real_output != 1)
+        return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is not implemented by this backend");
+    if (p->d.lmax > 4094) return fail(SHTK_EUNSUPPORTED, "split-precision vector transforms support lmax <= 4094");
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    // the contraction gives sin(theta) V; the phi FFT divides by sin(theta) unless Robert form
+    Job j{1, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, p->d.robert_form ? 0 : 1, ltr, "oz_synth_sphtor",
+          "fft_synth"};
+    j.prec = precision;
+    return run_vsplit_to_grid(p, j, S, T, Vt, Vp, nfields, dtype, stream);
+}
+
+int shtk_analysis_sphtor_ex(shtk_plan *p, const void *Vt, const void *Vp, void *S, void *T, int nfields, int dtype,
+                            int ltr, int precision, void *stream) {
+    if (precision == SHTK_PREC_FP64) return shtk_analysis_sphtor(p, Vt, Vp, S, T, nfields, dtype, ltr, stream);
+    if (precision != SHTK_PREC_SPLIT) return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
+    RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
+    if (p->d.lmax > 4094) return fail(SHTK_EUNSUPPORTED, "split-precision vector transforms support lmax <= 4094");
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    Job j{1, 1, p->d.robert_form ? p->winvs2 : p->winvs, 1.0, 0, 1, 0, ltr, "oz_anal_sphtor", "fft_anal"};
+    j.prec = precision;
+    return run_vsplit_to_spec(p, j, Vt, Vp, S, T, nfields, dtype, stream);
 }
 
 int shtk_adjoint_synthesis_sphtor(shtk_plan *p, const void *Vtb, const void *Vpb, void *Sb, void *Tb, int nfields,

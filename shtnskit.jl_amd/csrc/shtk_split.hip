@@ -871,6 +871,126 @@ hipError_t launch_oz_seed(const OzSeedArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
+namespace {
+
+// Vector (spheroidal/toroidal) transforms through the scalar split contraction.
+// With d_l = sqrt((l^2 - m^2)/(4 l^2 - 1)) the orthonormal rows satisfy
+//     sin(theta) dP_l/dtheta = l d_{l+1} P_{l+1} - (l+1) d_l P_{l-1},
+// so sin(theta) V_theta and sin(theta) V_phi of src/kernels.jl:155-169 are
+// same-order scalar syntheses to degree lmax + 1:
+//     sin V_theta = sum_l P_l [(l-1) d_l S_{l-1} - (l+2) d_{l+1} S_{l+1} - i m T_l]
+//     sin V_phi   = sum_l P_l [i m S_l + (l-1) d_l T_{l-1} - (l+2) d_{l+1} T_{l+1}]
+// and the analysis of src/kernels.jl:205-215 is two scalar analyses with the
+// weights w/sin(theta), G_theta and G_phi (degrees to lmax + 1), followed by
+//     S_l = [l d_{l+1} G_theta,l+1 - (l+1) d_l G_theta,l-1 - i m G_phi,l] / (l(l+1))
+//     T_l = [i m G_theta,l + l d_{l+1} G_phi,l+1 - (l+1) d_l G_phi,l-1] / (l(l+1)).
+__device__ __forceinline__ double vs_d(int l, int m) {
+    return l > m ? sqrt((double)(l * l - m * m) / (4.0 * l * l - 1.0)) : 0.0;
+}
+
+// Canonical coefficient l of one order, zero outside [lo, hi].
+template <class IT>
+__device__ __forceinline__ double2 vs_coef(const IT *A, const double *sm, int divide, int l, int lo, int hi) {
+    if (l < lo || l > hi) return make_double2(0.0, 0.0);
+    double c = 1.0;
+    if (sm) c = divide ? 1.0 / sm[l] : sm[l];
+    return make_double2(c * (double)A[2 * l], c * (double)A[2 * l + 1]);
+}
+
+// (S, T) of nf fields -> coefficients of 2 nf scalar fields of degree lmax + 1
+// ([theta fields][phi fields], column stride lmax + 2).  One block per
+// (order, field); S = T = 0 for l = 0 and l > ltr (src/kernels.jl:155-169 sums
+// l >= max(1, m)).
+template <class IT>
+__global__ __launch_bounds__(256) void k_vsplit_coef(const IT *S, const IT *T, double *C, const double *scale,
+                                                     int divide, int lmax, int mres, int nm, int ltr,
+                                                     long long in_fstride, long long out_fstride, int nf) {
+    const int im = blockIdx.x % nm, f = blockIdx.x / nm, m = im * mres;
+    const size_t col = (size_t)(lmax + 1) * m;
+    const IT *Sm = S + 2 * ((size_t)f * in_fstride + col);
+    const IT *Tm = T + 2 * ((size_t)f * in_fstride + col);
+    const double *sm = scale ? scale + col : nullptr;
+    double *Ct = C + 2 * ((size_t)f * out_fstride + (size_t)(lmax + 2) * m);
+    double *Cp = C + 2 * ((size_t)(nf + f) * out_fstride + (size_t)(lmax + 2) * m);
+    const int lo = max(1, m), hi = min(ltr, lmax);
+    const double mm = (double)m;
+    for (int l = m + (int)threadIdx.x; l <= lmax + 1; l += 256) {
+        const double dm = (l - 1) * vs_d(l, m), dp = (l + 2) * vs_d(l + 1, m);
+        const double2 s1 = vs_coef(Sm, sm, divide, l - 1, lo, hi), s0 = vs_coef(Sm, sm, divide, l, lo, hi);
+        const double2 s2 = vs_coef(Sm, sm, divide, l + 1, lo, hi);
+        const double2 t1 = vs_coef(Tm, sm, divide, l - 1, lo, hi), t0 = vs_coef(Tm, sm, divide, l, lo, hi);
+        const double2 t2 = vs_coef(Tm, sm, divide, l + 1, lo, hi);
+        *(double2 *)(Ct + 2 * l) = make_double2(dm * s1.x - dp * s2.x + mm * t0.y, dm * s1.y - dp * s2.y - mm * t0.x);
+        *(double2 *)(Cp + 2 * l) = make_double2(dm * t1.x - dp * t2.x - mm * s0.y, dm * t1.y - dp * t2.y + mm * s0.x);
+    }
+}
+
+// G of 2 nf scalar fields (degree lmax + 1) -> (S, T) in the configured
+// convention.  One block per (column, field); columns that are not orders and
+// rows l < max(1, m) or l > ltr are written as zero.
+template <class OT>
+__global__ __launch_bounds__(256) void k_vsplit_out(const double *G, OT *S, OT *T, const double *scale, int divide,
+                                                    int lmax, int mmax, int mres, int ltr, long long g_fstride,
+                                                    long long out_fstride, int nf) {
+    const int mc = blockIdx.x % (mmax + 1), f = blockIdx.x / (mmax + 1), m = mc;
+    const size_t col = (size_t)(lmax + 1) * mc;
+    OT *So = S + 2 * ((size_t)f * out_fstride + col);
+    OT *To = T + 2 * ((size_t)f * out_fstride + col);
+    const double2 *Gt = (const double2 *)G + (size_t)f * g_fstride + (size_t)(lmax + 2) * mc;
+    const double2 *Gp = (const double2 *)G + (size_t)(nf + f) * g_fstride + (size_t)(lmax + 2) * mc;
+    const bool order = mc % mres == 0;
+    const int lo = max(1, m), hi = min(ltr, lmax);
+    const double mm = (double)m;
+    for (int l = (int)threadIdx.x; l <= lmax; l += 256) {
+        double sr = 0.0, si = 0.0, tr = 0.0, ti = 0.0;
+        if (order && l >= lo && l <= hi) {
+            const double k = 1.0 / ((double)l * (l + 1));
+            const double dp = l * vs_d(l + 1, m), dm = (l + 1) * vs_d(l, m);
+            const double2 gt2 = Gt[l + 1], gt0 = Gt[l], gp2 = Gp[l + 1], gp0 = Gp[l];
+            const double2 gt1 = l - 1 >= m ? Gt[l - 1] : make_double2(0.0, 0.0);
+            const double2 gp1 = l - 1 >= m ? Gp[l - 1] : make_double2(0.0, 0.0);
+            double c = k;
+            if (scale) c = divide ? k / scale[col + l] : k * scale[col + l];
+            sr = c * (dp * gt2.x - dm * gt1.x + mm * gp0.y);
+            si = c * (dp * gt2.y - dm * gt1.y - mm * gp0.x);
+            tr = c * (dp * gp2.x - dm * gp1.x - mm * gt0.y);
+            ti = c * (dp * gp2.y - dm * gp1.y + mm * gt0.x);
+        }
+        So[2 * l] = (OT)sr;
+        So[2 * l + 1] = (OT)si;
+        To[2 * l] = (OT)tr;
+        To[2 * l + 1] = (OT)ti;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_vsplit_coef(const void *S, const void *T, double *C, int in_f32, const double *scale, int divide,
+                              int lmax, int mres, int nm, int ltr, long long in_fstride, long long out_fstride,
+                              int nf, hipStream_t st) {
+    const dim3 grid((unsigned)nm * nf);
+    if (in_f32)
+        hipLaunchKernelGGL(k_vsplit_coef<float>, grid, dim3(256), 0, st, (const float *)S, (const float *)T, C,
+                           scale, divide, lmax, mres, nm, ltr, in_fstride, out_fstride, nf);
+    else
+        hipLaunchKernelGGL(k_vsplit_coef<double>, grid, dim3(256), 0, st, (const double *)S, (const double *)T, C,
+                           scale, divide, lmax, mres, nm, ltr, in_fstride, out_fstride, nf);
+    return hipGetLastError();
+}
+
+hipError_t launch_vsplit_out(const double *G, void *S, void *T, int out_f32, const double *scale, int divide,
+                             int lmax, int mmax, int mres, int ltr, long long g_fstride, long long out_fstride,
+                             int nf, hipStream_t st) {
+    const dim3 grid((unsigned)(mmax + 1) * nf);
+    if (out_f32)
+        hipLaunchKernelGGL(k_vsplit_out<float>, grid, dim3(256), 0, st, G, (float *)S, (float *)T, scale, divide,
+                           lmax, mmax, mres, ltr, g_fstride, out_fstride, nf);
+    else
+        hipLaunchKernelGGL(k_vsplit_out<double>, grid, dim3(256), 0, st, G, (double *)S, (double *)T, scale,
+                           divide, lmax, mmax, mres, ltr, g_fstride, out_fstride, nf);
+    return hipGetLastError();
+}
+
 hipError_t launch_oz_synth(const OzArgs &a, hipStream_t st) {
     const dim3 g1((unsigned)a.nm * a.nfg), g2((unsigned)a.ch_total * a.nfg);
     if (a.in_f32) {

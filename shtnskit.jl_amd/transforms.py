@@ -239,6 +239,7 @@ def _synthesis_sphtor(args, kw, batch, op):
     real_output = kw.pop("real_output", True)
     kw.pop("use_rfft", None)
     ltr = kw.pop("ltr", -1)
+    prec = _precision(op, kw.pop("precision", "fp64"))
     if kw:
         raise TypeError(f"{op}: unexpected keywords {sorted(kw)}")
     S, rdt = _spec_in(cfg, S, adapt, op, batch)
@@ -254,8 +255,9 @@ def _synthesis_sphtor(args, kw, batch, op):
     Vt = _empty_colmajor(shape, rdt, S.device)
     Vp = _empty_colmajor(shape, rdt, S.device)
     lib = _lib.load()
-    _lib.check(lib.shtk_synthesis_sphtor(ctypes.c_void_p(cfg.plan), _ptr(S), _ptr(T), _ptr(Vt), _ptr(Vp), nf,
-                                         _code(rdt), 1 if real_output else 0, int(ltr), _stream(S.device)), op)
+    _lib.check(lib.shtk_synthesis_sphtor_ex(ctypes.c_void_p(cfg.plan), _ptr(S), _ptr(T), _ptr(Vt), _ptr(Vp), nf,
+                                            _code(rdt), 1 if real_output else 0, int(ltr), prec,
+                                            _stream(S.device)), op)
     return Vt, Vp
 
 
@@ -264,6 +266,7 @@ def _analysis_sphtor(args, kw, batch, op):
     cfg, Vt, Vp = args
     kw.pop("use_rfft", None)
     ltr = kw.pop("ltr", -1)
+    prec = _precision(op, kw.pop("precision", "fp64"))
     if kw:
         raise TypeError(f"{op}: unexpected keywords {sorted(kw)}")
     Vt, rdt = _grid_in(cfg, Vt, adapt, op, batch)
@@ -278,8 +281,8 @@ def _analysis_sphtor(args, kw, batch, op):
     S = _empty_colmajor(shape, _cplx(rdt), Vt.device)
     T = _empty_colmajor(shape, _cplx(rdt), Vt.device)
     lib = _lib.load()
-    _lib.check(lib.shtk_analysis_sphtor(ctypes.c_void_p(cfg.plan), _ptr(Vt), _ptr(Vp), _ptr(S), _ptr(T), nf,
-                                        _code(rdt), int(ltr), _stream(Vt.device)), op)
+    _lib.check(lib.shtk_analysis_sphtor_ex(ctypes.c_void_p(cfg.plan), _ptr(Vt), _ptr(Vp), _ptr(S), _ptr(T), nf,
+                                           _code(rdt), int(ltr), prec, _stream(Vt.device)), op)
     return S, T
 
 
@@ -294,10 +297,14 @@ def analysis_sphtor(*args, **kw):
 
 
 def synthesis_sphtor_batch(*args, **kw):
+    """``synthesis_sphtor_batch(cfg, Slm, Tlm (lmax+1, mmax+1, nf); real_output=True, precision="fp64")``
+    -> (Vt, Vp) each (nlat, nlon, nf).  ``precision="split"`` runs the contraction
+    on the int8 matrix cores (FP64-grade results)."""
     return _synthesis_sphtor(args, dict(kw), True, "synthesis_sphtor_batch")
 
 
 def analysis_sphtor_batch(*args, **kw):
+    """``analysis_sphtor_batch(cfg, Vt, Vp (nlat, nlon, nf); precision="fp64")`` -> (Slm, Tlm)."""
     return _analysis_sphtor(args, dict(kw), True, "analysis_sphtor_batch")
 
 

@@ -27,7 +27,11 @@ what the counters read for a plain device copy of known size (run with
 --calibrate-copy so that copy is in the same trace), and is read back from
 profiles/hbm_traffic.json when that file covers the kernel.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-split] [--calibrate-copy]
+Fourth line (`sphtor_batch`): 32 vector fields at lmax 511 (512 x 1024), S and
+T with l^-2 spectra, vector round trips in plain FP64 and in split precision.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-split] [--no-uneven]
+                       [--no-sphtor] [--calibrate-copy]
 """
 from __future__ import annotations
 
@@ -246,6 +250,57 @@ def measure_uneven(torch, np, shtk, dev, steps):
             "launches": "one hipGraph per direction; Legendre stages as persistent kernels over all groups"}
 
 
+def sphtor_spectrum(np, lmax, nfields, seed):
+    """Complex Gaussian (seed) scaled by l^-2 for l >= 1, zero for l = 0, l < m
+    entries and the m = 0 imaginary parts."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = rng.standard_normal((lmax + 1, lmax + 1, nfields)) + 1j * rng.standard_normal((lmax + 1, lmax + 1, nfields))
+    l = np.arange(lmax + 1)[:, None, None]
+    m = np.arange(lmax + 1)[None, :, None]
+    a = np.where((l >= m) & (l >= 1), a / np.maximum(l, 1.0) ** 2, 0)
+    a[:, 0, :] = a[:, 0, :].real
+    return a
+
+
+def measure_sphtor(torch, np, shtk, _lib, dev, steps, stream):
+    """Vector (sphtor) batch: 32 fields, lmax 511, 512 x 1024, S and T l^-2
+    spectra (seeds 4 and 5); synthesis_sphtor_batch then analysis_sphtor_batch
+    in plain FP64 and in split precision.  Value: vector field round trips per
+    second of the split batch."""
+    lmax, nlat, nlon, nf = 511, 512, 1024, 32
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+    S = to_device(torch, np, sphtor_spectrum(np, lmax, nf, 4), dev)
+    T = to_device(torch, np, sphtor_spectrum(np, lmax, nf, 5), dev)
+    torch.cuda.synchronize()
+    out = {}
+    for prec in ("fp64", "split"):
+        def vstep():
+            Vt, Vp = shtk.synthesis_sphtor_batch(cfg, S, T, precision=prec)
+            return shtk.analysis_sphtor_batch(cfg, Vt, Vp, precision=prec)
+
+        for _ in range(2):
+            S2, T2 = vstep()
+        torch.cuda.synchronize()
+        err = max(max(float(torch.linalg.vector_norm(S2[..., k] - S[..., k]) / torch.linalg.vector_norm(S[..., k])),
+                      float(torch.linalg.vector_norm(T2[..., k] - T[..., k]) / torch.linalg.vector_norm(T[..., k])))
+                  for k in range(nf))
+        no_ev, _, kern, _ = timed_steps(torch, _lib, cfg, vstep, steps, stream)
+        out[prec] = (no_ev / steps, err, {k: round(1000.0 * v[0] / v[1], 2) for k, v in kern.items()})
+    ms_split, err_split, k_split = out["split"]
+    ms_plain, err_plain, k_plain = out["fp64"]
+    cfg.close()
+    return {"value": round(1000.0 * nf / ms_split, 2), "unit": "vector field round trips/s", "fields": nf,
+            "steps": steps, "ms_per_batch_round_trip": round(ms_split, 4),
+            "fp64_path_value": round(1000.0 * nf / ms_plain, 2),
+            "fp64_path_ms_per_batch_round_trip": round(ms_plain, 4),
+            "speedup_split_vs_fp64": round(ms_plain / ms_split, 3),
+            "max_field_rel_l2_round_trip": {"split": err_split, "fp64": err_plain},
+            "config": {"workload": "vector (sphtor) batch round trip, synthesis_sphtor_batch+analysis_sphtor_batch",
+                       "lmax": lmax, "nlat": nlat, "nlon": nlon, "fields": nf,
+                       "data": "complex Gaussian S, T (seeds 4, 5), l^-2 spectra"},
+            "kernels_avg_us": {"split": k_split, "fp64": k_plain}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +309,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-split", action="store_true", help="skip the batched split-precision line")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-lmax batch line")
+    ap.add_argument("--no-sphtor", action="store_true", help="skip the vector (sphtor) batch line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 5 device copies of 256 MiB first (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -368,6 +424,9 @@ def main() -> int:
 
     if not args.no_split and not args.no_uneven:
         res["uneven_batch"] = measure_uneven(torch, np, shtk, dev, max(3, args.steps // 10))
+
+    if not args.no_sphtor:
+        res["sphtor_batch"] = measure_sphtor(torch, np, shtk, _lib, dev, max(4, args.steps // 5), stream)
 
     res["cpu_baseline"] = None if args.no_cpu else cpu_baseline(LMAX, NLAT, NLON)
     print(json.dumps(res), flush=True)

@@ -22,6 +22,7 @@ const LIB = get(ENV, "SHTK_MI355X_LIB", joinpath(@__DIR__, "..", "libshtk_mi355x
 # status codes of include/shtk.h
 const SHTK_OK, SHTK_EINVAL, SHTK_EUNSUPPORTED, SHTK_ENOMEM, SHTK_EHIP = 0, 1, 2, 3, 4
 const SHTK_F64, SHTK_F32 = 0, 1
+const SHTK_PREC_FP64, SHTK_PREC_SPLIT = 0, 1
 
 struct ShtkDesc          # layout of `shtk_desc`
     lmax::Cint; mmax::Cint; mres::Cint; nlat::Cint; nlon::Cint
@@ -69,6 +70,10 @@ end
 _dtype(::Type{Float64}) = SHTK_F64
 _dtype(::Type{Float32}) = SHTK_F32
 _stream() = AMDGPU.stream().stream       # the task's HIP stream (hipStream_t)
+# contraction precision of the batch transforms: :fp64 (plain) or :split (Ozaki
+# split on the int8 matrix cores, FP64-grade)
+_precision(p::Symbol) = p === :fp64 ? SHTK_PREC_FP64 : p === :split ? SHTK_PREC_SPLIT :
+    throw(ArgumentError("precision must be :fp64 or :split, got :$p"))
 
 function __init__()
     _register_gpu_adapter!(:mi355x, MI355X_ADAPTER)
@@ -83,7 +88,8 @@ _gpu_adapter_adapt(::MI355XAdapter, x::AbstractArray) = ROCArray(x)
 _gpu_adapter_dense(::MI355XAdapter, x::AbstractArray) = x isa ROCArray ? x : ROCArray(x)
 
 function _gpu_adapter_synthesis(::MI355XAdapter, cfg::SHTConfig, alm::AbstractArray;
-                                real_output::Bool=true, use_rfft::Bool=false, fft_scratch=nothing)
+                                real_output::Bool=true, use_rfft::Bool=false, fft_scratch=nothing,
+                                precision::Symbol=:fp64)
     fft_scratch === nothing || throw(ArgumentError("MI355X transforms do not accept a host fft_scratch"))
     real_output || throw(BackendUnavailableError(:synthesis, "complex-valued output is not implemented"))
     size(alm)[1:2] == (cfg.lmax + 1, cfg.mmax + 1) || throw(DimensionMismatch("alm size $(size(alm))"))
@@ -91,15 +97,15 @@ function _gpu_adapter_synthesis(::MI355XAdapter, cfg::SHTConfig, alm::AbstractAr
     nf = ndims(alm) == 3 ? size(alm, 3) : 1
     a = ROCArray{Complex{RT}}(alm)
     f = ndims(alm) == 3 ? AMDGPU.zeros(RT, cfg.nlat, cfg.nlon, nf) : AMDGPU.zeros(RT, cfg.nlat, cfg.nlon)
-    _check(:synthesis, ccall((:shtk_synthesis, LIB), Cint,
-        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Cvoid}),
-        _plan(cfg), pointer(a), pointer(f), nf, _dtype(RT), 1, cfg.lmax, _stream()))
+    _check(:synthesis, ccall((:shtk_synthesis_ex, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint, Ptr{Cvoid}),
+        _plan(cfg), pointer(a), pointer(f), nf, _dtype(RT), 1, cfg.lmax, _precision(precision), _stream()))
     AMDGPU.synchronize()                 # results ready on return, as the AMDGPU adapter
     return f
 end
 
 function _gpu_adapter_analysis(::MI355XAdapter, cfg::SHTConfig, f::AbstractArray;
-                               use_rfft::Bool=false, fft_scratch=nothing)
+                               use_rfft::Bool=false, fft_scratch=nothing, precision::Symbol=:fp64)
     fft_scratch === nothing || throw(ArgumentError("MI355X transforms do not accept a host fft_scratch"))
     size(f)[1:2] == (cfg.nlat, cfg.nlon) || throw(DimensionMismatch("field size $(size(f))"))
     RT = float(real(eltype(f)))
@@ -107,37 +113,40 @@ function _gpu_adapter_analysis(::MI355XAdapter, cfg::SHTConfig, f::AbstractArray
     x = ROCArray{RT}(f)
     alm = ndims(f) == 3 ? AMDGPU.zeros(Complex{RT}, cfg.lmax + 1, cfg.mmax + 1, nf) :
                           AMDGPU.zeros(Complex{RT}, cfg.lmax + 1, cfg.mmax + 1)
-    _check(:analysis, ccall((:shtk_analysis, LIB), Cint,
-        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}),
-        _plan(cfg), pointer(x), pointer(alm), nf, _dtype(RT), cfg.lmax, _stream()))
+    _check(:analysis, ccall((:shtk_analysis_ex, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Cvoid}),
+        _plan(cfg), pointer(x), pointer(alm), nf, _dtype(RT), cfg.lmax, _precision(precision), _stream()))
     AMDGPU.synchronize()
     return alm
 end
 
 function _gpu_adapter_synthesis_sphtor(::MI355XAdapter, cfg::SHTConfig, S, T;
-                                       real_output::Bool=true, use_rfft::Bool=false)
+                                       real_output::Bool=true, use_rfft::Bool=false, precision::Symbol=:fp64)
     real_output || throw(BackendUnavailableError(:synthesis_sphtor, "complex-valued output is not implemented"))
     RT = float(real(eltype(S)))
     nf = ndims(S) == 3 ? size(S, 3) : 1
     s = ROCArray{Complex{RT}}(S); t = ROCArray{Complex{RT}}(T)
     dims = ndims(S) == 3 ? (cfg.nlat, cfg.nlon, nf) : (cfg.nlat, cfg.nlon)
     Vt = AMDGPU.zeros(RT, dims...); Vp = AMDGPU.zeros(RT, dims...)
-    _check(:synthesis_sphtor, ccall((:shtk_synthesis_sphtor, LIB), Cint,
-        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Cvoid}),
-        _plan(cfg), pointer(s), pointer(t), pointer(Vt), pointer(Vp), nf, _dtype(RT), 1, cfg.lmax, _stream()))
+    _check(:synthesis_sphtor, ccall((:shtk_synthesis_sphtor_ex, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint, Ptr{Cvoid}),
+        _plan(cfg), pointer(s), pointer(t), pointer(Vt), pointer(Vp), nf, _dtype(RT), 1, cfg.lmax,
+        _precision(precision), _stream()))
     AMDGPU.synchronize()
     return Vt, Vp
 end
 
-function _gpu_adapter_analysis_sphtor(::MI355XAdapter, cfg::SHTConfig, Vt, Vp; use_rfft::Bool=false)
+function _gpu_adapter_analysis_sphtor(::MI355XAdapter, cfg::SHTConfig, Vt, Vp; use_rfft::Bool=false,
+                                      precision::Symbol=:fp64)
     RT = float(real(eltype(Vt)))
     nf = ndims(Vt) == 3 ? size(Vt, 3) : 1
     vt = ROCArray{RT}(Vt); vp = ROCArray{RT}(Vp)
     dims = ndims(Vt) == 3 ? (cfg.lmax + 1, cfg.mmax + 1, nf) : (cfg.lmax + 1, cfg.mmax + 1)
     S = AMDGPU.zeros(Complex{RT}, dims...); T = AMDGPU.zeros(Complex{RT}, dims...)
-    _check(:analysis_sphtor, ccall((:shtk_analysis_sphtor, LIB), Cint,
-        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Cvoid}),
-        _plan(cfg), pointer(vt), pointer(vp), pointer(S), pointer(T), nf, _dtype(RT), cfg.lmax, _stream()))
+    _check(:analysis_sphtor, ccall((:shtk_analysis_sphtor_ex, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Cvoid}),
+        _plan(cfg), pointer(vt), pointer(vp), pointer(S), pointer(T), nf, _dtype(RT), cfg.lmax,
+        _precision(precision), _stream()))
     AMDGPU.synchronize()
     return S, T
 end
@@ -154,7 +163,8 @@ end
 
 # Batch entry points have no protocol hook in core; the vendor extension adds
 # direct overloads (as ext/SHTnsKitAMDGPUExt.jl:1865-1994).  The C calls are the
-# same with nfields = size(x, 3).
+# same with nfields = size(x, 3); `precision = :split` selects the int8-matrix-core
+# contraction for many fields of one configuration.
 SHTnsKit.analysis_batch(cfg::SHTConfig, f::ROCArray{T,3}; kwargs...) where {T} =
     _gpu_adapter_analysis(MI355X_ADAPTER, cfg, f; kwargs...)
 SHTnsKit.synthesis_batch(cfg::SHTConfig, alm::ROCArray{T,3}; kwargs...) where {T} =

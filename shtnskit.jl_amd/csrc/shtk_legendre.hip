@@ -281,6 +281,192 @@ __global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Scalar synthesis with the per-degree operands in VGPRs (64-degree segments).
+// Lane 16 r + n of a wave holds the record (beta_{l+1}, c_l a_l) of degree
+// l0 + 16 g + n in record group g (the four 16-lane rows hold the same
+// records), and each use broadcasts lane n of the row with DPP row_newbcast:n
+// as the first operand of v_fmac_f64, a full-rate FP64 FMA: the degree loop has
+// no scalar loads and no waits.  The records of a wave's next segment are
+// loaded (vector loads) while the current one runs, like the seeds.
+// ---------------------------------------------------------------------------
+struct SRec {
+    double b;   // beta_{l+1}
+    double2 c;  // c_l a_l
+};
+
+// One degree for K chains: acc += (c_l a_l) u (u = Q_l, times 0 for chains that
+// are still scaled) and Q_{l+1} = x Q_l - beta_{l+1} Q_{l-1}.  NOP puts five
+// wait states before the first DPP read, for records just written by a VALU copy
+// or an EXEC write by the rescale.  The asm statements are volatile so that the
+// NOP stays in front of the others.
+template <int K, int PAR, int N, bool NOP, bool FAST>
+__device__ __forceinline__ void dstep(const SRec &r, const double *x, double *prev, double *cur, const double *cm,
+                                      double (*acc)[4]) {
+    double t[K], u[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        t[j] = x[j] * cur[j];
+        u[j] = FAST ? cur[j] : cur[j] * cm[j];
+    }
+    // the contraction FMAs of all chains first, so that each product x Q_l has
+    // 2K instructions to complete before its recurrence FMA
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (NOP && j == 0)
+            asm volatile(
+                "s_nop 4\n\t"
+                "v_fmac_f64_dpp %0, %2, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f64_dpp %1, %3, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf"
+                : "+v"(acc[j][2 * PAR]), "+v"(acc[j][2 * PAR + 1])
+                : "v"(r.c.x), "v"(r.c.y), "v"(u[j]), "i"(N));
+        else
+            asm volatile(
+                "v_fmac_f64_dpp %0, %2, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f64_dpp %1, %3, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf"
+                : "+v"(acc[j][2 * PAR]), "+v"(acc[j][2 * PAR + 1])
+                : "v"(r.c.x), "v"(r.c.y), "v"(u[j]), "i"(N));
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        asm volatile("v_fmac_f64_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                     : "+v"(t[j])
+                     : "v"(r.b), "v"(prev[j]), "i"(N));
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        prev[j] = cur[j];
+        cur[j] = t[j];
+    }
+}
+
+// Steps N = 0 .. 15 of one record group while l0g + N < lstop; (l - m) has
+// the parity of N.  The scaled chains are checked every 8 degrees with
+// c_l of the last degree (scalar load, unscaled blocks never take it).
+template <int K, bool FAST, int N>
+__device__ __forceinline__ bool dgroup_from(const SRec &r, cptr<double> qcm, int l0g, int nrem, const double *x,
+                                            double *prev, double *cur, int *kk, double *cm, double (*acc)[4]) {
+    if constexpr (N == 16) {
+        return true;
+    } else {
+        if (N >= nrem) return false;
+        dstep<K, N & 1, N, N == 0 || (!FAST && N == 8), FAST>(r, x, prev, cur, cm, acc);
+        if (!FAST && (N & 7) == 7) rescale<K>(prev, cur, kk, cm, qcm[l0g + N]);
+        return dgroup_from<K, FAST, N + 1>(r, qcm, l0g, nrem, x, prev, cur, kk, cm, acc);
+    }
+}
+
+template <int K, bool FAST>
+__device__ __forceinline__ void dsynth_seg(const SRec (&R)[4], cptr<double> qcm, int l0, int lstop, const double *x,
+                                           double *prev, double *cur, int *kk, double (*acc)[4]) {
+    double cm[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) cm[j] = kk[j] == 0 ? 1.0 : 0.0;
+    // a test per step (scalar, beside the VALU work) ends the segment at lstop;
+    // straight-line groups without it raise the VGPR count past 128
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int l0g = l0 + 16 * g;
+        if (!dgroup_from<K, FAST, 0>(R[g], qcm, l0g, lstop - l0g, x, prev, cur, kk, cm, acc)) return;
+    }
+}
+
+constexpr int DSEG = 64;  // segment length of k_synth_d
+
+template <int K>
+__global__ __launch_bounds__(256) void k_synth_d(const LegArgs a) {
+    __shared__ double red[4][4 * K][64];
+    int b, unit;
+    if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
+    const int im = unit % a.gy, f = unit / a.gy;
+    const int m = im * a.mres;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int np = a.np, lmax = a.lmax, ltr = a.ltr;
+    const int pb0 = b * 64 * K, pb1 = min(pb0 + 64 * K, np);
+    double x[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int p = pb0 + lane + 64 * j;
+        x[j] = p < np ? a.x[p] : 0.0;
+    }
+    const long long co = as_const(a.coff)[im] - m;
+    const cptr<double> qcm = as_const(a.qcc) + co;
+    const double *qbm = a.qb + co;  // padded: degrees to lmax + 64 are readable
+    // c_l a_l (k_spec_qscale), one order column; reads are clamped to lmax
+    const double2 *cal = (const double2 *)a.in0 + (size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m;
+    double acc[K][4];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0.0;
+    const int nseg = nseg_of(lmax, m, DSEG);
+    double nq0[K], nq1[K];
+    int nqk[K];
+    SRec nr[4];
+    auto load_next = [&](int s) {
+        const int sc = min(s, nseg - 1);
+        const int gs = a.segbase[im] + sc;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int p = pb0 + lane + 64 * j;
+            const bool act = p < np;
+            const size_t q = (size_t)gs * np + (act ? p : 0);
+            nq0[j] = act ? a.seedq0[q] : 0.0;
+            nq1[j] = act ? a.seedq1[q] : 0.0;
+            nqk[j] = act ? a.seedk[q] : 0;
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int l = m + sc * DSEG + 16 * g + (lane & 15);
+            nr[g].b = qbm[l + 1];
+            nr[g].c = cal[min(l, lmax)];
+        }
+    };
+    if (w < nseg) load_next(w);
+    for (int s = w; s < nseg; s += 4) {
+        const int l0 = m + s * DSEG;
+        if (l0 > ltr) break;
+        const int gs = a.segbase[im] + s;
+        double prev[K], cur[K];
+        int kk[K];
+        SRec R[4];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            prev[j] = nq0[j];
+            cur[j] = nq1[j];
+            kk[j] = nqk[j];
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) R[g] = nr[g];
+        if (s + 4 < nseg) load_next(s + 4);
+        if (pb1 <= a.pfirst[gs]) continue;
+        const bool fast = pb0 >= a.pfast[gs];
+        const int lstop = min(l0 + DSEG, ltr + 1);
+        if (fast) dsynth_seg<K, true>(R, qcm, l0, lstop, x, prev, cur, kk, acc);
+        else dsynth_seg<K, false>(R, qcm, l0, lstop, x, prev, cur, kk, acc);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[w][4 * j + q][lane] = acc[j][q];
+    __syncthreads();
+    if (w != 0) return;
+    const double gm = a.g * mfac_of(a.mfac_mode, m);
+    double2 *Fo = a.F0 + ((size_t)f * a.nm + im) * a.nlat;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int p = pb0 + lane + 64 * j;
+        if (p >= np) continue;
+        double t[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            t[q] = ((red[0][4 * j + q][lane] + red[1][4 * j + q][lane]) + red[2][4 * j + q][lane]) +
+                   red[3][4 * j + q][lane];
+        const double sc = gm * (a.wlat ? a.wlat[p] : 1.0);
+        Fo[p] = make_double2((t[0] + t[2]) * sc, (t[1] + t[3]) * sc);
+        const int ps = a.nlat - 1 - p;
+        if (ps != p) Fo[ps] = make_double2((t[0] - t[2]) * sc, (t[1] - t[3]) * sc);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Vector synthesis: s*Vt = sum_l [D_l S_l - i m P_l T_l], s*Vp = sum_l [i m P_l S_l + D_l T_l],
 // D_l = s dP_l/dtheta = l x P_l - g_l P_{l-1}; output times wlat (1/s, or 1 in
 // Robert form).  Parity: P(-x) = (-1)^(l-m) P(x), D(-x) = -(-1)^(l-m) D(x).
@@ -1094,7 +1280,9 @@ static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
         // inputs are the FP64 prologue records of k_vspec_prep
         hipLaunchKernelGGL((k_synth_v<K>), grid, dim3(256), 0, st, a);
     } else {
-        if (a.io_f32)
+        if (a.seg == DSEG)
+            hipLaunchKernelGGL((k_synth_d<K>), grid, dim3(256), 0, st, a);
+        else if (a.io_f32)
             hipLaunchKernelGGL((k_synth_s<K, float>), grid, dim3(256), 0, st, a);
         else
             hipLaunchKernelGGL((k_synth_s<K, double>), grid, dim3(256), 0, st, a);

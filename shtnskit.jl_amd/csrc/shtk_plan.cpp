@@ -1308,7 +1308,9 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     p->nm = d.mmax / d.mres + 1;
     p->seg = segment_length();
     p->xcd_map = env_int("SHTK_XCD", 0) != 0;
-    p->synth_chains = env_int("SHTK_SYNTH_CHAINS", 4);
+    // latitude pairs per lane of the scalar synthesis: 2 for the 64-degree VGPR-record
+    // kernel (4 waves per SIMD), 4 for the scalar-load kernel of other segment lengths
+    p->synth_chains = env_int("SHTK_SYNTH_CHAINS", p->seg == 64 ? 2 : 4);
     p->fft_tile = env_int("SHTK_FFT_PAIRS", 0);
     p->oz_run_len = std::max(1, env_int("SHTK_OZ_RUN", 8));
     p->canonical = d.norm == SHTK_NORM_ORTHONORMAL && !d.real_norm && d.cs_phase;

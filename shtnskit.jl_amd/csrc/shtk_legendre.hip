@@ -765,14 +765,18 @@ __device__ __forceinline__ void anal_w_run(const LegArgs &a, const Item &it, dou
     const double gm = a.g * mfac_of(a.mfac_mode, it.m);
     const int nst = it.lstop - it.l0;
     const long long co = as_const(a.coff)[it.im] - it.m;
-    const cptr<double> qbm = as_const(a.qb) + co;
     const cptr<double> qcm = as_const(a.qcc) + co;
     const int o = lane & 15, qg = lane >> 4;
+    // beta_{l+1} of the item's degrees, staged in the wave's LDS slice and read
+    // as broadcasts (no scalar loads, and so no scalar-load waits, in the loop)
+    double *bl = red + 16 * ANW_RS;
+    for (int i = lane; i < ((nst + 7) & ~7); i += 64) bl[i] = a.qb[co + it.l0 + i + 1];  // padded table
+    __builtin_amdgcn_wave_barrier();
     for (int c0 = 0; c0 < nst; c0 += 8) {
         const int lc = __builtin_amdgcn_readfirstlane(it.l0 + c0);
         double bq[8];
 #pragma unroll
-        for (int st = 0; st < 8; ++st) bq[st] = qbm[lc + st + 1];
+        for (int st = 0; st < 8; ++st) bq[st] = bl[c0 + st];
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
             double pr = 0.0, pi = 0.0;
@@ -830,7 +834,7 @@ __device__ __forceinline__ void anal_w_dispatch(const LegArgs &a, const Item &it
 
 template <typename OT>
 __global__ __launch_bounds__(64 * ANW_WAVES) void k_anal_w(const LegArgs a) {
-    __shared__ double red_all[ANW_WAVES][16 * ANW_RS];
+    __shared__ double red_all[ANW_WAVES][16 * ANW_RS + SEG_MAX + 8];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int item = blockIdx.x * ANW_WAVES + w;
     if (item >= a.gx * a.gy * a.gz) return;

@@ -669,26 +669,6 @@ struct Item {
     bool fast;
 };
 
-template <int PBLK>
-__device__ __forceinline__ Item make_item(const LegArgs &a) {
-    Item it;
-    const int item = blockIdx.x;
-    it.gs = item % a.gx;
-    it.pbi = (item / a.gx) % a.gy;
-    it.im = a.segm[it.gs];
-    it.s = it.gs - a.segbase[it.im];
-    it.m = it.im * a.mres;
-    it.l0 = it.m + it.s * a.seg;
-    it.lend = min(it.l0 + a.seg, a.lmax + 1);
-    it.lstop = min(it.lend, a.ltr + 1);
-    it.pb0 = it.pbi * PBLK;
-    it.pb1 = min(it.pb0 + PBLK, a.np);
-    it.pstart = max(it.pb0, a.pfirst[it.gs]);
-    it.fast = it.pstart >= a.pfast[it.gs];
-    it.f = item / (a.gx * a.gy);
-    return it;
-}
-
 // Zero the outputs of an item that no reduction writes (l beyond ltr, l < m,
 // orders between multiples of mres); skipped in accumulate mode (pre-zeroed).
 template <typename OT>
@@ -857,163 +837,6 @@ __global__ __launch_bounds__(64 * ANW_WAVES) void k_anal_w(const LegArgs a) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// Vector analysis: with G = wlat F (wlat = w scale / s),
-//   S_l = sum_i [D_l Gt - i m P_l Gp],  T_l = sum_i [i m P_l Gt + D_l Gp]
-// (times 1/(l(l+1)) for the analysis proper).  Folded per pair: even l-m uses
-// (Gt-, Gp+, Gt+, Gp-) for (SD, SQ, TQ, TD), odd l-m the opposite signs.
-// ---------------------------------------------------------------------------
-constexpr int ANV_THREADS = 128;
-
-template <int K, typename OT>
-__device__ __forceinline__ void anal_v_run(const LegArgs &a, const Item &it, const double2 *__restrict__ qcm,
-                                           const double2 *dseg, const double *cseg, double2 *tb, double2 *red,
-                                           OT *cs, OT *ct, int tid) {
-    constexpr int CH = 2;
-    const int np = a.np, nlat = a.nlat;
-    const size_t fo = ((size_t)it.f * a.nm + it.im) * nlat;
-    double x[K], prev[K], cur[K], cm[K];
-    double gt[K][4], gp[K][4];  // (+re, +im, -re, -im)
-    int kk[K];
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        const int p = it.pstart + tid + ANV_THREADS * j;
-        const bool act = p < it.pb1;
-        const int pc = act ? p : 0;
-        x[j] = a.x[pc];
-        const size_t q = (size_t)it.gs * np + pc;
-        prev[j] = act ? a.seedq0[q] : 0.0;
-        cur[j] = act ? a.seedq1[q] : 0.0;
-        kk[j] = act ? a.seedk[q] : 0;
-        cm[j] = (it.fast || kk[j] == 0) ? 1.0 : 0.0;
-        double2 tN = make_double2(0.0, 0.0), tS = tN, pN = tN, pS = tN;
-        double wl = 0.0;
-        if (act) {
-            tN = a.F0[fo + p];
-            pN = a.F1[fo + p];
-            const int ps = nlat - 1 - p;
-            if (ps != p) {
-                tS = a.F0[fo + ps];
-                pS = a.F1[fo + ps];
-            }
-            wl = a.wlat ? a.wlat[p] : 1.0;
-        }
-        gt[j][0] = wl * (tN.x + tS.x);
-        gt[j][1] = wl * (tN.y + tS.y);
-        gt[j][2] = wl * (tN.x - tS.x);
-        gt[j][3] = wl * (tN.y - tS.y);
-        gp[j][0] = wl * (pN.x + pS.x);
-        gp[j][1] = wl * (pN.y + pS.y);
-        gp[j][2] = wl * (pN.x - pS.x);
-        gp[j][3] = wl * (pN.y - pS.y);
-    }
-    const double gm = a.g * mfac_of(a.mfac_mode, it.m);
-    const double mm = (double)it.m;
-    const int nst = it.lstop - it.l0;
-    const int lmax = a.lmax;
-    // beta_{l+1} through scalar loads one chunk ahead; (l c_l, g_l c_{l-1}) and
-    // c_l from the segment tables in LDS
-    auto ldb = [&](int t0, double *bt) {
-#pragma unroll
-        for (int st = 0; st < CH; ++st) bt[st] = qcm[min(it.l0 + t0 + st, lmax) + 1].x;
-    };
-    double bcur[CH];
-    ldb(0, bcur);
-    for (int c0 = 0; c0 < nst; c0 += CH) {
-        double bnx[CH];
-        ldb(c0 + CH, bnx);
-#pragma unroll
-        for (int st = 0; st < CH; ++st) {
-            const int t = c0 + st;
-            if (t < nst) {
-                const double2 d = dseg[t];  // l c_l, g_l c_{l-1}
-                const double4 r = make_double4(bcur[st], d.x, d.y, 0.0);
-                // even: SD<-Gt-, SQ<-Gp+, TQ<-Gt+, TD<-Gp- ; odd: swap signs
-                const int e = (st & 1) ? 0 : 2, o = (st & 1) ? 2 : 0;
-                double sdr = 0, sdi = 0, sqr = 0, sqi = 0, tqr = 0, tqi = 0, tdr = 0, tdi = 0;
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    const double xc = x[j] * cur[j];
-                    const double D = fma(r.y, xc, -(r.z * prev[j])) * cm[j];
-                    const double Q = cur[j] * cm[j];
-                    sdr = fma(D, gt[j][e], sdr);
-                    sdi = fma(D, gt[j][e + 1], sdi);
-                    sqr = fma(Q, gp[j][o], sqr);
-                    sqi = fma(Q, gp[j][o + 1], sqi);
-                    tqr = fma(Q, gt[j][o], tqr);
-                    tqi = fma(Q, gt[j][o + 1], tqi);
-                    tdr = fma(D, gp[j][e], tdr);
-                    tdi = fma(D, gp[j][e + 1], tdi);
-                    const double nx = fma(-r.x, prev[j], xc);
-                    prev[j] = cur[j];
-                    cur[j] = nx;
-                }
-                tb[(st * 4 + 0) * ANV_THREADS + tid] = make_double2(sdr, sdi);
-                tb[(st * 4 + 1) * ANV_THREADS + tid] = make_double2(sqr, sqi);
-                tb[(st * 4 + 2) * ANV_THREADS + tid] = make_double2(tqr, tqi);
-                tb[(st * 4 + 3) * ANV_THREADS + tid] = make_double2(tdr, tdi);
-            }
-        }
-        if (!it.fast && ((c0 + CH) & (SCALE_CHECK - 1)) == 0) {
-            const double cl = cseg[c0 + min(CH, nst - c0) - 1];
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
-                    cur[j] *= kDown;
-                    prev[j] *= kDown;
-                    kk[j] -= 1;
-                    cm[j] = kk[j] == 0 ? 1.0 : 0.0;
-                }
-            }
-        }
-        __syncthreads();
-        {
-            // 8 (step, quantity) combos x ANV_THREADS partials: TPC threads per combo
-            constexpr int TPC = ANV_THREADS / 8;
-            const int combo = tid / TPC, q = tid % TPC;
-            double sr = 0.0, si = 0.0;
-#pragma unroll
-            for (int i = 0; i < ANV_THREADS / TPC; ++i) {
-                const double2 v = tb[combo * ANV_THREADS + q + TPC * i];
-                sr += v.x;
-                si += v.y;
-            }
-#pragma unroll
-            for (int msk = TPC / 2; msk >= 1; msk >>= 1) {
-                sr += __shfl_xor(sr, msk, TPC);
-                si += __shfl_xor(si, msk, TPC);
-            }
-            if (q == 0) red[combo] = make_double2(sr, si);
-        }
-        __syncthreads();
-        if (tid < CH) {
-            const int t = c0 + tid;
-            if (t < nst) {
-                const int l = it.l0 + t;
-                const double cl = cseg[t];
-                const double2 SD = red[tid * 4 + 0], SQ = red[tid * 4 + 1];
-                const double2 TQ = red[tid * 4 + 2], TD = red[tid * 4 + 3];
-                double sc = gm;
-                if (a.ll1_mode) sc *= l > 0 ? 1.0 / ((double)l * (l + 1)) : 0.0;
-                const double mc = mm * cl;
-                // S = SD - i m c SQ ; T = i m c TQ + TD
-                const double sre = sc * (SD.x + mc * SQ.y), sim = sc * (SD.y - mc * SQ.x);
-                const double tre = sc * (TD.x - mc * TQ.y), tim = sc * (TD.y + mc * TQ.x);
-                if (a.atomic) {
-                    add_c(cs, l, sre, sim);
-                    add_c(ct, l, tre, tim);
-                } else {
-                    st_c(cs, l, sre, sim);
-                    st_c(ct, l, tre, tim);
-                }
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int st = 0; st < CH; ++st) bcur[st] = bnx[st];
-    }
-}
-
 // Vector analysis, one wavefront per item (segment, pair block of 512,
 // field) with up to 8 chains per lane, so no coefficient has two writers.
 // Every 4 degrees the lanes' 32 partial sums (4 degrees x SD, SQ, TQ, TD x
@@ -1177,39 +1000,6 @@ __global__ __launch_bounds__(64 * AVW_WAVES) void k_anal_vw(const LegArgs a) {
     else if (keff <= 4) anal_vw_dispatch<4>(a, it, red, cs, ct, lane);
     else anal_vw_dispatch<8>(a, it, red, cs, ct, lane);
 }
-
-template <typename OT>
-__global__ __launch_bounds__(ANV_THREADS) void k_anal_v(const LegArgs a, const double2 *__restrict__ qc,
-                                                         const double2 *__restrict__ qd) {
-    __shared__ double2 tb[2 * 4 * ANV_THREADS];
-    __shared__ double2 red[8];
-    __shared__ double2 dseg[SEG_MAX + 8];
-    __shared__ double cseg[SEG_MAX + 8];
-    const Item it = make_item<256>(a);
-    const int tid = threadIdx.x;
-    const size_t co = 2 * ((size_t)it.f * a.spec_fstride + (size_t)(a.lmax + 1) * it.m);
-    OT *cs = (OT *)a.out0 + co;
-    OT *ct = (OT *)a.out1 + co;
-    const bool dead = it.lstop <= it.l0 || it.pstart >= it.pb1;
-    if (tid < 64) {
-        zero_fill(a, it, cs, dead, tid);
-        zero_fill(a, it, ct, dead, tid);
-    }
-    if (dead) return;
-    const double2 *__restrict__ qcm = qc + a.coff[it.im] - it.m;
-    const double2 *__restrict__ qdm = qd + a.coff[it.im] - it.m;
-    for (int t = tid; t < SEG_MAX + 8; t += ANV_THREADS) {
-        const int l = min(it.l0 + t, a.lmax);
-        dseg[t] = qdm[l];
-        cseg[t] = qcm[l].y;
-    }
-    __syncthreads();
-    const int keff = (it.pb1 - it.pstart + ANV_THREADS - 1) / ANV_THREADS;
-    if (keff == 1) anal_v_run<1>(a, it, qcm, dseg, cseg, tb, red, cs, ct, tid);
-    else anal_v_run<2>(a, it, qcm, dseg, cseg, tb, red, cs, ct, tid);
-}
-
-
 
 // ---------------------------------------------------------------------------
 // Spectral conversion: out = in * scale (or in / scale), zero for l < m and

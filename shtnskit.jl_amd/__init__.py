@@ -12,7 +12,9 @@ from .config import SHTConfig, create_gauss_config, create_gauss_fly_config, nlm
 from .layout import LM_index, pack_lm, unpack_lm
 from .transforms import (CPU, GPU, on_device, synthesis, analysis, synthesis_batch, analysis_batch,
                          synthesis_l, analysis_l, synthesis_sphtor, analysis_sphtor,
-                         synthesis_sphtor_batch, analysis_sphtor_batch, _adjoint_synthesis,
+                         synthesis_sphtor_batch, analysis_sphtor_batch, synthesis_sphtor_l, analysis_sphtor_l,
+                         synthesis_sph_l, synthesis_tor_l, synthesis_qst, analysis_qst, synthesis_qst_batch,
+                         analysis_qst_batch, synthesis_qst_l, analysis_qst_l, _adjoint_synthesis,
                          _adjoint_analysis, _adjoint_synthesis_sphtor, _adjoint_analysis_sphtor)
 from . import ad
 from .batch import UnevenBatch

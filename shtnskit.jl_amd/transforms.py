@@ -308,6 +308,94 @@ def analysis_sphtor_batch(*args, **kw):
     return _analysis_sphtor(args, dict(kw), True, "analysis_sphtor_batch")
 
 
+def synthesis_sphtor_l(cfg, S, T, ltr: int, **kw):
+    """Degree-truncated vector synthesis (``synthesis_sphtor_l``, src/sphtor_transforms.jl:956-1005):
+    coefficients above ``ltr`` are ignored."""
+    return _synthesis_sphtor((cfg, S, T), dict(kw, ltr=ltr), False, "synthesis_sphtor_l")
+
+
+def analysis_sphtor_l(cfg, Vt, Vp, ltr: int, **kw):
+    """Degree-truncated vector analysis (``analysis_sphtor_l``, src/sphtor_transforms.jl:1007-1035):
+    coefficients above ``ltr`` are zero."""
+    return _analysis_sphtor((cfg, Vt, Vp), dict(kw, ltr=ltr), False, "analysis_sphtor_l")
+
+
+def synthesis_sph_l(cfg, S, ltr: int, real_output: bool = True):
+    """Spheroidal part only (``synthesis_sph_l``, src/sphtor_transforms.jl:1049-1085): T = 0."""
+    return synthesis_sphtor_l(cfg, S, torch.zeros_like(S), ltr, real_output=real_output)
+
+
+def synthesis_tor_l(cfg, T, ltr: int, real_output: bool = True):
+    """Toroidal part only (``synthesis_tor_l``, src/sphtor_transforms.jl:1087-1123): S = 0."""
+    return synthesis_sphtor_l(cfg, torch.zeros_like(T), T, ltr, real_output=real_output)
+
+
+# --------------------------------------------------------------------------- QST
+def _qst_promoted(xs):
+    """The three components in one precision, the widest of the inputs (``_qst_promoted``
+    of src/qst_transforms.jl): FP32 only when every input is single precision."""
+    wide = any(isinstance(x, torch.Tensor) and x.dtype in (torch.float64, torch.complex128) or
+               not isinstance(x, torch.Tensor) and np.asarray(x).dtype in (np.float64, np.complex128)
+               for x in xs)
+    if wide:
+        return [x.to(torch.complex128 if x.is_complex() else torch.float64) if isinstance(x, torch.Tensor)
+                else np.asarray(x).astype(np.complex128 if np.iscomplexobj(x) else np.float64) for x in xs]
+    return list(xs)
+
+
+def _synthesis_qst(args, kw, batch, op):
+    adapt, args = _split_device(args)
+    cfg, Q, S, T = args
+    Q, S, T = _qst_promoted((Q, S, T))
+    dev = (GPU,) if adapt else ()
+    Vr = _synthesis(dev + (cfg, Q), dict(kw), batch, op)
+    Vt, Vp = _synthesis_sphtor(dev + (cfg, S, T), dict(kw), batch, op)
+    return Vr, Vt, Vp
+
+
+def _analysis_qst(args, kw, batch, op):
+    adapt, args = _split_device(args)
+    cfg, Vr, Vt, Vp = args
+    Vr, Vt, Vp = _qst_promoted((Vr, Vt, Vp))
+    dev = (GPU,) if adapt else ()
+    Q = _analysis(dev + (cfg, Vr), dict(kw), batch, op)
+    S, T = _analysis_sphtor(dev + (cfg, Vt, Vp), dict(kw), batch, op)
+    return Q, S, T
+
+
+def synthesis_qst(*args, **kw):
+    """``synthesis_qst(cfg, Qlm, Slm, Tlm; real_output=True)`` -> (Vr, Vt, Vp): the scalar
+    synthesis of Q and the sphtor synthesis of (S, T) on one grid, as
+    src/qst_transforms.jl:102-128 and its device form :206-217."""
+    return _synthesis_qst(args, dict(kw), False, "synthesis_qst")
+
+
+def analysis_qst(*args, **kw):
+    """``analysis_qst(cfg, Vr, Vt, Vp)`` -> (Qlm, Slm, Tlm) (src/qst_transforms.jl:137-160, :219-230)."""
+    return _analysis_qst(args, dict(kw), False, "analysis_qst")
+
+
+def synthesis_qst_batch(*args, **kw):
+    """Batched QST synthesis (src/batch_transforms.jl:830-985); ``precision="split"`` applies to
+    all three components."""
+    return _synthesis_qst(args, dict(kw), True, "synthesis_qst_batch")
+
+
+def analysis_qst_batch(*args, **kw):
+    """Batched QST analysis (src/batch_transforms.jl:830-985)."""
+    return _analysis_qst(args, dict(kw), True, "analysis_qst_batch")
+
+
+def synthesis_qst_l(cfg, Q, S, T, ltr: int, **kw):
+    """Degree-truncated QST synthesis (``synthesis_qst_l``, src/qst_transforms.jl:342-384)."""
+    return _synthesis_qst((cfg, Q, S, T), dict(kw, ltr=ltr), False, "synthesis_qst_l")
+
+
+def analysis_qst_l(cfg, Vr, Vt, Vp, ltr: int, **kw):
+    """Degree-truncated QST analysis (``analysis_qst_l``, src/qst_transforms.jl:308-340)."""
+    return _analysis_qst((cfg, Vr, Vt, Vp), dict(kw, ltr=ltr), False, "analysis_qst_l")
+
+
 # --------------------------------------------------------------------------- adjoints
 def _batchdim(t: torch.Tensor) -> bool:
     return t.dim() == 3

@@ -219,6 +219,33 @@ def test_fixture_analysis_sphtor(oracle, fid):
         assert FX.isapprox(FX.pack(fx, T[:, :, k]), Tq[:, k], fx["atol"], fx["rtol"])[0]
 
 
+def test_fixture_qst_full(oracle):
+    """QST = scalar Q plus sphtor (S, T) on one grid (src/qst_transforms.jl:102-160): the
+    oracle's scalar and vector restatements against the SHTns 3.7 QST fixture."""
+    fx = FX.fixture("qst_full")
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    Vr = cfg.synthesis(FX.unpack(fx, p["Q"]))
+    Vt, Vp = cfg.synthesis_sphtor(FX.unpack(fx, p["S"]), FX.unpack(fx, p["T"]))
+    for got, key in ((Vr, "Vr"), (Vt, "Vt"), (Vp, "Vp")):
+        assert FX.isapprox(got, p[key], fx["atol"], fx["rtol"])[0], key
+
+
+@pytest.mark.parametrize("fid", ["analysis_qst_full", "analysis_qst_l"])
+def test_fixture_analysis_qst(oracle, fid):
+    fx = FX.fixture(fid)
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    Q = cfg.analysis(p["Vr"], ltr=fx["ltr"])
+    S, T = cfg.analysis_sphtor(p["Vt"], p["Vp"], ltr=fx["ltr"])
+    for got, key in ((Q, "Q"), (S, "S"), (T, "T")):
+        ref = p[key]
+        if got.ndim == 2:
+            got, ref = got[:, :, None], ref[:, None]
+        for k in range(ref.shape[1]):
+            assert FX.isapprox(FX.pack(fx, got[:, :, k]), ref[:, k], fx["atol"], fx["rtol"])[0], key
+
+
 # --------------------------------------------------------------------------- identities
 def test_closed_form_synthesis(oracle):
     """Y_1^0 and Y_2^1 on the grid (test/parity/scalar_full.jl:48-80)."""

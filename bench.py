@@ -30,8 +30,11 @@ profiles/hbm_traffic.json when that file covers the kernel.
 Fourth line (`sphtor_batch`): 32 vector fields at lmax 511 (512 x 1024), S and
 T with l^-2 spectra, vector round trips in plain FP64 and in split precision.
 
+Fifth line (`adjoint_batch`): loss and gradient of 16 FP32 fields at lmax 511
+through the synthesis rrule (forward synthesis, weighted squared error, adjoint).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-split] [--no-uneven]
-                       [--no-sphtor] [--calibrate-copy]
+                       [--no-sphtor] [--no-adjoint] [--calibrate-copy]
 """
 from __future__ import annotations
 
@@ -301,6 +304,50 @@ def measure_sphtor(torch, np, shtk, _lib, dev, steps, stream):
             "kernels_avg_us": {"split": k_split, "fp64": k_plain}}
 
 
+def measure_adjoint(torch, np, shtk, _lib, dev, steps, stream):
+    """Loss and gradient through the synthesis rrule: 16 fields, lmax 511,
+    512 x 1024, FP32 coefficients (complex Gaussian, seed 6, (l+1)^-1 spectrum),
+    target t = synthesis(alm_t) (seed 7), L = sum_k sum_ij w_i (f_k - t_k)^2 with
+    f_k = synthesis(alm_k); a step is the forward synthesis, the loss and the
+    backward pass (the adjoint synthesis).  Value: fields per second."""
+    lmax, nlat, nlon, nf = 511, 512, 1024, 16
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+
+    def spec(seed):
+        rng = np.random.Generator(np.random.PCG64(seed))
+        a = rng.standard_normal((lmax + 1, lmax + 1, nf)) + 1j * rng.standard_normal((lmax + 1, lmax + 1, nf))
+        l = np.arange(lmax + 1)[:, None, None]
+        m = np.arange(lmax + 1)[None, :, None]
+        a = np.where(l >= m, a / (l + 1.0), 0)
+        a[:, 0, :] = a[:, 0, :].real
+        return to_device(torch, np, a.astype(np.complex64), dev)
+
+    a = spec(6).requires_grad_(True)
+    t = shtk.synthesis_batch(cfg, spec(7)).detach()
+    w = torch.as_tensor(cfg.w, dtype=torch.float32, device=dev)[:, None, None]
+    torch.cuda.synchronize()
+
+    def step():
+        a.grad = None
+        f = shtk.ad.synthesis(cfg, a)
+        loss = (w * (f - t) ** 2).sum()
+        loss.backward()
+        return loss
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    no_ev, _, kern, _ = timed_steps(torch, _lib, cfg, step, steps, stream)
+    ms = no_ev / steps
+    cfg.close()
+    return {"value": round(1000.0 * nf / ms, 2), "unit": "field loss+gradient/s", "fields": nf, "steps": steps,
+            "ms_per_batch": round(ms, 4), "dtype": "f32 I/O, FP64 Legendre and FFT",
+            "config": {"workload": "loss and gradient through the synthesis rrule (forward synthesis + adjoint)",
+                       "lmax": lmax, "nlat": nlat, "nlon": nlon, "fields": nf,
+                       "data": "complex Gaussian alm (seed 6) and target (seed 7), (l+1)^-1 spectra, FP32"},
+            "kernels_avg_us": {k: round(1000.0 * v[0] / v[1], 2) for k, v in kern.items()}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,6 +357,7 @@ def main() -> int:
     ap.add_argument("--no-split", action="store_true", help="skip the batched split-precision line")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-lmax batch line")
     ap.add_argument("--no-sphtor", action="store_true", help="skip the vector (sphtor) batch line")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the loss+gradient line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run 5 device copies of 256 MiB first (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -427,6 +475,9 @@ def main() -> int:
 
     if not args.no_sphtor:
         res["sphtor_batch"] = measure_sphtor(torch, np, shtk, _lib, dev, max(4, args.steps // 5), stream)
+
+    if not args.no_adjoint:
+        res["adjoint_batch"] = measure_adjoint(torch, np, shtk, _lib, dev, max(4, args.steps // 5), stream)
 
     res["cpu_baseline"] = None if args.no_cpu else cpu_baseline(LMAX, NLAT, NLON)
     print(json.dumps(res), flush=True)

@@ -6,13 +6,15 @@
 // DFT.  The inverse here applies no 1/N: the Legendre stage folds
 // phi_inv_scale/nlon into its output factor.
 //
-// Design: one workgroup per (field, latitude pair).  The two real rows of a
-// north/south pair travel as one complex row z = f_N + i f_S, so a single
-// complex FFT of length N serves both rows (no separate real-FFT pass).  The
-// row stays resident in LDS (16 N bytes) through a Stockham autosort FFT with
-// radix 8/4/2/3/5 passes; twiddles come from a per-plan table.  Consecutive
-// latitude pairs are mapped to the same XCD so the L2 merges the strided
-// longitude-major stores of neighbouring rows.
+// Design: persistent workgroups over tiles of 2G adjacent latitudes (the grid
+// is latitude-fastest, so a tile reads and writes 2G-latitude runs at every
+// longitude, and the order-major Fourier columns give 2G-latitude runs at every
+// order).  Two real rows travel as one complex row z = f_a + i f_b (latitudes
+// k and 2G-1-k of the tile), so a single complex FFT of length N serves both
+// rows (no separate real-FFT pass).  The rows stay resident in LDS (16 N bytes
+// each) through a Stockham autosort FFT with radix 8/4/2/3/5 passes; twiddles
+// come from a per-plan table.  Consecutive tiles are mapped to the same XCD so
+// the L2 merges the strided longitude-major stores of neighbouring rows.
 #include <algorithm>
 
 #include "shtk_internal.h"
@@ -167,14 +169,16 @@ __device__ void fft_lds(double2 *z, const FftArgs &a, int tid, int nt) {
 }
 
 // ---------------------------------------------------------------------------
-// Tiles of G adjacent latitude pairs (G = 4 for nlon <= 2560, less for longer
-// rows: the G complex rows must fit in LDS).  The grid and the Fourier
-// workspace both keep the latitude index fastest, so a tile's G north and G
-// south latitudes are two contiguous runs at every longitude / order: lanes
-// are grouped by 2G over one longitude / order (lane k < G: north latitude
-// p0 + k; lane k >= G: south latitude of pair p0 + 2G-1-k), which turns the
-// per-element accesses of a one-pair block into 8G-byte runs.  The partner of
-// lane k (the other half of its pair) is lane k ^ (2G-1).
+// Tiles of 2G adjacent latitudes, held as G packed complex rows (G = 4 for
+// nlon <= 2560, less for longer rows: the G complex rows must fit in LDS).  The
+// grid and the Fourier workspace both keep the latitude index fastest, so a
+// tile is one contiguous run of 2G latitudes at every longitude / order: lanes
+// are grouped by 2G over one longitude / order (lane k: latitude L0 + k), which
+// gives 16G-byte grid runs and 32G-byte Fourier runs (a tile of G north/south
+// pairs would give two runs of half that length).  Lane k and its partner
+// k ^ (2G-1) share packed row min(k, 2G-1-k); the split into the N/S halves of a
+// latitude pair happens in the Legendre kernels, which address both latitudes
+// of a pair directly.
 //
 // Work distribution: the (field, tile) units are dealt to the 8 XCDs in
 // contiguous ranges and every workgroup of an XCD sweeps that range with a
@@ -187,14 +191,16 @@ struct FftTile {
 
 __device__ __forceinline__ FftTile fft_tiles(const FftArgs &a, int G, int nfields) {
     FftTile t;
-    t.ntile = (a.np + G - 1) / G;
+    t.ntile = (a.nlat + 2 * G - 1) / (2 * G);
     t.total = t.ntile * nfields;
     t.per = (t.total + 7) / 8;
     t.nslot = gridDim.x >> 3;
     return t;
 }
 
-__device__ __forceinline__ double fft_row_scale(const FftArgs &a, int p) {
+// Row factor of latitude lat (pair index p = min(lat, nlat - 1 - lat)).
+__device__ __forceinline__ double fft_row_scale(const FftArgs &a, int lat) {
+    const int p = min(lat, a.nlat - 1 - lat);
     if (a.mode == 1) {
         const double st = a.s[p];
         return st > 0.0 ? 1.0 / st : 1.0;
@@ -203,8 +209,9 @@ __device__ __forceinline__ double fft_row_scale(const FftArgs &a, int p) {
 }
 
 // Synthesis: Fourier data -> grid rows.  Row g of the tile holds the packed
-// pair z = f_N + i f_S of pair p0 + g at positions zp(j) (row stride RS = N + N/16 + 1
-// complex: the rows of one longitude fall in different LDS banks).
+// rows z = f_a + i f_b of latitudes L0 + g and L0 + 2G - 1 - g at positions zp(j)
+// (row stride RS = N + N/16 + 1 complex: the rows of one longitude fall in
+// different LDS banks).
 template <int G, typename GT>
 __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfields) {
     extern __shared__ double2 zs[];
@@ -213,16 +220,14 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
     const int tid = threadIdx.x, nt = blockDim.x, tr = nt / G;
     const int row = tid / tr, lt = tid % tr;
     const int k = tid % L;                // lane role within its group of 2G
-    const int gk = k < G ? k : L - 1 - k;  // pair of this lane within the tile
+    const int gk = k < G ? k : L - 1 - k;  // packed row of this lane within the tile
     const FftTile T = fft_tiles(a, G, nfields);
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
     for (int u = slot; u < T.per; u += T.nslot) {
         const int ug = xcd * T.per + u;
         if (ug >= T.total) break;
-        const int f = ug / T.ntile, p0 = (ug % T.ntile) * G;
-        const int p = p0 + gk;
-        const int lat = k < G ? p0 + k : nlat - L - p0 + k;
-        const bool live = p < a.np && (k < G || nlat - 1 - p != p);
+        const int f = ug / T.ntile, lat = (ug % T.ntile) * L + k;
+        const bool live = lat < nlat;
         for (int q = tid; q < G * RS; q += nt) zs[q] = make_double2(0.0, 0.0);
         __syncthreads();
         // packed rows: lane k < G writes z[m], its partner z[N - m]
@@ -241,15 +246,15 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
                 const int im = (e0 + tid + q * nt) / L;
                 const double2 mine = ld[q];
                 const double2 other = make_double2(__shfl_xor(mine.x, L - 1), __shfl_xor(mine.y, L - 1));
-                const double2 FN = k < G ? mine : other, FS = k < G ? other : mine;
+                const double2 Fa = k < G ? mine : other, Fb = k < G ? other : mine;
                 const int m = im * a.mres;
                 if (im < a.nm) {
                     if (m == 0) {
-                        if (k < G) z[0] = make_double2(FN.x, FS.x);
+                        if (k < G) z[0] = make_double2(Fa.x, Fb.x);
                     } else if (k < G) {
-                        z[zp(m)] = make_double2(FN.x - FS.y, FN.y + FS.x);
+                        z[zp(m)] = make_double2(Fa.x - Fb.y, Fa.y + Fb.x);
                     } else {
-                        z[zp(N - m)] = make_double2(FN.x + FS.y, FS.x - FN.y);
+                        z[zp(N - m)] = make_double2(Fa.x + Fb.y, Fb.x - Fa.y);
                     }
                 }
             }
@@ -258,7 +263,7 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
         fft_lds<+1>(zs + row * RS, a, lt, tr);
         GT *gf = (GT *)a.grid + (size_t)f * nlat * N + lat;
         if (live) {
-            const double sc = fft_row_scale(a, p);
+            const double sc = fft_row_scale(a, lat);
             for (int e = tid; e < N * L; e += nt) {
                 const int j = e / L;
                 const double2 v = z[zp(j)];
@@ -269,8 +274,8 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
     }
 }
 
-// Analysis: grid rows -> Fourier data, FN = (Z_m + conj Z_{N-m}) / 2,
-// FS = (Z_m - conj Z_{N-m}) / (2i) of the packed row Z.
+// Analysis: grid rows -> Fourier data, F_a = (Z_m + conj Z_{N-m}) / 2,
+// F_b = (Z_m - conj Z_{N-m}) / (2i) of the packed row Z = f_a + i f_b.
 template <int G, typename GT>
 __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfields) {
     extern __shared__ double2 zs[];
@@ -285,11 +290,9 @@ __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfield
     for (int u = slot; u < T.per; u += T.nslot) {
         const int ug = xcd * T.per + u;
         if (ug >= T.total) break;
-        const int f = ug / T.ntile, p0 = (ug % T.ntile) * G;
-        const int p = p0 + gk;
-        const int lat = k < G ? p0 + k : nlat - L - p0 + k;
-        const bool live = p < a.np && (k < G || nlat - 1 - p != p);
-        const double sc = p < a.np ? fft_row_scale(a, p) : 0.0;
+        const int f = ug / T.ntile, lat = (ug % T.ntile) * L + k;
+        const bool live = lat < nlat;
+        const double sc = live ? fft_row_scale(a, lat) : 0.0;
         const GT *gf = (const GT *)a.grid + (size_t)f * nlat * N + lat;
         double2 *z = zs + gk * RS;
         constexpr int PF = 8;  // loads in flight per thread
@@ -338,7 +341,7 @@ hipError_t launch_fft_g(const FftArgs &a, int nfields, int inverse, int tr, hipS
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    const int ntile = (a.np + G - 1) / G;
+    const int ntile = (a.nlat + 2 * G - 1) / (2 * G);
     const int total = ntile * nfields;
     // persistent grid: as many workgroups as fit the chip (LDS-limited), a multiple of 8
     const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
@@ -366,7 +369,7 @@ hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st
     const int tr = (std::max(64, (N + FFT_VPT - 1) / FFT_VPT) + 63) / 64 * 64;
     const size_t row = sizeof(double2) * (size_t)(N + (N >> 4) + 1);
     // widest tile that fits LDS and still gives every CU work (a.tile_pairs > 0 forces it)
-    const long long units4 = (long long)((a.np + 3) / 4) * nfields, units2 = (long long)((a.np + 1) / 2) * nfields;
+    const long long units4 = (long long)((a.nlat + 7) / 8) * nfields, units2 = (long long)((a.nlat + 3) / 4) * nfields;
     int G = a.tile_pairs;
     if (G <= 0) G = units4 >= 512 ? 4 : units2 >= 256 ? 2 : 1;
     if (G >= 4 && 4 * row <= 160 * 1024 && 4 * tr <= 1024) return launch_fft_g<4>(a, nfields, inverse, tr, st);

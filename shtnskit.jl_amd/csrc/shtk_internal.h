@@ -120,7 +120,7 @@ struct FftArgs {
     int io_f32;
     int mode;          // grid row factor: 0 none, 1 1/sin(theta), 2 sin(theta) (Robert form)
     const double *s;   // [np] sin(theta) for Robert scaling
-    int tile_pairs;    // latitude pairs per workgroup (0: chosen from the sizes)
+    int tile_pairs;    // packed rows (latitude pairs' worth) per tile (0: chosen from the sizes)
 };
 
 }  // namespace shtk

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
     double nq0[K], nq1[K];
     int nqk[K];
     auto load_seeds = [&](int s) {
-        const int gs = a.segbase[im] + min(s, nseg - 1);
+        const int gs = as_const(a.segbase)[im] + min(s, nseg - 1);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const int p = pb0 + lane + 64 * j;
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
     for (int s = w; s < nseg; s += 4) {
         const int l0 = m + s * S;
         if (l0 > ltr) break;
-        const int gs = a.segbase[im] + s;
+        const int gs = as_const(a.segbase)[im] + s;
         double prev[K], cur[K];
         int kk[K];
 #pragma unroll
@@ -250,8 +250,8 @@ __global__ __launch_bounds__(256) void k_synth_s(const LegArgs a) {
             kk[j] = nqk[j];
         }
         if (s + 4 < nseg) load_seeds(s + 4);
-        if (pb1 <= a.pfirst[gs]) continue;
-        const bool fast = pb0 >= a.pfast[gs];
+        if (pb1 <= as_const(a.pfirst)[gs]) continue;
+        const bool fast = pb0 >= as_const(a.pfast)[gs];
         const int lstop = min(l0 + S, ltr + 1);
         if (fast) synth_seg<K, true>(qbm, qcm, al, l0, lstop, lmax, x, prev, cur, kk, acc);
         else synth_seg<K, false>(qbm, qcm, al, l0, lstop, lmax, x, prev, cur, kk, acc);
@@ -370,10 +370,13 @@ __device__ __forceinline__ void dsynth_seg(const SRec (&R)[4], cptr<double> qcm,
 }
 
 constexpr int DSEG = 64;  // segment length of k_synth_d
+// waves per k_synth_d workgroup, taking the segments round-robin (2 waves time the
+// same; 1 wave per workgroup leaves a long tail of low orders: 82 against 65 us)
+constexpr int DSW = 4;
 
 template <int K>
-__global__ __launch_bounds__(256) void k_synth_d(const LegArgs a) {
-    __shared__ double red[4][4 * K][64];
+__global__ __launch_bounds__(64 * DSW) void k_synth_d(const LegArgs a) {
+    __shared__ double red[DSW][4 * K][64];
     int b, unit;
     if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
     const int im = unit % a.gy, f = unit / a.gy;
@@ -397,12 +400,18 @@ __global__ __launch_bounds__(256) void k_synth_d(const LegArgs a) {
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0.0;
     const int nseg = nseg_of(lmax, m, DSEG);
+    // the next segment's seeds, records and pair bounds are loaded while the
+    // current one runs (the bounds through scalar loads: the degree loop issues
+    // none, so their wait at the next segment start finds them in)
+    const int segb = as_const(a.segbase)[im];
     double nq0[K], nq1[K];
-    int nqk[K];
+    int nqk[K], npf = 0, npa = 0;
     SRec nr[4];
     auto load_next = [&](int s) {
         const int sc = min(s, nseg - 1);
-        const int gs = a.segbase[im] + sc;
+        const int gs = segb + sc;
+        npf = as_const(a.pfirst)[gs];
+        npa = as_const(a.pfast)[gs];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const int p = pb0 + lane + 64 * j;
@@ -420,10 +429,10 @@ __global__ __launch_bounds__(256) void k_synth_d(const LegArgs a) {
         }
     };
     if (w < nseg) load_next(w);
-    for (int s = w; s < nseg; s += 4) {
+    for (int s = w; s < nseg; s += DSW) {
         const int l0 = m + s * DSEG;
         if (l0 > ltr) break;
-        const int gs = a.segbase[im] + s;
+        const int pf = npf, pfa = npa;
         double prev[K], cur[K];
         int kk[K];
         SRec R[4];
@@ -435,9 +444,9 @@ __global__ __launch_bounds__(256) void k_synth_d(const LegArgs a) {
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) R[g] = nr[g];
-        if (s + 4 < nseg) load_next(s + 4);
-        if (pb1 <= a.pfirst[gs]) continue;
-        const bool fast = pb0 >= a.pfast[gs];
+        if (s + DSW < nseg) load_next(s + DSW);
+        if (pb1 <= pf) continue;
+        const bool fast = pb0 >= pfa;
         const int lstop = min(l0 + DSEG, ltr + 1);
         if (fast) dsynth_seg<K, true>(R, qcm, l0, lstop, x, prev, cur, kk, acc);
         else dsynth_seg<K, false>(R, qcm, l0, lstop, x, prev, cur, kk, acc);
@@ -456,9 +465,12 @@ __global__ __launch_bounds__(256) void k_synth_d(const LegArgs a) {
         if (p >= np) continue;
         double t[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            t[q] = ((red[0][4 * j + q][lane] + red[1][4 * j + q][lane]) + red[2][4 * j + q][lane]) +
-                   red[3][4 * j + q][lane];
+        for (int q = 0; q < 4; ++q) {
+            double v = red[0][4 * j + q][lane];
+#pragma unroll
+            for (int ww = 1; ww < DSW; ++ww) v += red[ww][4 * j + q][lane];
+            t[q] = v;
+        }
         const double sc = gm * (a.wlat ? a.wlat[p] : 1.0);
         Fo[p] = make_double2((t[0] + t[2]) * sc, (t[1] + t[3]) * sc);
         const int ps = a.nlat - 1 - p;
@@ -575,9 +587,9 @@ __global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
     for (int s = w; s < nseg; s += 4) {
         const int l0 = m + s * S;
         if (l0 > ltr) break;
-        const int gs = a.segbase[im] + s;
-        if (pb1 <= a.pfirst[gs]) continue;
-        const bool fast = pb0 >= a.pfast[gs];
+        const int gs = as_const(a.segbase)[im] + s;
+        if (pb1 <= as_const(a.pfirst)[gs]) continue;
+        const bool fast = pb0 >= as_const(a.pfast)[gs];
         const int lstop = min(l0 + S, ltr + 1);
         double prev[K], cur[K];
         int kk[K];
@@ -699,16 +711,16 @@ __device__ __forceinline__ Item make_item_at(const LegArgs &a, int item, int pbl
     Item it;
     it.gs = item % a.gx;
     it.pbi = (item / a.gx) % a.gy;
-    it.im = a.segm[it.gs];
-    it.s = it.gs - a.segbase[it.im];
+    it.im = as_const(a.segm)[it.gs];
+    it.s = it.gs - as_const(a.segbase)[it.im];
     it.m = it.im * a.mres;
     it.l0 = it.m + it.s * a.seg;
     it.lend = min(it.l0 + a.seg, a.lmax + 1);
     it.lstop = min(it.lend, a.ltr + 1);
     it.pb0 = it.pbi * pblk;
     it.pb1 = min(it.pb0 + pblk, a.np);
-    it.pstart = max(it.pb0, a.pfirst[it.gs]);
-    it.fast = it.pstart >= a.pfast[it.gs];
+    it.pstart = max(it.pb0, as_const(a.pfirst)[it.gs]);
+    it.fast = it.pstart >= as_const(a.pfast)[it.gs];
     it.f = item / (a.gx * a.gy);
     return it;
 }
@@ -1075,7 +1087,7 @@ static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
         hipLaunchKernelGGL((k_synth_v<K>), grid, dim3(256), 0, st, a);
     } else {
         if (a.seg == DSEG)
-            hipLaunchKernelGGL((k_synth_d<K>), grid, dim3(256), 0, st, a);
+            hipLaunchKernelGGL((k_synth_d<K>), grid, dim3(64 * DSW), 0, st, a);
         else if (a.io_f32)
             hipLaunchKernelGGL((k_synth_s<K, float>), grid, dim3(256), 0, st, a);
         else

@@ -19,6 +19,8 @@
 //     with exact int32 accumulation, and the 6 order levels combine in FP64.
 // Worst per-order relative L2 error of this scheme at lmax = 1023 (numpy
 // emulation, l^-1.5 spectra): 7.8e-13.
+#include <type_traits>
+
 #include "shtk_internal.h"
 
 namespace shtk {
@@ -134,10 +136,24 @@ __device__ __forceinline__ void oz_bexp_body(const OzArgs &a, int im, int fg) {
         const int fl = 4 * w + k, f = fg * OZ_FIELDS + fl;
         double mr = 0.0, mi = 0.0;
         if (f < a.nf) {
-            const IT *src = (const IT *)a.spec_in + 2 * ((size_t)f * a.spec_fstride + (size_t)(a.lmax + 1) * m);
-            for (int l = m + lane; l <= lend; l += 64) {
-                mr = fmax(mr, fabs((double)src[2 * l]));
-                mi = fmax(mi, fabs((double)src[2 * l + 1]));
+            // one (re, im) vector load per coefficient, four in flight per lane
+            using C2 = typename std::conditional<sizeof(IT) == 8, double2, float2>::type;
+            const C2 *src = (const C2 *)a.spec_in + (size_t)f * a.spec_fstride + (size_t)(a.lmax + 1) * m;
+            int l = m + lane;
+            for (; l + 192 <= lend; l += 256) {
+                C2 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = src[l + 64 * u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    mr = fmax(mr, fabs((double)v[u].x));
+                    mi = fmax(mi, fabs((double)v[u].y));
+                }
+            }
+            for (; l <= lend; l += 64) {
+                const C2 v = src[l];
+                mr = fmax(mr, fabs((double)v.x));
+                mi = fmax(mi, fabs((double)v.y));
             }
         }
 #pragma unroll
@@ -178,8 +194,9 @@ __device__ __forceinline__ void oz_bdig_body(const OzArgs &a, int gc, int fg, do
         const int l = m + g * Kq + 32 * rho + l32;
         double2 v = make_double2(0.0, 0.0);
         if (f < a.nf && l <= lend) {
-            const IT *src = (const IT *)a.spec_in + 2 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m + l);
-            v = make_double2((double)src[0], (double)src[1]);
+            using C2 = typename std::conditional<sizeof(IT) == 8, double2, float2>::type;
+            const C2 c = ((const C2 *)a.spec_in)[(size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m + l];
+            v = make_double2((double)c.x, (double)c.y);
         }
         stg[fl * OZ_BSTG + g * 32 + l32] = v;
     }

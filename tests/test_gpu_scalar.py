@@ -148,3 +148,23 @@ def test_batch_odd_nlat_all_transforms(shtk, oracle, cuda):
     Vt_r, Vp_r = ocfg.synthesis_sphtor(S, T)
     assert rel_l2(to_host(Vt), Vt_r) < 1e-12 and rel_l2(to_host(Vp), Vp_r) < 1e-12
     cfg.close()
+
+
+@pytest.mark.parametrize("chains", ["1", "2", "4"])
+@pytest.mark.parametrize("fft_pairs", ["1", "2", "4"])
+def test_launch_shapes(shtk, oracle, cuda, monkeypatch, chains, fft_pairs):
+    """Every synthesis chain count (pairs per lane) and FFT tile width gives the
+    same transform, including a partial last FFT tile (nlat = 302 is no multiple
+    of 8) and a truncation degree inside a 16-degree record group."""
+    lmax, nlat, nlon, ltr = 300, 302, 640, 290
+    monkeypatch.setenv("SHTK_SYNTH_CHAINS", chains)
+    monkeypatch.setenv("SHTK_FFT_PAIRS", fft_pairs)
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+    ocfg = oracle.Config(lmax, nlat, nlon=nlon)
+    alm = oracle.random_alm(lmax, lmax, seed=14, power=1.0, nf=2)
+    f_ref = ocfg.synthesis(alm, ltr=ltr)
+    f = shtk.synthesis_batch(cfg, to_dev(alm), ltr=ltr)
+    assert rel_l2(to_host(f), f_ref) < TOL
+    a_ref = ocfg.analysis(f_ref)
+    assert rel_l2(to_host(shtk.analysis_batch(cfg, f)), a_ref) < TOL
+    cfg.close()

@@ -85,9 +85,11 @@ int shtk_plan_nodes(const shtk_plan *plan, double *x, double *w);
 /* Bytes of device workspace a call with `nfields` fields uses (grown lazily). */
 size_t shtk_workspace_size(const shtk_plan *plan, int nfields, int vector);
 
-/* Scalar synthesis: alm (lmax+1, mmax+1, nfields) complex -> f (nlat, nlon, nfields)
- * real.  ltr < 0 means lmax; 0 <= ltr <= lmax truncates the degree sum.
- * real_output must be 1 (the Hermitian real grid of src/core_transforms.jl:294-344).
+/* Scalar synthesis: alm (lmax+1, mmax+1, nfields) complex -> f (nlat, nlon, nfields).
+ * ltr < 0 means lmax; 0 <= ltr <= lmax truncates the degree sum.  real_output = 1:
+ * the real grid (Hermitian mirror of the bins m > 0); real_output = 0: the complex
+ * grid (interleaved re/im of dtype) of the bins m >= 0 alone, as synthesis_cplx
+ * and _synthesis with real_output = false (src/core_transforms.jl:289-344).
  * Replaces synthesis / synthesis_batch (src/core_transforms.jl:271-344,
  * src/batch_transforms.jl:455-655) behind _gpu_adapter_synthesis
  * (src/device_utils.jl:91-93). */

@@ -10,7 +10,7 @@ from ._lib import BackendUnavailableError, EXPORTS
 from ._lib import load as load_library
 from .config import SHTConfig, create_gauss_config, create_gauss_fly_config, nlm_calc
 from .layout import LM_index, pack_lm, unpack_lm
-from .transforms import (CPU, GPU, on_device, synthesis, analysis, synthesis_batch, analysis_batch,
+from .transforms import (CPU, GPU, on_device, synthesis, analysis, synthesis_batch, analysis_batch, synthesis_cplx,
                          synthesis_l, analysis_l, synthesis_sphtor, analysis_sphtor,
                          synthesis_sphtor_batch, analysis_sphtor_batch, synthesis_sphtor_l, analysis_sphtor_l,
                          synthesis_sph_l, synthesis_tor_l, synthesis_qst, analysis_qst, synthesis_qst_batch,

@@ -274,6 +274,44 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
     }
 }
 
+// Complex-valued synthesis (real_output = false, src/core_transforms.jl:294-344):
+// each latitude is its own complex row with the bins m = 0 .. mmax of the
+// Fourier data (no Hermitian mirror) and the grid element is the complex value.
+// Tiles of G adjacent latitudes, one LDS row each.
+template <int G, typename GT>
+__global__ __launch_bounds__(1024) void k_fft_synth_c(const FftArgs a, int nfields) {
+    extern __shared__ double2 zs[];
+    const int N = a.nlon, nlat = a.nlat, RS = N + (N >> 4) + 1;
+    const int tid = threadIdx.x, nt = blockDim.x, tr = nt / G;
+    const int row = tid / tr, lt = tid % tr;
+    const int k = tid % G;  // latitude of this lane within the tile
+    const int ntile = (nlat + G - 1) / G, total = ntile * nfields;
+    for (int u = blockIdx.x; u < total; u += gridDim.x) {
+        const int f = u / ntile, lat = (u % ntile) * G + k;
+        const bool live = lat < nlat;
+        for (int q = tid; q < G * RS; q += nt) zs[q] = make_double2(0.0, 0.0);
+        __syncthreads();
+        const double2 *Ff = a.F + (size_t)f * a.nm * nlat + lat;
+        double2 *z = zs + k * RS;
+        for (int e = tid; e < a.nm * G; e += nt) {
+            const int im = e / G;
+            if (live) z[zp(im * a.mres)] = Ff[(size_t)im * nlat];
+        }
+        __syncthreads();
+        fft_lds<+1>(zs + row * RS, a, lt, tr);
+        GT *gf = (GT *)a.grid + 2 * ((size_t)f * nlat * N + lat);
+        if (live) {
+            for (int e = tid; e < N * G; e += nt) {
+                const int j = e / G;
+                const double2 v = z[zp(j)];
+                gf[2 * (size_t)j * nlat] = (GT)v.x;
+                gf[2 * (size_t)j * nlat + 1] = (GT)v.y;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Analysis: grid rows -> Fourier data, F_a = (Z_m + conj Z_{N-m}) / 2,
 // F_b = (Z_m - conj Z_{N-m}) / (2i) of the packed row Z = f_a + i f_b.
 template <int G, typename GT>
@@ -361,6 +399,39 @@ hipError_t launch_fft_g(const FftArgs &a, int nfields, int inverse, int tr, hipS
     return hipGetLastError();
 }
 }  // namespace
+
+hipError_t launch_fft_cplx(const FftArgs &a, int nfields, hipStream_t st) {
+    if (a.nrad <= 0) return hipErrorNotSupported;
+    const int N = a.nlon;
+    const int tr = (std::max(64, (N + FFT_VPT - 1) / FFT_VPT) + 63) / 64 * 64;
+    const size_t row = sizeof(double2) * (size_t)(N + (N >> 4) + 1);
+    const int G = (4 * row <= 160 * 1024 && 4 * tr <= 1024) ? 4 : (2 * row <= 160 * 1024 && 2 * tr <= 1024) ? 2 : 1;
+    if (row > 160 * 1024 || tr > 1024) return hipErrorNotSupported;
+    const size_t lds = row * G;
+    const int total = (a.nlat + G - 1) / G * nfields;
+    const int grid = std::max(1, std::min(256 * (int)std::max<size_t>(1, (160 * 1024) / lds), total));
+    const void *fns[3][2] = {{(const void *)k_fft_synth_c<1, double>, (const void *)k_fft_synth_c<1, float>},
+                             {(const void *)k_fft_synth_c<2, double>, (const void *)k_fft_synth_c<2, float>},
+                             {(const void *)k_fft_synth_c<4, double>, (const void *)k_fft_synth_c<4, float>}};
+    const int gi = G == 4 ? 2 : G == 2 ? 1 : 0;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fns[gi][a.io_f32 ? 1 : 0], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 gd(grid), bd(G * tr);
+    if (G == 4) {
+        if (a.io_f32) hipLaunchKernelGGL((k_fft_synth_c<4, float>), gd, bd, lds, st, a, nfields);
+        else hipLaunchKernelGGL((k_fft_synth_c<4, double>), gd, bd, lds, st, a, nfields);
+    } else if (G == 2) {
+        if (a.io_f32) hipLaunchKernelGGL((k_fft_synth_c<2, float>), gd, bd, lds, st, a, nfields);
+        else hipLaunchKernelGGL((k_fft_synth_c<2, double>), gd, bd, lds, st, a, nfields);
+    } else {
+        if (a.io_f32) hipLaunchKernelGGL((k_fft_synth_c<1, float>), gd, bd, lds, st, a, nfields);
+        else hipLaunchKernelGGL((k_fft_synth_c<1, double>), gd, bd, lds, st, a, nfields);
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
     if (a.nrad <= 0) return hipErrorNotSupported;

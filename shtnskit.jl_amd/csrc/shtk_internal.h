@@ -289,6 +289,8 @@ hipError_t launch_vspec_prep(const void *S, const void *T, double *rec, int in_f
                              int ll1, const double *qcc, const long long *coff, int lmax, int mres, int nm,
                              long long per_field, int nfields, hipStream_t st);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
+// Complex-valued synthesis FFT: bins m = 0 .. mmax of F -> complex grid rows.
+hipError_t launch_fft_cplx(const FftArgs &a, int nfields, hipStream_t st);
 // Split-precision vector transforms: (S, T) -> coefficients of 2 nf scalar
 // fields of degree lmax + 1, and the analysis sums of those back to (S, T).
 hipError_t launch_vsplit_coef(const void *S, const void *T, double *C, int in_f32, const double *scale, int divide,

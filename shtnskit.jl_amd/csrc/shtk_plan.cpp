@@ -482,6 +482,7 @@ struct Job {
     int mfac_mode, ll1_mode, fft_mode, ltr;
     const char *leg_name, *fft_name;
     int prec = SHTK_PREC_FP64;
+    int cplx_out = 0;  // scalar synthesis to a complex grid (real_output = false)
 };
 
 // spectral -> grid: [convert] -> Legendre synthesis -> inverse phi FFT
@@ -554,7 +555,12 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
         fa.grid = grids[q];
         fa.io_f32 = dtype == SHTK_F32;
         fa.mode = j.fft_mode;
-        HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 1, st); }), "phi synthesis launch");
+        if (j.cplx_out)
+            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft_cplx(fa, nf, st); }),
+                    "phi synthesis launch");
+        else
+            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 1, st); }),
+                    "phi synthesis launch");
     }
     return SHTK_OK;
 }
@@ -1399,27 +1405,25 @@ size_t shtk_workspace_size(const shtk_plan *plan, int nfields, int vector) {
 // --- scalar ------------------------------------------------------------------
 int shtk_synthesis(shtk_plan *p, const void *alm, void *f, int nfields, int dtype, int real_output, int ltr,
                    void *stream) {
-    RC_TRY(check_common(p, nfields, dtype, {alm, f}));
-    if (real_output != 1)
-        return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is not implemented by this backend");
-    RC_TRY(norm_ltr(p, ltr, &ltr));
-    // f = Re ifft(phi_inv_scale * F): the 1/nlon of the inverse FFT is folded in g
-    Job j{0, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, 0, ltr, "leg_synth", "fft_synth"};
-    return run_to_grid(p, j, alm, nullptr, f, nullptr, nfields, dtype, stream);
+    return shtk_synthesis_ex(p, alm, f, nfields, dtype, real_output, ltr, SHTK_PREC_FP64, stream);
 }
 
 int shtk_synthesis_ex(shtk_plan *p, const void *alm, void *f, int nfields, int dtype, int real_output, int ltr,
                       int precision, void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {alm, f}));
-    if (real_output != 1)
-        return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is not implemented by this backend");
     if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT)
         return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
     if (precision == SHTK_PREC_SPLIT && p->d.lmax > 4095)
         return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
+    if (real_output != 1 && precision == SHTK_PREC_SPLIT)
+        return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is implemented for SHTK_PREC_FP64 only");
     RC_TRY(norm_ltr(p, ltr, &ltr));
+    // real output: f = Re ifft(phi_inv_scale * F) with the Hermitian mirror, the 1/nlon
+    // of the inverse FFT folded in g; complex output (src/core_transforms.jl:294-344):
+    // the bins m >= 0 alone, f = ifft(phi_inv_scale * F)
     Job j{0, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, 0, ltr, "leg_synth", "fft_synth"};
     j.prec = precision;
+    j.cplx_out = real_output != 1;
     return run_to_grid(p, j, alm, nullptr, f, nullptr, nfields, dtype, stream);
 }
 

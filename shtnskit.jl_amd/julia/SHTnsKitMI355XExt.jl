@@ -91,15 +91,18 @@ function _gpu_adapter_synthesis(::MI355XAdapter, cfg::SHTConfig, alm::AbstractAr
                                 real_output::Bool=true, use_rfft::Bool=false, fft_scratch=nothing,
                                 precision::Symbol=:fp64)
     fft_scratch === nothing || throw(ArgumentError("MI355X transforms do not accept a host fft_scratch"))
-    real_output || throw(BackendUnavailableError(:synthesis, "complex-valued output is not implemented"))
     size(alm)[1:2] == (cfg.lmax + 1, cfg.mmax + 1) || throw(DimensionMismatch("alm size $(size(alm))"))
+    real_output || precision === :fp64 ||
+        throw(BackendUnavailableError(:synthesis, "complex-valued output with precision=:split"))
     RT = float(real(eltype(alm)))
+    OT = real_output ? RT : Complex{RT}   # synthesis_cplx: complex grid of the bins m >= 0
     nf = ndims(alm) == 3 ? size(alm, 3) : 1
     a = ROCArray{Complex{RT}}(alm)
-    f = ndims(alm) == 3 ? AMDGPU.zeros(RT, cfg.nlat, cfg.nlon, nf) : AMDGPU.zeros(RT, cfg.nlat, cfg.nlon)
+    f = ndims(alm) == 3 ? AMDGPU.zeros(OT, cfg.nlat, cfg.nlon, nf) : AMDGPU.zeros(OT, cfg.nlat, cfg.nlon)
     _check(:synthesis, ccall((:shtk_synthesis_ex, LIB), Cint,
         (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint, Ptr{Cvoid}),
-        _plan(cfg), pointer(a), pointer(f), nf, _dtype(RT), 1, cfg.lmax, _precision(precision), _stream()))
+        _plan(cfg), pointer(a), pointer(f), nf, _dtype(RT), real_output ? 1 : 0, cfg.lmax, _precision(precision),
+        _stream()))
     AMDGPU.synchronize()                 # results ready on return, as the AMDGPU adapter
     return f
 end

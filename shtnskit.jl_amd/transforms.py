@@ -209,6 +209,12 @@ def analysis(*args, **kw):
     return _analysis(args, dict(kw), False, "analysis")
 
 
+def synthesis_cplx(cfg, alm):
+    """``synthesis_cplx(cfg, alm)`` -> complex grid (nlat, nlon): the bins m >= 0 alone,
+    as ``synthesis(cfg, alm; real_output=false)`` (src/core_transforms.jl:289-292)."""
+    return _synthesis((cfg, alm), {"real_output": False}, alm.dim() == 3, "synthesis_cplx")
+
+
 def synthesis_batch(*args, **kw):
     """``synthesis_batch(cfg, alm (lmax+1, mmax+1, nf); real_output=True, precision="fp64")``
     -> (nlat, nlon, nf).  ``precision="split"`` runs the shared-recurrence Ozaki

@@ -168,3 +168,30 @@ def test_launch_shapes(shtk, oracle, cuda, monkeypatch, chains, fft_pairs):
     a_ref = ocfg.analysis(f_ref)
     assert rel_l2(to_host(shtk.analysis_batch(cfg, f)), a_ref) < TOL
     cfg.close()
+
+
+@pytest.mark.parametrize("lmax,nlat,nlon,mmax,mres,nf", [
+    (3, 8, 10, None, 1, 1),
+    (40, 42, 96, None, 1, 3),
+    (60, 63, 128, 60, 2, 2),
+    (1023, 1024, 2048, None, 1, 1),
+])
+def test_complex_output_synthesis(shtk, oracle, cuda, lmax, nlat, nlon, mmax, mres, nf):
+    """real_output = false / synthesis_cplx: the complex grid of the bins m >= 0
+    (src/core_transforms.jl:289-344), FP64 and FP32 I/O."""
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon, mmax=mmax, mres=mres)
+    ocfg = oracle.Config(lmax, nlat, nlon=nlon, mmax=mmax, mres=mres)
+    mm = lmax if mmax is None else mmax
+    alm = oracle.random_alm(lmax, mm, 23, 1.0, nf=nf)
+    for m in range(mm + 1):
+        if m % mres:
+            alm[:, m] = 0
+    ref = ocfg.synthesis(alm, real_output=False)
+    got = to_host(shtk.synthesis_batch(cfg, to_dev(alm), real_output=False))
+    assert got.dtype == np.complex128
+    assert rel_l2(got, ref) < TOL
+    if nf == 1:
+        z = to_host(shtk.synthesis_cplx(cfg, to_dev(alm[:, :, 0])))
+        assert rel_l2(z, ref[:, :, 0]) < TOL
+    got32 = to_host(shtk.synthesis_batch(cfg, to_dev(alm.astype(np.complex64)), real_output=False))
+    assert got32.dtype == np.complex64 and rel_l2(got32, ref) < 1e-6

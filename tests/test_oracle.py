@@ -219,6 +219,29 @@ def test_fixture_analysis_sphtor(oracle, fid):
         assert FX.isapprox(FX.pack(fx, T[:, :, k]), Tq[:, k], fx["atol"], fx["rtol"])[0]
 
 
+def test_complex_output_against_real_output(oracle):
+    """synthesis with real_output = false keeps the bins m >= 0 alone
+    (src/core_transforms.jl:294-344), so for real a_l0 the real grid is
+    2 Re(complex grid) minus the m = 0 part, which pins the complex path on the
+    fixture-pinned real one."""
+    lmax, nlat, nlon = 12, 14, 32
+    cfg = oracle.Config(lmax, nlat, nlon=nlon)
+    alm = oracle.random_alm(lmax, lmax, 3, 1.0)
+    alm[:, 0] = alm[:, 0].real
+    zc = cfg.synthesis(alm, real_output=False)
+    a0 = np.zeros_like(alm)
+    a0[:, 0] = alm[:, 0]
+    f = cfg.synthesis(alm)
+    f0 = cfg.synthesis(a0)
+    assert np.allclose(2 * zc.real - f0, f, atol=1e-12, rtol=0)
+    # and one bin alone is a pure phase along longitude
+    a1 = np.zeros_like(alm)
+    a1[3, 2] = 1.0 + 0.5j
+    z1 = cfg.synthesis(a1, real_output=False)
+    phi = 2 * np.pi * np.arange(nlon) / nlon
+    assert np.allclose(z1, z1[:, :1] * np.exp(2j * phi)[None, :], atol=1e-13)
+
+
 def test_fixture_qst_full(oracle):
     """QST = scalar Q plus sphtor (S, T) on one grid (src/qst_transforms.jl:102-160): the
     oracle's scalar and vector restatements against the SHTns 3.7 QST fixture."""

@@ -301,11 +301,12 @@ __global__ __launch_bounds__(1024) void k_fft_synth_c(const FftArgs a, int nfiel
         fft_lds<+1>(zs + row * RS, a, lt, tr);
         GT *gf = (GT *)a.grid + 2 * ((size_t)f * nlat * N + lat);
         if (live) {
+            const double sc = fft_row_scale(a, lat);
             for (int e = tid; e < N * G; e += nt) {
                 const int j = e / G;
                 const double2 v = z[zp(j)];
-                gf[2 * (size_t)j * nlat] = (GT)v.x;
-                gf[2 * (size_t)j * nlat + 1] = (GT)v.y;
+                gf[2 * (size_t)j * nlat] = (GT)(v.x * sc);
+                gf[2 * (size_t)j * nlat + 1] = (GT)(v.y * sc);
             }
         }
         __syncthreads();

@@ -722,7 +722,12 @@ int run_vsplit_to_grid(shtk_plan *p, const Job &j, const void *S, const void *T,
         fa.grid = grids[q];
         fa.io_f32 = dtype == SHTK_F32;
         fa.mode = j.fft_mode;
-        HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 1, st); }), "phi synthesis launch");
+        if (j.cplx_out)
+            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft_cplx(fa, nf, st); }),
+                    "phi synthesis launch");
+        else
+            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 1, st); }),
+                    "phi synthesis launch");
     }
     return SHTK_OK;
 }
@@ -1415,8 +1420,6 @@ int shtk_synthesis_ex(shtk_plan *p, const void *alm, void *f, int nfields, int d
         return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
     if (precision == SHTK_PREC_SPLIT && p->d.lmax > 4095)
         return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
-    if (real_output != 1 && precision == SHTK_PREC_SPLIT)
-        return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is implemented for SHTK_PREC_FP64 only");
     RC_TRY(norm_ltr(p, ltr, &ltr));
     // real output: f = Re ifft(phi_inv_scale * F) with the Hermitian mirror, the 1/nlon
     // of the inverse FFT folded in g; complex output (src/core_transforms.jl:294-344):
@@ -1467,12 +1470,12 @@ int shtk_adjoint_analysis(shtk_plan *p, const void *almbar, void *fbar, int nfie
 int shtk_synthesis_sphtor(shtk_plan *p, const void *S, const void *T, void *Vt, void *Vp, int nfields, int dtype,
                           int real_output, int ltr, void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
-    if (real_output != 1)
-        return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is not implemented by this backend");
     RC_TRY(norm_ltr(p, ltr, &ltr));
-    // kernel sums are sin(theta) V; Robert form keeps them, otherwise divide by sin(theta)
+    // kernel sums are sin(theta) V; Robert form keeps them, otherwise divide by sin(theta);
+    // real_output = 0: complex grids of the bins m >= 0 (src/sphtor_transforms.jl:162-241)
     Job j{1, 0, p->d.robert_form ? nullptr : p->invs, p->inv_scale / p->d.nlon, 0, 0, 0, ltr,
           "leg_synth_sphtor", "fft_synth"};
+    j.cplx_out = real_output != 1;
     return run_to_grid(p, j, S, T, Vt, Vp, nfields, dtype, stream);
 }
 
@@ -1490,14 +1493,13 @@ int shtk_synthesis_sphtor_ex(shtk_plan *p, const void *S, const void *T, void *V
         return shtk_synthesis_sphtor(p, S, T, Vt, Vp, nfields, dtype, real_output, ltr, stream);
     if (precision != SHTK_PREC_SPLIT) return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
     RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
-    if (real_output != 1)
-        return fail(SHTK_EUNSUPPORTED, "complex-output synthesis is not implemented by this backend");
     if (p->d.lmax > 4094) return fail(SHTK_EUNSUPPORTED, "split-precision vector transforms support lmax <= 4094");
     RC_TRY(norm_ltr(p, ltr, &ltr));
     // the contraction gives sin(theta) V; the phi FFT divides by sin(theta) unless Robert form
     Job j{1, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, p->d.robert_form ? 0 : 1, ltr, "oz_synth_sphtor",
           "fft_synth"};
     j.prec = precision;
+    j.cplx_out = real_output != 1;
     return run_vsplit_to_grid(p, j, S, T, Vt, Vp, nfields, dtype, stream);
 }
 

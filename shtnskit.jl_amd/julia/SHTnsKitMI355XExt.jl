@@ -92,8 +92,6 @@ function _gpu_adapter_synthesis(::MI355XAdapter, cfg::SHTConfig, alm::AbstractAr
                                 precision::Symbol=:fp64)
     fft_scratch === nothing || throw(ArgumentError("MI355X transforms do not accept a host fft_scratch"))
     size(alm)[1:2] == (cfg.lmax + 1, cfg.mmax + 1) || throw(DimensionMismatch("alm size $(size(alm))"))
-    real_output || precision === :fp64 ||
-        throw(BackendUnavailableError(:synthesis, "complex-valued output with precision=:split"))
     RT = float(real(eltype(alm)))
     OT = real_output ? RT : Complex{RT}   # synthesis_cplx: complex grid of the bins m >= 0
     nf = ndims(alm) == 3 ? size(alm, 3) : 1
@@ -125,15 +123,15 @@ end
 
 function _gpu_adapter_synthesis_sphtor(::MI355XAdapter, cfg::SHTConfig, S, T;
                                        real_output::Bool=true, use_rfft::Bool=false, precision::Symbol=:fp64)
-    real_output || throw(BackendUnavailableError(:synthesis_sphtor, "complex-valued output is not implemented"))
     RT = float(real(eltype(S)))
+    OT = real_output ? RT : Complex{RT}   # complex grids of the bins m >= 0
     nf = ndims(S) == 3 ? size(S, 3) : 1
     s = ROCArray{Complex{RT}}(S); t = ROCArray{Complex{RT}}(T)
     dims = ndims(S) == 3 ? (cfg.nlat, cfg.nlon, nf) : (cfg.nlat, cfg.nlon)
-    Vt = AMDGPU.zeros(RT, dims...); Vp = AMDGPU.zeros(RT, dims...)
+    Vt = AMDGPU.zeros(OT, dims...); Vp = AMDGPU.zeros(OT, dims...)
     _check(:synthesis_sphtor, ccall((:shtk_synthesis_sphtor_ex, LIB), Cint,
         (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint, Ptr{Cvoid}),
-        _plan(cfg), pointer(s), pointer(t), pointer(Vt), pointer(Vp), nf, _dtype(RT), 1, cfg.lmax,
+        _plan(cfg), pointer(s), pointer(t), pointer(Vt), pointer(Vp), nf, _dtype(RT), real_output ? 1 : 0, cfg.lmax,
         _precision(precision), _stream()))
     AMDGPU.synchronize()
     return Vt, Vp

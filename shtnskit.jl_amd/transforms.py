@@ -258,8 +258,9 @@ def _synthesis_sphtor(args, kw, batch, op):
         raise ValueError(f"{op}: Slm and Tlm dims differ")
     nf = _nf(S, batch)
     shape = (cfg.nlat, cfg.nlon) + ((nf,) if batch else ())
-    Vt = _empty_colmajor(shape, rdt, S.device)
-    Vp = _empty_colmajor(shape, rdt, S.device)
+    odt = rdt if real_output else _cplx(rdt)  # complex grids of the bins m >= 0 for real_output=False
+    Vt = _empty_colmajor(shape, odt, S.device)
+    Vp = _empty_colmajor(shape, odt, S.device)
     lib = _lib.load()
     _lib.check(lib.shtk_synthesis_sphtor_ex(ctypes.c_void_p(cfg.plan), _ptr(S), _ptr(T), _ptr(Vt), _ptr(Vp), nf,
                                             _code(rdt), 1 if real_output else 0, int(ltr), prec,

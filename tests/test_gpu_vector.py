@@ -132,3 +132,29 @@ def test_high_lmax_polar_underflow(shtk, oracle, cuda):
     assert rel_l2(to_host(f), f_ref) < TOL
     a = shtk.analysis(cfg, f)
     assert rel_l2(to_host(a), alm) < TOL
+
+
+@pytest.mark.parametrize("robert", [False, True])
+def test_sphtor_complex_output(shtk, oracle, cuda, robert):
+    """synthesis_sphtor(...; real_output=false): complex grids of the bins m >= 0
+    (src/sphtor_transforms.jl:162-241), plain FP64 and split precision."""
+    lmax, nlat, nlon, nf = 40, 42, 96, 2
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon, robert_form=robert)
+    ocfg = oracle.Config(lmax, nlat, nlon=nlon, robert_form=robert)
+    S, T = _st(oracle, lmax, lmax, 41, nf=nf)
+    Vt_r, Vp_r = ocfg.synthesis_sphtor(S, T, real_output=False)
+    Vt, Vp = shtk.synthesis_sphtor_batch(cfg, to_dev(S), to_dev(T), real_output=False)
+    assert to_host(Vt).dtype == np.complex128
+    assert rel_l2(to_host(Vt), Vt_r) < TOL and rel_l2(to_host(Vp), Vp_r) < TOL
+    Vt2, Vp2 = shtk.synthesis_sphtor_batch(cfg, to_dev(S), to_dev(T), real_output=False, precision="split")
+    assert rel_l2(to_host(Vt2), Vt_r) < 1e-11 and rel_l2(to_host(Vp2), Vp_r) < 1e-11
+
+
+def test_split_scalar_complex_output(shtk, oracle, cuda):
+    lmax, nlat, nlon, nf = 63, 64, 128, 3
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+    ocfg = oracle.Config(lmax, nlat, nlon=nlon)
+    alm = oracle.random_alm(lmax, lmax, 42, 1.0, nf=nf)
+    ref = ocfg.synthesis(alm, real_output=False)
+    got = to_host(shtk.synthesis_batch(cfg, to_dev(alm), real_output=False, precision="split"))
+    assert rel_l2(got, ref) < 1e-11

@@ -42,6 +42,11 @@ enum shtk_status {
 
 enum shtk_norm { SHTK_NORM_ORTHONORMAL = 0, SHTK_NORM_FOURPI = 1, SHTK_NORM_SCHMIDT = 2 };
 enum shtk_phi_scale { SHTK_PHI_DFT = 0, SHTK_PHI_QUAD = 1 };
+/* Latitude grids of create_gauss_config and create_regular_config (src/config.jl:633-655,
+ * 1235-1330): Gauss-Legendre nodes; Fejer-1 midpoints theta = (i + 1/2) pi / nlat
+ * (include_poles = false); Clenshaw-Curtis nodes theta = i pi / (nlat - 1) with both
+ * poles (include_poles = true). */
+enum shtk_grid { SHTK_GRID_GAUSS = 0, SHTK_GRID_REGULAR = 1, SHTK_GRID_REGULAR_POLES = 2 };
 enum shtk_dtype { SHTK_F64 = 0, SHTK_F32 = 1 };
 
 /* Contraction precision of the batched entry points (*_ex).
@@ -53,9 +58,13 @@ enum shtk_precision { SHTK_PREC_FP64 = 0, SHTK_PREC_SPLIT = 1 };
 
 /* Grid and spectral descriptor.  Mirrors the keyword set of
  * create_gauss_config(lmax, nlat; mmax, mres, nlon, norm, cs_phase, real_norm,
- * robert_form) (src/config.jl:633-655) plus the phi scaling mode
- * (src/SHTnsKit.jl:141-177).  Constraints as src/config.jl:465-481:
- * nlat >= lmax+1, nlon >= 2*mmax+1, 0 <= mmax <= lmax, mres >= 1. */
+ * robert_form) (src/config.jl:633-655) and of create_regular_config(...;
+ * include_poles) (src/config.jl:1235-1330) plus the phi scaling mode
+ * (src/SHTnsKit.jl:141-177).  Constraints as src/config.jl:465-481 and the
+ * grid minimums of _min_nlat (src/config.jl:452-458): nlat >= lmax+1 (Gauss,
+ * regular with poles, and at least 2 there), nlat >= lmax+2 (regular),
+ * nlon >= 2*mmax+1, 0 <= mmax <= lmax, mres >= 1.  The Driscoll-Healy grid
+ * (use_dh_weights) is not provided. */
 typedef struct shtk_desc {
     int lmax, mmax, mres, nlat, nlon;
     int norm;        /* enum shtk_norm */
@@ -63,12 +72,14 @@ typedef struct shtk_desc {
     int real_norm;   /* 1: SHTns REAL_NORM convention */
     int robert_form; /* 1: vector fields in Robert form (multiplied by sin(theta)) */
     int phi_scale;   /* enum shtk_phi_scale */
+    int grid;        /* enum shtk_grid */
 } shtk_desc;
 
 typedef struct shtk_plan shtk_plan;
 
-/* Builds a Gauss-Legendre plan: nodes and weights (src/legendre.jl:956-1003,
- * north first as src/config.jl:639), FP64 recurrence coefficients, sectoral
+/* Builds a plan: nodes and weights (Gauss-Legendre src/legendre.jl:956-1003,
+ * north first as src/config.jl:639; Fejer-1 and Clenshaw-Curtis
+ * src/legendre.jl:1017-1075 as create_regular_config), FP64 recurrence coefficients, sectoral
  * seeds with scaled exponents, FFT twiddles.  Replaces create_gauss_config
  * (src/config.jl:633-655) and the device table builders of
  * ext/SHTnsKitAMDGPUExt.jl:495-557 (which hold a full nlat x (lmax+1) x (mmax+1)

@@ -112,24 +112,62 @@ def fft(v: np.ndarray, inverse: bool = False) -> np.ndarray:
     return a
 
 
+def fejer1(n):
+    """Fejer's first rule, north first: theta_j = (j + 1/2) pi / n and
+    w_j = (2/n) (1 - 2 sum_{k=1}^{n//2} cos(2 k theta_j) / (4 k^2 - 1))
+    (_fejer1_weights, src/legendre.jl:1017-1032; create_regular_config
+    src/config.jl:1294-1300).  Returns (x, w)."""
+    th = (np.arange(n) + 0.5) * (math.pi / n)
+    k = np.arange(1, n // 2 + 1)
+    series = (np.cos(2.0 * np.outer(th, k)) / (4.0 * k * k - 1.0)).sum(axis=1) if k.size else np.zeros(n)
+    return np.cos(th), (2.0 / n) * (1.0 - 2.0 * series)
+
+
+def clenshaw_curtis(n):
+    """Clenshaw-Curtis rule with both poles, north first: theta_j = j pi / N,
+    N = n - 1, endpoint weight 1/(N^2 - 1) (N even) or 1/N^2 (N odd), interior
+    (2/N) (1 - 2 sum_k cos(2 k theta)/(4k^2 - 1) [- cos(N theta)/(N^2 - 1), N even])
+    (_clenshaw_curtis_weights, src/legendre.jl:1044-1075; create_regular_config
+    src/config.jl:1285-1292).  Returns (x, w) with x = +-1 exactly at the poles."""
+    N = n - 1
+    th = np.arange(n) * (math.pi / N)
+    x = np.cos(th)
+    x[0], x[-1] = 1.0, -1.0
+    kmax = N // 2 - 1 if N % 2 == 0 else (N - 1) // 2
+    k = np.arange(1, kmax + 1)
+    v = 1.0 - 2.0 * (np.cos(2.0 * np.outer(th, k)) / (4.0 * k * k - 1.0)).sum(axis=1)
+    if N % 2 == 0:
+        v -= np.cos(N * th) / (N * N - 1.0)
+    w = 2.0 * v / N
+    w[0] = w[-1] = 1.0 / (N * N - 1.0) if N % 2 == 0 else 1.0 / (N * N)
+    return x, w
+
+
 class Config:
-    """Gauss-Legendre configuration mirroring ``create_gauss_config``
-    (src/config.jl:633-655): nodes stored north first."""
+    """Grid configuration mirroring ``create_gauss_config`` (src/config.jl:633-655)
+    and ``create_regular_config`` (src/config.jl:1235-1330, grids "regular" and
+    "regular_poles"): nodes stored north first."""
 
     def __init__(self, lmax, nlat, mmax=None, mres=1, nlon=None, norm="orthonormal",
-                 cs_phase=True, real_norm=False, robert_form=False, phi_scale="dft"):
+                 cs_phase=True, real_norm=False, robert_form=False, phi_scale="dft", grid="gauss"):
         mmax = lmax if mmax is None else mmax
         if nlon is None:
             nlon = 2 * lmax + 1
             nlon += nlon & 1
-        if not (0 <= mmax <= lmax) or nlat < lmax + 1 or nlon < 2 * mmax + 1 or mres < 1:
+        min_nlat = {"gauss": lmax + 1, "regular": lmax + 2, "regular_poles": max(2, lmax + 1)}[grid]
+        if not (0 <= mmax <= lmax) or nlat < min_nlat or nlon < 2 * mmax + 1 or mres < 1:
             raise ValueError("invalid spectral layout")
         self.lmax, self.mmax, self.mres, self.nlat, self.nlon = lmax, mmax, mres, nlat, nlon
         self.norm, self.cs_phase, self.real_norm = norm, cs_phase, real_norm
-        self.robert_form, self.phi_scale = robert_form, phi_scale
-        xa, wa = gausslegendre(nlat)
-        self.x = np.ascontiguousarray(xa[::-1])
-        self.w = np.ascontiguousarray(wa[::-1])
+        self.robert_form, self.phi_scale, self.grid = robert_form, phi_scale, grid
+        if grid == "gauss":
+            xa, wa = gausslegendre(nlat)
+            self.x = np.ascontiguousarray(xa[::-1])
+            self.w = np.ascontiguousarray(wa[::-1])
+        else:
+            xa, wa = fejer1(nlat) if grid == "regular" else clenshaw_curtis(nlat)
+            self.x = np.ascontiguousarray(xa)
+            self.w = np.ascontiguousarray(wa)
         self._c = _CCfg(lmax, mmax, mres, nlat, nlon, NORMS[norm], int(cs_phase), int(real_norm),
                         int(robert_form), 1 if phi_scale == "quad" else 0,
                         _ptr(self.x), _ptr(self.w))

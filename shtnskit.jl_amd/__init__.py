@@ -8,7 +8,8 @@ binds the same C ABI lives in ``julia/``.
 """
 from ._lib import BackendUnavailableError, EXPORTS
 from ._lib import load as load_library
-from .config import SHTConfig, create_gauss_config, create_gauss_fly_config, nlm_calc
+from .config import (SHTConfig, create_config, create_gauss_config, create_gauss_fly_config,
+                     create_regular_config, nlm_calc)
 from .layout import LM_index, pack_lm, unpack_lm
 from .transforms import (CPU, GPU, on_device, synthesis, analysis, synthesis_batch, analysis_batch, synthesis_cplx,
                          synthesis_l, analysis_l, synthesis_sphtor, analysis_sphtor,

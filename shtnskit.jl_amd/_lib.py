@@ -41,7 +41,7 @@ class BackendUnavailableError(RuntimeError):
 class ShtkDesc(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "lmax", "mmax", "mres", "nlat", "nlon", "norm", "cs_phase", "real_norm", "robert_form",
-        "phi_scale")]
+        "phi_scale", "grid")]
 
 
 _lib = None

@@ -130,6 +130,7 @@ struct shtk_plan {
     int np = 0, nm = 0, seg = 128, nseg_total = 0;
     int xcd_map = 0, synth_chains = 4, fft_tile = 0;  // launch-shape knobs
     bool canonical = true;
+    bool poles = false;  // nodes at theta = 0 and pi (the first pair)
     double inv_scale = 0, scale_phi = 0;
     std::vector<double> hx, hw;  // north-first nodes and weights (nlat)
     int device = 0;
@@ -289,6 +290,12 @@ hipError_t launch_vspec_prep(const void *S, const void *T, double *rec, int in_f
                              int ll1, const double *qcc, const long long *coff, int lmax, int mres, int nm,
                              long long per_field, int nfields, hipStream_t st);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
+// Pole rows of order 1 for the vector transforms on grids with nodes at the
+// poles (one block per field): synthesis rewrites them from the prologue
+// records in a.in0 (robert = 1: zeros every order's pole rows), analysis adds
+// the weighted pole terms to the order-1 outputs in a.out0 / a.out1.
+hipError_t launch_vpole_synth(const LegArgs &a, int nfields, int robert, hipStream_t st);
+hipError_t launch_vpole_anal(const LegArgs &a, int nfields, double wpole, hipStream_t st);
 // Complex-valued synthesis FFT: bins m = 0 .. mmax of F -> complex grid rows.
 hipError_t launch_fft_cplx(const FftArgs &a, int nfields, hipStream_t st);
 // Split-precision vector transforms: (S, T) -> coefficients of 2 nf scalar

@@ -674,6 +674,105 @@ __global__ __launch_bounds__(256) void k_vspec_prep(const IT *S, const IT *T, do
 }
 
 // ---------------------------------------------------------------------------
+// Vector transforms on grids with nodes at the poles (first pair = theta 0, pi).
+// The Legendre kernels weight those rows by 1/sin(theta) := 0, which is the
+// exact limit for m = 0 and m >= 2.  For m = 1 the pole limits are finite:
+// dP_l/dtheta = P_l/sin(theta) = -h_l at the north pole and dP_l/dtheta =
+// (-1)^(l+1) h_l, P_l/sin(theta) = (-1)^l h_l at the south pole, with
+// h_l = sqrt((2l+1) l (l+1) / (4 pi)) / 2 (orthonormal, Condon-Shortley phase;
+// _dPdtheta_at_pole / _P_over_sinth_at_pole, src/kernels.jl:72-99, and
+// src/legendre.jl:405-423).  These kernels rewrite the pole rows of order 1
+// (one block per field); in Robert form the synthesis rows are sin(theta) times
+// the field, i.e. zero for every order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double pole_h(int l) {
+    return 0.5 * 0.28209479177387814 * sqrt((2.0 * l + 1.0) * (double)l * (double)(l + 1));
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum256(double (*red)[256], double *v) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = red[k][0];
+}
+
+__global__ __launch_bounds__(256) void k_vpole_synth(const LegArgs a, int robert) {
+    __shared__ double red[8][256];
+    const int f = blockIdx.x, nlat = a.nlat;
+    if (robert) {
+        for (int im = threadIdx.x; im < a.nm; im += 256) {
+            const size_t fo = ((size_t)f * a.nm + im) * nlat;
+            const double2 z = make_double2(0.0, 0.0);
+            a.F0[fo] = z;
+            a.F1[fo] = z;
+            a.F0[fo + nlat - 1] = z;
+            a.F1[fo + nlat - 1] = z;
+        }
+        return;
+    }
+    // prologue records (S', T', c S', c T') of order 1
+    const double *rec = (const double *)a.in0 + 8 * ((size_t)f * a.spec_fstride + (size_t)(a.lmax + 1));
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int lt = min(a.ltr, a.lmax);
+    for (int l = 1 + (int)threadIdx.x; l <= lt; l += 256) {
+        const double h = pole_h(l), hs = (l & 1) ? -h : h;
+        const double *r = rec + 8 * (size_t)l;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = fma(h, r[k], v[k]);
+            v[4 + k] = fma(hs, r[k], v[4 + k]);
+        }
+    }
+    block_sum256<8>(red, v);
+    if (threadIdx.x == 0) {
+        const double gm = a.g * mfac_of(a.mfac_mode, 1);
+        const size_t fo = ((size_t)f * a.nm + 1) * nlat;
+        // north: Vt = -sum h S' + i sum h T', Vp = -i sum h S' - sum h T'
+        a.F0[fo] = make_double2(gm * (-v[0] - v[3]), gm * (v[2] - v[1]));
+        a.F1[fo] = make_double2(gm * (v[1] - v[2]), gm * (-v[0] - v[3]));
+        // south (sums with (-1)^l h): Vt = -sum S' - i sum T', Vp = i sum S' - sum T'
+        a.F0[fo + nlat - 1] = make_double2(gm * (v[7] - v[4]), gm * (-v[5] - v[6]));
+        a.F1[fo + nlat - 1] = make_double2(gm * (-v[5] - v[6]), gm * (v[4] - v[7]));
+    }
+}
+
+// Analysis: adds the weighted pole rows of order 1 to S_l1, T_l1 (after the
+// Legendre analysis wrote them), S += w [dP Vt - i P/s Vp], T += w [i P/s Vt + dP Vp].
+template <typename OT>
+__global__ __launch_bounds__(256) void k_vpole_anal(const LegArgs a, double wpole) {
+    const int f = blockIdx.x, nlat = a.nlat;
+    const size_t fo = ((size_t)f * a.nm + 1) * nlat;
+    const double2 tN = a.F0[fo], pN = a.F1[fo], tS = a.F0[fo + nlat - 1], pS = a.F1[fo + nlat - 1];
+    const double gm = a.g * mfac_of(a.mfac_mode, 1) * wpole;
+    const size_t co = 2 * ((size_t)f * a.spec_fstride + (size_t)(a.lmax + 1));
+    OT *cs = (OT *)a.out0 + co, *ct = (OT *)a.out1 + co;
+    const int lt = min(a.ltr, a.lmax);
+    for (int l = 1 + (int)threadIdx.x; l <= lt; l += 256) {
+        const double h = pole_h(l), hs = (l & 1) ? -h : h;
+        const double dN = -h, qN = -h, dS = -hs, qS = hs;  // dP/dtheta and P/sin(theta)
+        double sc = gm;
+        if (a.ll1_mode) sc /= (double)l * (l + 1);
+        const double sr = dN * tN.x + qN * pN.y + dS * tS.x + qS * pS.y;
+        const double si = dN * tN.y - qN * pN.x + dS * tS.y - qS * pS.x;
+        const double tr = -qN * tN.y + dN * pN.x - qS * tS.y + dS * pS.x;
+        const double ti = qN * tN.x + dN * pN.y + qS * tS.x + dS * pS.y;
+        cs[2 * l] = (OT)((double)cs[2 * l] + sc * sr);
+        cs[2 * l + 1] = (OT)((double)cs[2 * l + 1] + sc * si);
+        ct[2 * l] = (OT)((double)ct[2 * l] + sc * tr);
+        ct[2 * l + 1] = (OT)((double)ct[2 * l + 1] + sc * ti);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Analysis work items: (global segment, pair block of 512, field).
 // ---------------------------------------------------------------------------
 struct Item {
@@ -1170,6 +1269,19 @@ hipError_t launch_vspec_prep(const void *S, const void *T, double *rec, int in_f
     else
         hipLaunchKernelGGL(k_vspec_prep<double>, grid, dim3(256), 0, st, (const double *)S, (const double *)T, rec,
                            scale, divide, ll1, qcc, coff, lmax, mres, nm, per_field);
+    return hipGetLastError();
+}
+
+hipError_t launch_vpole_synth(const LegArgs &a, int nfields, int robert, hipStream_t st) {
+    hipLaunchKernelGGL(k_vpole_synth, dim3((unsigned)nfields), dim3(256), 0, st, a, robert);
+    return hipGetLastError();
+}
+
+hipError_t launch_vpole_anal(const LegArgs &a, int nfields, double wpole, hipStream_t st) {
+    if (a.io_f32)
+        hipLaunchKernelGGL(k_vpole_anal<float>, dim3((unsigned)nfields), dim3(256), 0, st, a, wpole);
+    else
+        hipLaunchKernelGGL(k_vpole_anal<double>, dim3((unsigned)nfields), dim3(256), 0, st, a, wpole);
     return hipGetLastError();
 }
 

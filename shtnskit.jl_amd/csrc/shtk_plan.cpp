@@ -99,6 +99,46 @@ void gauss_nodes(int n, std::vector<double> &x, std::vector<double> &w) {
     }
 }
 
+// Fejer's first rule on the midpoint nodes theta_j = (j + 1/2) pi / n (weights in
+// x = cos(theta)): w_j = (2/n) (1 - 2 sum_{k=1}^{floor(n/2)} cos(2 k theta_j) / (4k^2 - 1)),
+// the formula of _fejer1_weights (src/legendre.jl:1017-1032).  Nodes north first.
+void fejer1_nodes(int n, std::vector<double> &x, std::vector<double> &w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        // node angle as create_regular_config (src/config.jl:1296), weight angle as the rule
+        x[j] = std::cos((j + 0.5) * (M_PI / n));
+        const double th = (j + 0.5) * M_PI * (1.0 / n);
+        double series = 0.0;
+        for (int k = 1; k <= n / 2; ++k) series += std::cos(2.0 * k * th) / (4.0 * k * k - 1.0);
+        w[j] = (2.0 / n) * (1.0 - 2.0 * series);
+    }
+}
+
+// Clenshaw-Curtis rule on theta_j = j pi / N, N = n - 1, both poles included,
+// with the parity-dependent endpoint term of _clenshaw_curtis_weights
+// (src/legendre.jl:1044-1075).  Nodes north first; the exact poles get x = +-1.
+void clenshaw_curtis_nodes(int n, std::vector<double> &x, std::vector<double> &w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    const int N = n - 1;
+    const bool even = (N % 2) == 0;
+    const double endpoint = even ? 1.0 / ((double)N * N - 1.0) : 1.0 / ((double)N * N);
+    for (int j = 0; j <= N; ++j) {
+        const double th = j * M_PI / N;
+        x[j] = j == 0 ? 1.0 : (j == N ? -1.0 : std::cos(j * (M_PI / N)));
+        if (j == 0 || j == N) {
+            w[j] = endpoint;
+            continue;
+        }
+        double v = 1.0;
+        const int kmax = even ? N / 2 - 1 : (N - 1) / 2;
+        for (int k = 1; k <= kmax; ++k) v -= 2.0 * std::cos(2.0 * k * th) / (4.0 * k * k - 1.0);
+        if (even) v -= std::cos(N * th) / ((double)N * N - 1.0);
+        w[j] = 2.0 * v / N;
+    }
+}
+
 double coef_scale(const shtk_desc &d, int l, int m) {
     double nrm = 1.0;
     if (d.norm == SHTK_NORM_FOURPI) nrm = std::sqrt(4.0 * M_PI);
@@ -138,8 +178,14 @@ int validate(const shtk_desc *d) {
     if (d->lmax < 0) return fail(SHTK_EINVAL, "lmax must be >= 0");
     if (d->mmax < 0 || d->mmax > d->lmax) return fail(SHTK_EINVAL, "mmax must satisfy 0 <= mmax <= lmax");
     if (d->mres < 1) return fail(SHTK_EINVAL, "mres must be >= 1");
-    if (d->nlat < d->lmax + 1)
+    if (d->grid < SHTK_GRID_GAUSS || d->grid > SHTK_GRID_REGULAR_POLES)
+        return fail(SHTK_EINVAL, "unknown grid type (gauss, regular or regular with poles)");
+    if (d->grid == SHTK_GRID_GAUSS && d->nlat < d->lmax + 1)
         return fail(SHTK_EINVAL, "Gauss-Legendre quadrature requires nlat >= lmax+1");
+    if (d->grid == SHTK_GRID_REGULAR && d->nlat < d->lmax + 2)
+        return fail(SHTK_EINVAL, "nlat must be >= lmax+2 for regular grids");
+    if (d->grid == SHTK_GRID_REGULAR_POLES && d->nlat < std::max(2, d->lmax + 1))
+        return fail(SHTK_EINVAL, "nlat must be >= max(2, lmax+1) for regular grids with poles");
     if (d->nlon < 2 * d->mmax + 1) return fail(SHTK_EINVAL, "nlon must be >= 2*mmax+1");
     if (d->norm < 0 || d->norm > 2) return fail(SHTK_EINVAL, "unknown normalization");
     if (d->phi_scale != SHTK_PHI_DFT && d->phi_scale != SHTK_PHI_QUAD)
@@ -483,7 +529,15 @@ struct Job {
     const char *leg_name, *fft_name;
     int prec = SHTK_PREC_FP64;
     int cplx_out = 0;  // scalar synthesis to a complex grid (real_output = false)
+    int pole_fix = 0;  // forward vector transform: order-1 pole limits on grids with poles
 };
+
+// Vector transforms on grids with nodes at the poles: the FP64 forward pair
+// takes the pole limits of order 1; the split and adjoint forms are not provided.
+int pole_vector_check(const shtk_plan *p, const char *what) {
+    if (!p->poles) return SHTK_OK;
+    return fail(SHTK_EUNSUPPORTED, std::string(what) + " is not provided on grids with nodes at the poles");
+}
 
 // spectral -> grid: [convert] -> Legendre synthesis -> inverse phi FFT
 int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void *g0, void *g1, int nf, int dtype,
@@ -547,6 +601,8 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
         HIP_TRY(timed(p, j.leg_name, st,
                       [&] { return shtk::launch_leg_synth(a, nf, j.vector, p->synth_chains, st); }),
                 "Legendre synthesis launch");
+        if (j.pole_fix && p->poles && (p->d.robert_form || (p->d.mres == 1 && p->d.mmax >= 1)))
+            HIP_TRY(shtk::launch_vpole_synth(a, nf, p->d.robert_form, st), "pole rows launch");
     }
     void *grids[2] = {g0, g1};
     for (int q = 0; q <= j.vector; ++q) {
@@ -625,6 +681,8 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
     }
     HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_leg_anal(a, nf, j.vector, st); }),
             "Legendre analysis launch");
+    if (j.pole_fix && p->poles && p->d.mres == 1 && p->d.mmax >= 1)
+        HIP_TRY(shtk::launch_vpole_anal(a, nf, p->hw[0] * p->scale_phi, st), "pole rows launch");
     if (!p->canonical) {
         void *dst[2] = {s0, s1};
         for (int q = 0; q <= j.vector; ++q)
@@ -1328,9 +1386,17 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     p->inv_scale = d.phi_scale == SHTK_PHI_QUAD ? d.nlon / (2.0 * M_PI) : (double)d.nlon;
     p->scale_phi = (2.0 * M_PI / d.nlon) * d.nlon / p->inv_scale;
     std::vector<double> xa, wa;
-    gauss_nodes(d.nlat, xa, wa);
-    p->hx.assign(xa.rbegin(), xa.rend());  // north first (src/config.jl:639)
-    p->hw.assign(wa.rbegin(), wa.rend());
+    if (d.grid == SHTK_GRID_GAUSS) {
+        gauss_nodes(d.nlat, xa, wa);
+        p->hx.assign(xa.rbegin(), xa.rend());  // north first (src/config.jl:639)
+        p->hw.assign(wa.rbegin(), wa.rend());
+    } else {
+        if (d.grid == SHTK_GRID_REGULAR) fejer1_nodes(d.nlat, xa, wa);
+        else clenshaw_curtis_nodes(d.nlat, xa, wa);
+        p->hx = xa;
+        p->hw = wa;
+    }
+    p->poles = d.grid == SHTK_GRID_REGULAR_POLES;
     rc = plan_tables(p);
     if (rc) {
         free_plan(p);
@@ -1476,6 +1542,7 @@ int shtk_synthesis_sphtor(shtk_plan *p, const void *S, const void *T, void *Vt, 
     Job j{1, 0, p->d.robert_form ? nullptr : p->invs, p->inv_scale / p->d.nlon, 0, 0, 0, ltr,
           "leg_synth_sphtor", "fft_synth"};
     j.cplx_out = real_output != 1;
+    j.pole_fix = 1;
     return run_to_grid(p, j, S, T, Vt, Vp, nfields, dtype, stream);
 }
 
@@ -1483,7 +1550,13 @@ int shtk_analysis_sphtor(shtk_plan *p, const void *Vt, const void *Vp, void *S, 
                          int ltr, void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
     RC_TRY(norm_ltr(p, ltr, &ltr));
+    // Robert-form rows cannot be divided by sin(theta) at weighted pole nodes when
+    // order 1 is stored (_validate_robert_analysis, src/config.jl:608-624)
+    if (p->poles && p->d.robert_form && p->d.mres == 1 && p->d.mmax >= 1 && ltr >= 1)
+        return fail(SHTK_EINVAL, "Robert-form components with m=1 cannot be unweighted at nonzero-weight pole "
+                                 "nodes; use a Gauss or pole-free regular grid");
     Job j{1, 1, p->d.robert_form ? p->winvs2 : p->winvs, 1.0, 0, 1, 0, ltr, "leg_anal_sphtor", "fft_anal"};
+    j.pole_fix = 1;
     return run_to_spec(p, j, Vt, Vp, S, T, nfields, dtype, stream);
 }
 
@@ -1494,6 +1567,7 @@ int shtk_synthesis_sphtor_ex(shtk_plan *p, const void *S, const void *T, void *V
     if (precision != SHTK_PREC_SPLIT) return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
     RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
     if (p->d.lmax > 4094) return fail(SHTK_EUNSUPPORTED, "split-precision vector transforms support lmax <= 4094");
+    RC_TRY(pole_vector_check(p, "split-precision vector synthesis"));
     RC_TRY(norm_ltr(p, ltr, &ltr));
     // the contraction gives sin(theta) V; the phi FFT divides by sin(theta) unless Robert form
     Job j{1, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, p->d.robert_form ? 0 : 1, ltr, "oz_synth_sphtor",
@@ -1509,6 +1583,7 @@ int shtk_analysis_sphtor_ex(shtk_plan *p, const void *Vt, const void *Vp, void *
     if (precision != SHTK_PREC_SPLIT) return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
     RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
     if (p->d.lmax > 4094) return fail(SHTK_EUNSUPPORTED, "split-precision vector transforms support lmax <= 4094");
+    RC_TRY(pole_vector_check(p, "split-precision vector analysis"));
     RC_TRY(norm_ltr(p, ltr, &ltr));
     Job j{1, 1, p->d.robert_form ? p->winvs2 : p->winvs, 1.0, 0, 1, 0, ltr, "oz_anal_sphtor", "fft_anal"};
     j.prec = precision;
@@ -1518,6 +1593,7 @@ int shtk_analysis_sphtor_ex(shtk_plan *p, const void *Vt, const void *Vp, void *
 int shtk_adjoint_synthesis_sphtor(shtk_plan *p, const void *Vtb, const void *Vpb, void *Sb, void *Tb, int nfields,
                                   int dtype, int real_output, void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {Vtb, Vpb, Sb, Tb}));
+    RC_TRY(pole_vector_check(p, "the adjoint vector synthesis"));
     Job j{1, 0, p->invs, p->inv_scale / p->d.nlon, real_output ? 1 : 0, 0, p->d.robert_form ? 2 : 0, p->d.lmax,
           "leg_anal_sphtor", "fft_anal"};
     return run_to_spec(p, j, Vtb, Vpb, Sb, Tb, nfields, dtype, stream);
@@ -1526,6 +1602,7 @@ int shtk_adjoint_synthesis_sphtor(shtk_plan *p, const void *Vtb, const void *Vpb
 int shtk_adjoint_analysis_sphtor(shtk_plan *p, const void *Sb, const void *Tb, void *Vtb, void *Vpb, int nfields,
                                  int dtype, void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {Vtb, Vpb, Sb, Tb}));
+    RC_TRY(pole_vector_check(p, "the adjoint vector analysis"));
     Job j{1, 1, p->winvs, 1.0, 2, 1, p->d.robert_form ? 1 : 0, p->d.lmax, "leg_synth_sphtor", "fft_synth"};
     return run_to_grid(p, j, Sb, Tb, Vtb, Vpb, nfields, dtype, stream);
 }

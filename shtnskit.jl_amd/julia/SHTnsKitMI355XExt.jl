@@ -27,6 +27,7 @@ const SHTK_PREC_FP64, SHTK_PREC_SPLIT = 0, 1
 struct ShtkDesc          # layout of `shtk_desc`
     lmax::Cint; mmax::Cint; mres::Cint; nlat::Cint; nlon::Cint
     norm::Cint; cs_phase::Cint; real_norm::Cint; robert_form::Cint; phi_scale::Cint
+    grid::Cint
 end
 
 function _last_error()
@@ -45,9 +46,17 @@ function _check(op::Symbol, status::Cint)
 end
 
 _norm_code(cfg) = cfg.norm === :orthonormal ? 0 : cfg.norm === :fourpi ? 1 : 2
+# shtk_grid: Gauss-Legendre, Fejer midpoints, Clenshaw-Curtis with poles (no Driscoll-Healy)
+function _grid_code(cfg)
+    g = cfg.grid_type
+    (g === :gauss || g === :gauss_fly) && return 0
+    g === :regular && return 1
+    g === :regular_poles && return 2
+    throw(BackendUnavailableError(:plan, "grid_type $(g) is not provided by the MI355X backend"))
+end
 _desc(cfg::SHTConfig) = ShtkDesc(cfg.lmax, cfg.mmax, cfg.mres, cfg.nlat, cfg.nlon, _norm_code(cfg),
                                  cfg.cs_phase, cfg.real_norm, cfg.robert_form,
-                                 cfg.phi_scale === :quad ? 1 : 0)
+                                 cfg.phi_scale === :quad ? 1 : 0, _grid_code(cfg))
 
 # plan cache keyed by (device, config signature); plans are read-only after creation
 mutable struct MI355XAdapter

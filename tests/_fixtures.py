@@ -47,6 +47,11 @@ def isapprox(x, y, atol, rtol):
     return d <= max(atol, rtol * max(np.linalg.norm(x), np.linalg.norm(y))), d
 
 
+def grid_of(fx):
+    """Grid name of a fixture for the configurations here ("gauss_fly" is a Gauss grid)."""
+    return {"gauss": "gauss", "gauss_fly": "gauss", "regular": "regular", "regular_poles": "regular_poles"}[fx["grid"]]
+
+
 def config_kwargs(fx):
     return dict(mmax=fx["mmax"], mres=fx["mres"], nlon=fx["nphi"], norm=fx["norm"],
                 cs_phase=fx["cs_phase"], real_norm=fx["real_norm"])

@@ -115,7 +115,7 @@ def test_fft_brute_force_dft(oracle):
 
 # --------------------------------------------------------------------------- SHTns 3.7 fixtures
 def _cfg(oracle, fx):
-    return oracle.Config(fx["lmax"], fx["nlat"], **FX.config_kwargs(fx))
+    return oracle.Config(fx["lmax"], fx["nlat"], grid=FX.grid_of(fx), **FX.config_kwargs(fx))
 
 
 def test_fixture_scalar_real_full(oracle):
@@ -366,3 +366,104 @@ def test_adjoint_synthesis_finite_difference(oracle):
     h = 1e-6
     fd = (loss(a + h * d) - loss(a - h * d)) / (2 * h)
     assert abs(fd - _inner_real(g, d)) <= 5e-4 * abs(fd)
+
+
+# --------------------------------------------------------------------------- regular grids
+@pytest.mark.parametrize("n", [2, 3, 5, 6, 9, 10, 17, 32])
+def test_regular_rules_exactness(oracle, n):
+    """Fejer-1 and Clenshaw-Curtis with n nodes integrate x^k exactly on [-1, 1]
+    for k <= n - 1 (and k = n for odd n), nodes at the stated angles, north first."""
+    for rule, th in ((oracle.fejer1, (np.arange(n) + 0.5) * np.pi / n),
+                     (oracle.clenshaw_curtis, np.arange(n) * np.pi / max(n - 1, 1))):
+        if rule is oracle.fejer1 or n >= 2:
+            x, w = rule(n)
+            assert np.allclose(x, np.cos(th), atol=1e-15, rtol=0)
+            assert np.all(np.diff(x) < 0)
+            kmax = n - 1 if n % 2 == 0 else n
+            for k in range(kmax + 1):
+                exact = 2.0 / (k + 1) if k % 2 == 0 else 0.0
+                assert abs(np.dot(w, x ** k) - exact) < 1e-14, (rule.__name__, n, k)
+    x, w = oracle.clenshaw_curtis(n)
+    assert x[0] == 1.0 and x[-1] == -1.0
+
+
+def test_regular_rules_small_cases(oracle):
+    """Hand values: Fejer-1 on 2 nodes is the 2-point midpoint rule in theta with
+    weights 1, 1; Clenshaw-Curtis on 3 nodes is Simpson's rule (1/3, 4/3, 1/3)."""
+    x, w = oracle.fejer1(2)
+    assert np.allclose(w, [1.0, 1.0], atol=1e-15)
+    x, w = oracle.clenshaw_curtis(3)
+    assert np.allclose(w, [1 / 3, 4 / 3, 1 / 3], atol=1e-15) and np.allclose(x, [1.0, 0.0, -1.0], atol=1e-16)
+    x, w = oracle.clenshaw_curtis(2)
+    assert np.allclose(w, [1.0, 1.0], atol=1e-15)
+
+
+@pytest.mark.parametrize("fid", ["packed_storage", "qst_batch", "sphtor_l", "qst_l"])
+def test_fixture_regular_grid_synthesis(oracle, fid):
+    """SHTns 3.7 syntheses on regular grids: Fejer midpoints (sphtor_l, fourpi,
+    no Condon-Shortley phase, ltr = 2) and the pole-inclusive grid (packed_storage,
+    qst_batch with real_norm and two fields), whose pole rows take the order-1
+    limits of dP/dtheta and P/sin(theta)."""
+    fx = FX.fixture(fid)
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    ltr = fx["ltr"]
+    tol = (fx["atol"], fx["rtol"])
+    if "coefficients" in p:
+        assert FX.isapprox(cfg.synthesis(FX.unpack(fx, p["coefficients"]), ltr=ltr), p["field"], *tol)[0]
+    if "Q" in p:
+        assert FX.isapprox(cfg.synthesis(FX.unpack(fx, p["Q"]), ltr=ltr), p["Vr"], *tol)[0]
+    if "S" in p:
+        Vt, Vp = cfg.synthesis_sphtor(FX.unpack(fx, p["S"]), FX.unpack(fx, p["T"]), ltr=ltr)
+        assert FX.isapprox(Vt, p["Vt"], *tol)[0]
+        assert FX.isapprox(Vp, p["Vp"], *tol)[0]
+
+
+def cplx_split(lmax, q):
+    """SHTns complex packed coefficients (index l (l + 1) + m, m = -l..l,
+    LM_cplx_index src/complex_packed.jl:63-72) -> (A, B) with
+    f = synth_cplx(A) + conj(synth_cplx(B)): A holds m >= 0, and since order -m
+    uses the row of |m| with exp(-i |m| phi) (src/complex_packed.jl:276-286),
+    i.e. Y_l^{-m} = conj(Y_l^m), B_lm = conj(a_{l,-m}) for m > 0."""
+    A = np.zeros((lmax + 1, lmax + 1), dtype=np.complex128, order="F")
+    B = np.zeros_like(A)
+    for l in range(lmax + 1):
+        for m in range(-l, l + 1):
+            v = q[l * (l + 1) + m]
+            if m >= 0:
+                A[l, m] = v
+            else:
+                B[l, -m] = np.conj(v)
+    return A, B
+
+
+def test_fixture_scalar_complex_full(oracle):
+    """Complex-valued synthesis (real_output = false, bins m >= 0) pinned on the
+    SHTns complex fixture through the split of the full m = -l..l sum."""
+    fx = FX.fixture("scalar_complex_full")
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    A, B = cplx_split(fx["lmax"], p["coefficients"])
+    f = cfg.synthesis(A, real_output=False) + np.conj(cfg.synthesis(B, real_output=False))
+    assert FX.isapprox(f, p["field"], fx["atol"], fx["rtol"])[0]
+
+
+@pytest.mark.parametrize("grid", ["regular", "regular_poles"])
+def test_regular_grid_round_trips(oracle, grid):
+    """With nlat = 2 lmax + 1 both rules integrate every product of band-limited
+    rows exactly, so analysis inverts synthesis (scalar and vector, the vector
+    one through the pole limits on the pole grid)."""
+    lmax = 11
+    cfg = oracle.Config(lmax, 2 * lmax + 1, nlon=24, grid=grid)
+    alm = oracle.random_alm(lmax, lmax, 2, 1.0)
+    alm[:, 0] = alm[:, 0].real
+    back = cfg.analysis(cfg.synthesis(alm))
+    assert np.abs(back - alm).max() < 1e-13
+    S = oracle.random_alm(lmax, lmax, 2, 1.0)
+    T = oracle.random_alm(lmax, lmax, 2, 1.0)
+    S[0] = T[0] = 0
+    S[:, 0] = S[:, 0].real
+    T[:, 0] = T[:, 0].real
+    Vt, Vp = cfg.synthesis_sphtor(S, T)
+    S2, T2 = cfg.analysis_sphtor(Vt, Vp)
+    assert np.abs(S2 - S).max() < 1e-12 and np.abs(T2 - T).max() < 1e-12

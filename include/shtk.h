@@ -87,6 +87,26 @@ typedef struct shtk_plan shtk_plan;
  * device; allocates device memory. */
 int shtk_plan_create(const shtk_desc *desc, shtk_plan **plan);
 
+/* Single-order transforms of stored order im (physical m = im * mres) for degrees
+ * m..ltr (ltr < 0: lmax; ltr >= m), with no phi transform.  Coefficient columns
+ * Ql (ltr - m + 1, nfields) and Fourier columns Vm (nlat, nfields) are complex
+ * (interleaved re/im of dtype).  Synthesis: Vm = phi_inv_scale * sum_l Ql' P_l^m;
+ * analysis: Ql = (the convention map of) phi_scale * sum_i w_i Vm_i P_l^m(x_i).
+ * Replace synthesis_packed_ml / analysis_packed_ml (src/transforms.jl:351-437). */
+int shtk_synthesis_ml(shtk_plan *plan, int im, const void *Ql, void *Vm, int nfields, int dtype, int ltr,
+                      void *stream);
+int shtk_analysis_ml(shtk_plan *plan, int im, const void *Vm, void *Ql, int nfields, int dtype, int ltr,
+                     void *stream);
+/* Vector pair of one order: Vt = dS/dtheta - (i m / sin) T, Vp = (i m / sin) S + dT/dtheta
+ * (times sin(theta) in Robert form), and the analysis that inverts it, with the order-1
+ * pole limits on grids with poles.  Replace synthesis_sphtor_ml / analysis_sphtor_ml
+ * (src/sphtor_transforms.jl:1188-1322); synthesis_sph_ml, synthesis_tor_ml,
+ * synthesis_grad_ml and the QST _ml pair (src/qst_transforms.jl:386-425) compose them. */
+int shtk_synthesis_sphtor_ml(shtk_plan *plan, int im, const void *Sl, const void *Tl, void *Vt, void *Vp,
+                             int nfields, int dtype, int ltr, void *stream);
+int shtk_analysis_sphtor_ml(shtk_plan *plan, int im, const void *Vt, const void *Vp, void *Sl, void *Tl,
+                            int nfields, int dtype, int ltr, void *stream);
+
 /* Frees the plan and its workspace (gpu_clear_cache!, ext/GPUCommon.jl:202-441). */
 int shtk_plan_destroy(shtk_plan *plan);
 

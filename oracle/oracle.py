@@ -179,6 +179,77 @@ class Config:
     def coef_scale(self, l, m):
         return lib().ora_coef_scale(self.cref, l, m)
 
+    # --- single order (pure-Python loops over the oracle's Legendre rows) -----
+    def _phi_inv_scale(self):
+        return self.nlon / (2 * math.pi) if self.phi_scale == "quad" else float(self.nlon)
+
+    def synthesis_ml(self, im, Ql, ltr):
+        """synthesis_packed_ml (src/transforms.jl:394-437): Vr_m[i] = phi_inv_scale *
+        sum_{l=m}^{ltr} conv(l, m) Q_l P_l^m(x_i)."""
+        m = im * self.mres
+        Q = np.array([Ql[l - m] * self.coef_scale(l, m) for l in range(m, ltr + 1)])
+        out = np.zeros(self.nlat, dtype=np.complex128)
+        for i in range(self.nlat):
+            P = plm_row(self.x[i], ltr, m)
+            out[i] = np.dot(Q, P[m:]) * self._phi_inv_scale()
+        return out
+
+    def analysis_ml(self, im, Vm, ltr):
+        """analysis_packed_ml (src/transforms.jl:351-385): Q_l = phi_scale sum_i w_i V_i
+        P_l^m(x_i), then the convention map."""
+        m = im * self.mres
+        sphi = (2 * math.pi / self.nlon) * self.nlon / self._phi_inv_scale()
+        Q = np.zeros(ltr - m + 1, dtype=np.complex128)
+        for i in range(self.nlat):
+            P = plm_row(self.x[i], ltr, m)
+            Q += self.w[i] * Vm[i] * P[m:]
+        return np.array([Q[l - m] * sphi / self.coef_scale(l, m) for l in range(m, ltr + 1)])
+
+    def synthesis_sphtor_ml(self, im, Sl, Tl, ltr):
+        """synthesis_sphtor_ml (src/sphtor_transforms.jl:1264-1322): Vt = dS/dtheta -
+        i m (P/s) T, Vp = i m (P/s) S + dT/dtheta, times sin(theta) in Robert form."""
+        m = im * self.mres
+        S = [Sl[l - m] * self.coef_scale(l, m) for l in range(m, ltr + 1)]
+        T = [Tl[l - m] * self.coef_scale(l, m) for l in range(m, ltr + 1)]
+        Vt = np.zeros(self.nlat, dtype=np.complex128)
+        Vp = np.zeros(self.nlat, dtype=np.complex128)
+        for i in range(self.nlat):
+            x = self.x[i]
+            st = math.sqrt(max(0.0, 1 - x * x))
+            _, dP, Ps = plm_dtheta_over_sinth_row(x, ltr, m)
+            gt = gp = 0j
+            for l in range(max(1, m), ltr + 1):
+                term = 1j * m * Ps[l]
+                gt += dP[l] * S[l - m] - term * T[l - m]
+                gp += term * S[l - m] + dP[l] * T[l - m]
+            if self.robert_form:
+                gt *= st
+                gp *= st
+            Vt[i] = gt * self._phi_inv_scale()
+            Vp[i] = gp * self._phi_inv_scale()
+        return Vt, Vp
+
+    def analysis_sphtor_ml(self, im, Vt, Vp, ltr):
+        """analysis_sphtor_ml (src/sphtor_transforms.jl:1188-1243)."""
+        m = im * self.mres
+        sphi = (2 * math.pi / self.nlon) * self.nlon / self._phi_inv_scale()
+        S = np.zeros(ltr - m + 1, dtype=np.complex128)
+        T = np.zeros(ltr - m + 1, dtype=np.complex128)
+        for i in range(self.nlat):
+            x = self.x[i]
+            st = math.sqrt(max(0.0, 1 - x * x))
+            ft, fp = Vt[i], Vp[i]
+            if self.robert_form and st > 0:
+                ft, fp = ft / st, fp / st
+            _, dP, Ps = plm_dtheta_over_sinth_row(x, ltr, m)
+            for l in range(max(1, m), ltr + 1):
+                c = self.w[i] * sphi / (l * (l + 1))
+                term = 1j * m * Ps[l]
+                S[l - m] += c * (ft * dP[l] + np.conj(term) * fp)
+                T[l - m] += c * (-np.conj(term) * ft + dP[l] * fp)
+        cs = np.array([self.coef_scale(l, m) for l in range(m, ltr + 1)])
+        return S / cs, T / cs
+
     # --- shapes ---------------------------------------------------------
     def _nf(self, a, lead):
         return 1 if a.ndim == lead else a.shape[lead]

@@ -14,7 +14,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_NAME = "libshtk_mi355x.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
-SOURCES = ["shtk_plan.cpp", "shtk_legendre.hip", "shtk_fft.hip", "shtk_split.hip"]
+SOURCES = ["shtk_plan.cpp", "shtk_legendre.hip", "shtk_fft.hip", "shtk_split.hip", "shtk_ml.hip"]
 ARCH = os.environ.get("SHTK_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 instructions are disabled for all device code; the host pass

@@ -21,6 +21,7 @@ EXPORTS = (
     "shtk_plan_create", "shtk_plan_destroy", "shtk_plan_nodes", "shtk_workspace_size",
     "shtk_synthesis", "shtk_analysis", "shtk_synthesis_ex", "shtk_analysis_ex", "shtk_synthesis_sphtor", "shtk_analysis_sphtor",
     "shtk_synthesis_sphtor_ex", "shtk_analysis_sphtor_ex",
+    "shtk_synthesis_ml", "shtk_analysis_ml", "shtk_synthesis_sphtor_ml", "shtk_analysis_sphtor_ml",
     "shtk_adjoint_synthesis", "shtk_adjoint_analysis", "shtk_adjoint_synthesis_sphtor",
     "shtk_adjoint_analysis_sphtor", "shtk_batch_create", "shtk_batch_destroy", "shtk_batch_plan",
     "shtk_batch_set_graph", "shtk_batch_set_tiles", "shtk_batch_synthesis", "shtk_batch_analysis",
@@ -73,6 +74,10 @@ def load():
     lib.shtk_analysis_sphtor.argtypes = [p, p, p, p, p, i, i, i, p]
     lib.shtk_synthesis_sphtor_ex.argtypes = [p, p, p, p, p, i, i, i, i, i, p]
     lib.shtk_analysis_sphtor_ex.argtypes = [p, p, p, p, p, i, i, i, i, p]
+    lib.shtk_synthesis_ml.argtypes = [p, i, p, p, i, i, i, p]
+    lib.shtk_analysis_ml.argtypes = [p, i, p, p, i, i, i, p]
+    lib.shtk_synthesis_sphtor_ml.argtypes = [p, i, p, p, p, p, i, i, i, p]
+    lib.shtk_analysis_sphtor_ml.argtypes = [p, i, p, p, p, p, i, i, i, p]
     lib.shtk_adjoint_synthesis.argtypes = [p, p, p, i, i, i, p]
     lib.shtk_adjoint_analysis.argtypes = [p, p, p, i, i, p]
     lib.shtk_adjoint_synthesis_sphtor.argtypes = [p, p, p, p, p, i, i, i, p]

@@ -133,6 +133,7 @@ struct shtk_plan {
     bool poles = false;  // nodes at theta = 0 and pi (the first pair)
     double inv_scale = 0, scale_phi = 0;
     std::vector<double> hx, hw;  // north-first nodes and weights (nlat)
+    std::vector<long long> hcoff;  // host copy of coff: first table entry of each order
     int device = 0;
 
     // device tables
@@ -290,6 +291,30 @@ hipError_t launch_vspec_prep(const void *S, const void *T, double *rec, int in_f
                              int ll1, const double *qcc, const long long *coff, int lmax, int mres, int nm,
                              long long per_field, int nfields, hipStream_t st);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
+
+// Single-order transforms (shtk_ml.hip): one stored order, coefficient columns
+// [field][l - m] (ld_spec = ltr - m + 1) and Fourier columns [field][lat] (ld_grid = nlat).
+struct MlArgs {
+    int lmax, m, ltr, nlat, np;
+    int vector;          // 1: spheroidal/toroidal pair
+    int robert;          // vector rows in Robert form
+    int io_f32;          // complex float inputs and outputs
+    const double *x;     // [np] cos(theta) of the north nodes
+    const double *s;     // [np] sin(theta) of the north nodes
+    const double *w;     // [np] weights times the phi scale (analysis)
+    const double2 *ab;   // (a_l, b_l) of this order at index l - m
+    const double *gtab;  // g_l of this order at index l - m
+    const double *sectv; // [np] scaled sectoral seeds of this order
+    const int *sectk;
+    const double *conv;  // convention factors indexed by l, or nullptr (canonical)
+    double g;            // synthesis: phi inverse scale; analysis: 1
+    const void *in0, *in1;
+    void *out0, *out1;
+    double *acc;         // analysis: FP64 sums [field][l - m][4], cleared first
+    long long ld_spec, ld_grid;
+};
+hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st);
+hipError_t launch_ml_anal(const MlArgs &a, int nfields, hipStream_t st);
 // Pole rows of order 1 for the vector transforms on grids with nodes at the
 // poles (one block per field): synthesis rewrites them from the prologue
 // records in a.in0 (robert = 1: zeros every order's pole rows), analysis adds

@@ -1,0 +1,255 @@
+// Single-order transforms: the Legendre sums of one stored order between a
+// coefficient column (degrees m..ltr) and one Fourier column (nlat latitudes),
+// with no phi transform -- analysis_packed_ml / synthesis_packed_ml
+// (src/transforms.jl:351-437) and analysis_sphtor_ml / synthesis_sphtor_ml
+// (src/sphtor_transforms.jl:1188-1322).
+//
+// One lane per latitude pair (north p, south nlat-1-p).  Each lane runs the
+// orthonormal P-form recurrence P_l = a_l x P_{l-1} - b_l P_{l-2} from the
+// plan's scaled sectoral seed (value * 2^(-512 k), live at k = 0, rescaled by
+// 2^-512 when it passes 2^100 every 8 degrees), and D_l = s dP_l/dtheta =
+// l x P_l - g_l P_{l-1}.  South values follow from the parity (-1)^(l-m).
+// Vector rows use dP/dtheta = D/s and P/s; at exact poles (s = 0) order 1 takes
+// the limits -h_l (north), h_l = sqrt((2l+1) l (l+1) / (4 pi)) / 2, and other
+// orders 0 (src/kernels.jl:72-99).  Analysis sums reduce over the wavefront and
+// are added to an FP64 accumulator; a second kernel applies the convention
+// map (and 1/(l(l+1)) for the vector pair) and writes the output dtype.
+#include "shtk_internal.h"
+
+namespace shtk {
+
+namespace {
+
+constexpr double kHiM = 0x1p100;
+constexpr double kDownM = 0x1p-512;
+
+__device__ __forceinline__ double ml_pole_h(int l) {
+    return 0.5 * 0.28209479177387814 * sqrt((2.0 * l + 1.0) * (double)l * (double)(l + 1));
+}
+
+// Recurrence state of one lane; step() moves from degree l to l + 1.
+struct MlRec {
+    double x, cur, prev;
+    int k;
+    __device__ __forceinline__ void step(const MlArgs &a, int l) {
+        const double2 co = a.ab[l + 1 - a.m];
+        const double nx = fma(co.x * x, cur, -(co.y * prev));
+        prev = cur;
+        cur = nx;
+        if (k > 0 && ((l - a.m) & (SCALE_CHECK - 1)) == SCALE_CHECK - 1 && fmax(fabs(cur), fabs(prev)) > kHiM) {
+            cur *= kDownM;
+            prev *= kDownM;
+            --k;
+        }
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ double2 ld_c(const T *p, size_t i) {
+    return make_double2((double)p[2 * i], (double)p[2 * i + 1]);
+}
+
+template <typename T>
+__device__ __forceinline__ void st_c2(T *p, size_t i, double re, double im) {
+    p[2 * i] = (T)re;
+    p[2 * i + 1] = (T)im;
+}
+
+// Order 0: dP_l/dtheta from the differentiated recurrence d_l = sqrt((2l+1)/(2l-1))
+// (x d_{l-1} - l s P_{l-1}) (_zonal_dtheta_row!, src/legendre.jl:331-338), which
+// keeps its accuracy near the poles where l x P_l - g_l P_{l-1} cancels.
+__device__ __forceinline__ double ml_zonal_step(int l, double x, double s, double Pm1, double d) {
+    return sqrt((2.0 * l + 1.0) / (2.0 * l - 1.0)) * fma(x, d, -((double)l * s * Pm1));
+}
+
+// (dn, qn): dP/dtheta and P/sin(theta) at the north node of the lane (Robert
+// form synthesis: D and P themselves, zero at the poles); dz: the order-0
+// derivative of this degree.
+__device__ __forceinline__ void ml_vrow(const MlArgs &a, int l, double P, double Pm1, double x, double s,
+                                        double invs, double dz, bool pole, bool robert_syn, double &dn,
+                                        double &qn) {
+    const double D = a.m == 0 ? s * dz : fma((double)l * x, P, -(a.gtab[l - a.m] * Pm1));
+    if (a.m == 0 && !robert_syn) {
+        dn = dz;
+        qn = P * invs;
+        return;
+    }
+    if (pole) {
+        const double v = (a.m == 1 && !robert_syn) ? -ml_pole_h(l) : 0.0;
+        dn = v;
+        qn = v;
+    } else if (robert_syn) {
+        dn = D;
+        qn = P;
+    } else {
+        dn = D * invs;
+        qn = P * invs;
+    }
+}
+
+}  // namespace
+
+template <typename IT, typename OT>
+__global__ __launch_bounds__(64) void k_ml_synth(const MlArgs a) {
+    const int p = blockIdx.x * 64 + threadIdx.x, f = blockIdx.y;
+    if (p >= a.np) return;  // no cross-lane operations in this kernel
+    const int m = a.m;
+    MlRec r{a.x[p], a.sectv[p], 0.0, a.sectk[p]};
+    const double s = a.s[p];
+    const bool pole = s == 0.0;
+    const double invs = pole ? 0.0 : 1.0 / s;
+    const IT *in0 = (const IT *)a.in0 + 2 * (size_t)f * a.ld_spec;
+    const IT *in1 = a.vector ? (const IT *)a.in1 + 2 * (size_t)f * a.ld_spec : nullptr;
+    double2 n0 = make_double2(0.0, 0.0), s0 = n0, n1 = n0, s1 = n0;
+    const double mm = (double)m;
+    double dz = 0.0;
+    for (int l = m; l <= a.ltr; ++l) {
+        const double live = r.k == 0 ? 1.0 : 0.0;
+        const double P = r.cur * live, Pm1 = r.prev * live;
+        if (m == 0 && l > 0) dz = ml_zonal_step(l, r.x, s, Pm1, dz);
+        const double c = a.conv ? a.conv[l] : 1.0;
+        const double sg = ((l - m) & 1) ? -1.0 : 1.0;
+        if (!a.vector) {
+            const double2 q = ld_c(in0, (size_t)(l - m));
+            const double v = c * P;
+            n0.x = fma(v, q.x, n0.x);
+            n0.y = fma(v, q.y, n0.y);
+            s0.x = fma(sg * v, q.x, s0.x);
+            s0.y = fma(sg * v, q.y, s0.y);
+        } else if (l >= 1) {
+            double dn, qn;
+            ml_vrow(a, l, P, Pm1, r.x, s, invs, dz, pole, a.robert, dn, qn);
+            const double2 S = ld_c(in0, (size_t)(l - m)), T = ld_c(in1, (size_t)(l - m));
+            const double Sr = c * S.x, Si = c * S.y, Tr = c * T.x, Ti = c * T.y;
+            const double mq = mm * qn;
+            // Vt = d S - i m q T, Vp = i m q S + d T; south: d -> -sg d, q -> sg q
+            n0.x += dn * Sr + mq * Ti;
+            n0.y += dn * Si - mq * Tr;
+            n1.x += -mq * Si + dn * Tr;
+            n1.y += mq * Sr + dn * Ti;
+            s0.x += sg * (-dn * Sr + mq * Ti);
+            s0.y += sg * (-dn * Si - mq * Tr);
+            s1.x += sg * (-mq * Si - dn * Tr);
+            s1.y += sg * (mq * Sr - dn * Ti);
+        }
+        if (l < a.ltr) r.step(a, l);
+    }
+    const double g = a.g;  // Robert form: the sums above already carry sin(theta)
+    OT *o0 = (OT *)a.out0 + 2 * (size_t)f * a.ld_grid;
+    OT *o1 = a.vector ? (OT *)a.out1 + 2 * (size_t)f * a.ld_grid : nullptr;
+    const int ps = a.nlat - 1 - p;
+    st_c2(o0, p, g * n0.x, g * n0.y);
+    if (a.vector) st_c2(o1, p, g * n1.x, g * n1.y);
+    if (ps != p) {
+        st_c2(o0, ps, g * s0.x, g * s0.y);
+        if (a.vector) st_c2(o1, ps, g * s1.x, g * s1.y);
+    }
+}
+
+template <typename IT>
+__global__ __launch_bounds__(64) void k_ml_anal(const MlArgs a) {
+    const int p0 = blockIdx.x * 64 + threadIdx.x, f = blockIdx.y;
+    const bool act = p0 < a.np;
+    const int p = act ? p0 : 0;
+    const int m = a.m;
+    MlRec r{a.x[p], act ? a.sectv[p] : 0.0, 0.0, act ? a.sectk[p] : 0};
+    const double s = a.s[p];
+    const bool pole = s == 0.0;
+    const double invs = pole ? 0.0 : 1.0 / s;
+    const int ps = a.nlat - 1 - p;
+    const double wl = act ? a.w[p] : 0.0;
+    // weighted inputs at the north and south rows (Robert form: divided by s off the poles)
+    const double rs = (a.vector && a.robert && !pole) ? invs : 1.0;
+    double2 vN0 = make_double2(0.0, 0.0), vS0 = vN0, vN1 = vN0, vS1 = vN0;
+    {
+        const IT *in0 = (const IT *)a.in0 + 2 * (size_t)f * a.ld_grid;
+        vN0 = ld_c(in0, p);
+        if (ps != p) vS0 = ld_c(in0, ps);
+        if (a.vector) {
+            const IT *in1 = (const IT *)a.in1 + 2 * (size_t)f * a.ld_grid;
+            vN1 = ld_c(in1, p);
+            if (ps != p) vS1 = ld_c(in1, ps);
+        }
+        const double sc = wl * rs;
+        vN0.x *= sc; vN0.y *= sc; vS0.x *= sc; vS0.y *= sc;
+        vN1.x *= sc; vN1.y *= sc; vS1.x *= sc; vS1.y *= sc;
+    }
+    double *acc = a.acc + 4 * (size_t)f * (a.ltr - m + 1);
+    const double mm = (double)m;
+    double dz = 0.0;
+    for (int l = m; l <= a.ltr; ++l) {
+        const double live = r.k == 0 ? 1.0 : 0.0;
+        const double P = r.cur * live, Pm1 = r.prev * live;
+        if (m == 0 && l > 0) dz = ml_zonal_step(l, r.x, s, Pm1, dz);
+        const double sg = ((l - m) & 1) ? -1.0 : 1.0;
+        double v[4];
+        if (!a.vector) {
+            v[0] = P * (vN0.x + sg * vS0.x);
+            v[1] = P * (vN0.y + sg * vS0.y);
+            v[2] = v[3] = 0.0;
+        } else {
+            double dn = 0.0, qn = 0.0;
+            if (l >= 1) ml_vrow(a, l, P, Pm1, r.x, s, invs, dz, pole, false, dn, qn);
+            // north (d, q) and south (-sg d, sg q) rows:
+            // S += d Vt - i m q Vp,  T += i m q Vt + d Vp
+            const double mq = mm * qn;
+            const double tdx = vN0.x - sg * vS0.x, tdy = vN0.y - sg * vS0.y;  // Vt under d
+            const double tqx = vN0.x + sg * vS0.x, tqy = vN0.y + sg * vS0.y;  // Vt under q
+            const double pdx = vN1.x - sg * vS1.x, pdy = vN1.y - sg * vS1.y;  // Vp under d
+            const double pqx = vN1.x + sg * vS1.x, pqy = vN1.y + sg * vS1.y;  // Vp under q
+            v[0] = dn * tdx + mq * pqy;
+            v[1] = dn * tdy - mq * pqx;
+            v[2] = -mq * tqy + dn * pdx;
+            v[3] = mq * tqx + dn * pdy;
+        }
+        const int nv = a.vector ? 4 : 2;
+        for (int q = 0; q < nv; ++q) {
+            double t = v[q];
+#pragma unroll
+            for (int msk = 32; msk >= 1; msk >>= 1) t += __shfl_xor(t, msk);
+            if (threadIdx.x == 0) atomicAdd(acc + 4 * (size_t)(l - m) + q, t);
+        }
+        if (l < a.ltr) r.step(a, l);
+    }
+}
+
+template <typename OT>
+__global__ __launch_bounds__(256) void k_ml_fin(const MlArgs a) {
+    const int f = blockIdx.x, m = a.m, nl = a.ltr - m + 1;
+    const double *acc = a.acc + 4 * (size_t)f * nl;
+    OT *o0 = (OT *)a.out0 + 2 * (size_t)f * a.ld_spec;
+    OT *o1 = a.vector ? (OT *)a.out1 + 2 * (size_t)f * a.ld_spec : nullptr;
+    for (int t = threadIdx.x; t < nl; t += 256) {
+        const int l = m + t;
+        double sc = a.g;
+        if (a.conv) sc /= a.conv[l];
+        if (a.vector) sc = l > 0 ? sc / ((double)l * (l + 1)) : 0.0;
+        st_c2(o0, t, sc * acc[4 * t], sc * acc[4 * t + 1]);
+        if (a.vector) st_c2(o1, t, sc * acc[4 * t + 2], sc * acc[4 * t + 3]);
+    }
+}
+
+hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st) {
+    const dim3 grid((unsigned)((a.np + 63) / 64), (unsigned)nfields);
+    if (a.io_f32)
+        hipLaunchKernelGGL((k_ml_synth<float, float>), grid, dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_ml_synth<double, double>), grid, dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ml_anal(const MlArgs &a, int nfields, hipStream_t st) {
+    const dim3 grid((unsigned)((a.np + 63) / 64), (unsigned)nfields);
+    if (a.io_f32) {
+        hipLaunchKernelGGL(k_ml_anal<float>, grid, dim3(64), 0, st, a);
+        if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
+        hipLaunchKernelGGL(k_ml_fin<float>, dim3((unsigned)nfields), dim3(256), 0, st, a);
+    } else {
+        hipLaunchKernelGGL(k_ml_anal<double>, grid, dim3(64), 0, st, a);
+        if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
+        hipLaunchKernelGGL(k_ml_fin<double>, dim3((unsigned)nfields), dim3(256), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace shtk

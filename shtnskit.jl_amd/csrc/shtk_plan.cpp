@@ -1315,6 +1315,7 @@ static int plan_tables(shtk_plan *p) {
     RC_TRY(upload(&p->ab, ab, "plan tables"));
     RC_TRY(upload(&p->gtab, gt, "plan tables"));
     RC_TRY(upload(&p->coff, coff, "plan tables"));
+    p->hcoff = coff;
     RC_TRY(upload(&p->segbase, segbase, "plan tables"));
     RC_TRY(upload(&p->segm, segm, "plan tables"));
     RC_TRY(upload(&p->convscale, conv, "plan tables"));
@@ -1588,6 +1589,103 @@ int shtk_analysis_sphtor_ex(shtk_plan *p, const void *Vt, const void *Vp, void *
     Job j{1, 1, p->d.robert_form ? p->winvs2 : p->winvs, 1.0, 0, 1, 0, ltr, "oz_anal_sphtor", "fft_anal"};
     j.prec = precision;
     return run_vsplit_to_spec(p, j, Vt, Vp, S, T, nfields, dtype, stream);
+}
+
+// --- single order ------------------------------------------------------------
+
+// Checks of _validate_stored_order (src/sphtor_transforms.jl:1156-1169) and
+// synthesis_packed_ml (src/transforms.jl:394-404); fills the order's tables.
+static int ml_args(shtk_plan *p, int im, int ltr, int nfields, int dtype, int vector, shtk::MlArgs *a) {
+    if (!p) return fail(SHTK_EINVAL, "plan is NULL");
+    if (nfields < 1) return fail(SHTK_EINVAL, "nfields must be >= 1");
+    if (dtype != SHTK_F64 && dtype != SHTK_F32) return fail(SHTK_EINVAL, "dtype must be SHTK_F64 or SHTK_F32");
+    if (im < 0 || im > p->d.mmax / p->d.mres)
+        return fail(SHTK_EINVAL, "im must satisfy 0 <= im <= mmax/mres = " + std::to_string(p->d.mmax / p->d.mres));
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    const int m = im * p->d.mres;
+    if (ltr < m) return fail(SHTK_EINVAL, "ltr must satisfy ltr >= im*mres = " + std::to_string(m));
+    shtk::MlArgs &A = *a;
+    A = shtk::MlArgs{};
+    A.lmax = p->d.lmax;
+    A.m = m;
+    A.ltr = ltr;
+    A.nlat = p->d.nlat;
+    A.np = p->np;
+    A.vector = vector;
+    A.robert = p->d.robert_form;
+    A.io_f32 = dtype == SHTK_F32;
+    A.x = p->x;
+    A.s = p->s;
+    A.w = p->wscaled;
+    A.ab = p->ab + p->hcoff[im];
+    A.gtab = p->gtab + p->hcoff[im];
+    A.sectv = p->sectv + (size_t)im * p->np;
+    A.sectk = p->sectk + (size_t)im * p->np;
+    A.conv = p->canonical ? nullptr : p->convscale + (size_t)(p->d.lmax + 1) * m;
+    A.ld_spec = ltr - m + 1;
+    A.ld_grid = p->d.nlat;
+    return SHTK_OK;
+}
+
+static int ml_synth(shtk_plan *p, int im, const void *S, const void *T, void *V0, void *V1, int nfields, int dtype, int ltr,
+             int vector, void *stream) {
+    shtk::MlArgs a;
+    RC_TRY(ml_args(p, im, ltr, nfields, dtype, vector, &a));
+    if (!S || !V0 || (vector && (!T || !V1))) return fail(SHTK_EINVAL, "NULL array argument");
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->device);
+    a.g = p->inv_scale;
+    a.in0 = S;
+    a.in1 = T;
+    a.out0 = V0;
+    a.out1 = V1;
+    HIP_TRY(shtk::launch_ml_synth(a, nfields, (hipStream_t)stream), "single-order synthesis launch");
+    return SHTK_OK;
+}
+
+static int ml_anal(shtk_plan *p, int im, const void *V0, const void *V1, void *S, void *T, int nfields, int dtype, int ltr,
+            int vector, void *stream) {
+    shtk::MlArgs a;
+    RC_TRY(ml_args(p, im, ltr, nfields, dtype, vector, &a));
+    if (!S || !V0 || (vector && (!T || !V1))) return fail(SHTK_EINVAL, "NULL array argument");
+    // Robert-form rows cannot be divided by sin(theta) at weighted pole nodes for
+    // order 1 (_validate_robert_analysis with m, src/config.jl:608-624)
+    if (vector && p->poles && p->d.robert_form && a.m == 1 && a.ltr >= 1)
+        return fail(SHTK_EINVAL, "Robert-form components with m=1 cannot be unweighted at nonzero-weight pole "
+                                 "nodes; use a Gauss or pole-free regular grid");
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->device);
+    hipStream_t st = (hipStream_t)stream;
+    const size_t acc_bytes = sizeof(double) * 4 * (size_t)nfields * a.ld_spec;
+    RC_TRY(ensure_ws(p, acc_bytes));
+    a.g = 1.0;
+    a.in0 = V0;
+    a.in1 = V1;
+    a.out0 = S;
+    a.out1 = T;
+    a.acc = (double *)p->ws;
+    HIP_TRY(shtk::launch_zero(a.acc, acc_bytes, st), "accumulator clear");
+    HIP_TRY(shtk::launch_ml_anal(a, nfields, st), "single-order analysis launch");
+    return SHTK_OK;
+}
+
+int shtk_synthesis_ml(shtk_plan *p, int im, const void *Ql, void *Vm, int nfields, int dtype, int ltr,
+                      void *stream) {
+    return ml_synth(p, im, Ql, nullptr, Vm, nullptr, nfields, dtype, ltr, 0, stream);
+}
+
+int shtk_analysis_ml(shtk_plan *p, int im, const void *Vm, void *Ql, int nfields, int dtype, int ltr, void *stream) {
+    return ml_anal(p, im, Vm, nullptr, Ql, nullptr, nfields, dtype, ltr, 0, stream);
+}
+
+int shtk_synthesis_sphtor_ml(shtk_plan *p, int im, const void *Sl, const void *Tl, void *Vt, void *Vp, int nfields,
+                             int dtype, int ltr, void *stream) {
+    return ml_synth(p, im, Sl, Tl, Vt, Vp, nfields, dtype, ltr, 1, stream);
+}
+
+int shtk_analysis_sphtor_ml(shtk_plan *p, int im, const void *Vt, const void *Vp, void *Sl, void *Tl, int nfields,
+                            int dtype, int ltr, void *stream) {
+    return ml_anal(p, im, Vt, Vp, Sl, Tl, nfields, dtype, ltr, 1, stream);
 }
 
 int shtk_adjoint_synthesis_sphtor(shtk_plan *p, const void *Vtb, const void *Vpb, void *Sb, void *Tb, int nfields,

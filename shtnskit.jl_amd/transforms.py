@@ -337,6 +337,92 @@ def synthesis_tor_l(cfg, T, ltr: int, real_output: bool = True):
     return synthesis_sphtor_l(cfg, torch.zeros_like(T), T, ltr, real_output=real_output)
 
 
+# --------------------------------------------------------------------------- single order
+def _ml_ltr(cfg: SHTConfig, im: int, ltr, op: str) -> tuple[int, int]:
+    """(m, ltr) after the checks of _validate_stored_order (src/sphtor_transforms.jl:1156-1169)."""
+    im = int(im)
+    if not 0 <= im <= cfg.mmax // cfg.mres:
+        raise ValueError(f"{op}: im must satisfy 0 <= im <= {cfg.mmax // cfg.mres}")
+    ltr = cfg.lmax if ltr is None or int(ltr) < 0 else int(ltr)
+    if ltr > cfg.lmax:
+        raise ValueError(f"{op}: ltr must be <= lmax={cfg.lmax}")
+    m = im * cfg.mres
+    if ltr < m:
+        raise ValueError(f"{op}: ltr must satisfy ltr >= im*mres={m}")
+    return m, ltr
+
+
+def _col_in(x, n: int, op: str, what: str):
+    """A complex column (n,) or a batch of columns (n, nf) on the device, column-major."""
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise BackendUnavailableError(op, "inputs must be ROCm device tensors")
+    if x.dim() not in (1, 2) or x.shape[0] != n:
+        raise ValueError(f"{op}: {what} must have length {n}, got shape {tuple(x.shape)}")
+    rdt = _real_dtype(x)
+    return _colmajor(x.to(_cplx(rdt))), rdt
+
+
+def _ml_call(fn, cfg, im, ins, n_in, n_out, ltr, op):
+    m, ltr = _ml_ltr(cfg, im, ltr, op)
+    cols = [_col_in(x, n_in(m, ltr), op, "column") for x in ins]
+    rdt = torch.float32 if all(r == torch.float32 for _, r in cols) else torch.float64
+    ts = [_colmajor(t.to(_cplx(rdt))) for t, _ in cols]
+    if any(t.shape != ts[0].shape for t in ts):
+        raise ValueError(f"{op}: inputs must have equal shapes")
+    nf = 1 if ts[0].dim() == 1 else int(ts[0].shape[1])
+    shape = (n_out(m, ltr),) + (() if ts[0].dim() == 1 else (nf,))
+    outs = [_empty_colmajor(shape, _cplx(rdt), ts[0].device) for _ in ts]
+    args = [_ptr(t) for t in ts] + [_ptr(o) for o in outs]
+    _lib.check(fn(ctypes.c_void_p(cfg.plan), int(im), *args, nf, _code(rdt), ltr, _stream(ts[0].device)), op)
+    return outs
+
+
+def synthesis_packed_ml(cfg: SHTConfig, im: int, Ql, ltr: int):
+    """``synthesis_packed_ml(cfg, mval, Ql, ltr)`` (src/transforms.jl:394-437): one stored
+    order's coefficients (degrees m..ltr) -> its complex Fourier column (nlat,)."""
+    lib = _lib.load()
+    return _ml_call(lib.shtk_synthesis_ml, cfg, im, (Ql,), lambda m, l: l - m + 1, lambda m, l: cfg.nlat, ltr,
+                    "synthesis_packed_ml")[0]
+
+
+def analysis_packed_ml(cfg: SHTConfig, im: int, Vr_m, ltr: int):
+    """``analysis_packed_ml(cfg, mval, Vr_m, ltr)`` (src/transforms.jl:351-385)."""
+    lib = _lib.load()
+    return _ml_call(lib.shtk_analysis_ml, cfg, im, (Vr_m,), lambda m, l: cfg.nlat, lambda m, l: l - m + 1, ltr,
+                    "analysis_packed_ml")[0]
+
+
+def synthesis_sphtor_ml(cfg: SHTConfig, im: int, Sl, Tl, ltr: int):
+    """``synthesis_sphtor_ml(cfg, im, Sl, Tl, ltr)`` (src/sphtor_transforms.jl:1264-1322)."""
+    lib = _lib.load()
+    Vt, Vp = _ml_call(lib.shtk_synthesis_sphtor_ml, cfg, im, (Sl, Tl), lambda m, l: l - m + 1,
+                      lambda m, l: cfg.nlat, ltr, "synthesis_sphtor_ml")
+    return Vt, Vp
+
+
+def analysis_sphtor_ml(cfg: SHTConfig, im: int, Vt_m, Vp_m, ltr: int):
+    """``analysis_sphtor_ml(cfg, im, Vt_m, Vp_m, ltr)`` (src/sphtor_transforms.jl:1188-1243)."""
+    lib = _lib.load()
+    Sl, Tl = _ml_call(lib.shtk_analysis_sphtor_ml, cfg, im, (Vt_m, Vp_m), lambda m, l: cfg.nlat,
+                      lambda m, l: l - m + 1, ltr, "analysis_sphtor_ml")
+    return Sl, Tl
+
+
+def synthesis_sph_ml(cfg: SHTConfig, im: int, Sl, ltr: int):
+    """Spheroidal part of one order (``synthesis_sph_ml``, src/sphtor_transforms.jl:1125-1130): T = 0."""
+    return synthesis_sphtor_ml(cfg, im, Sl, torch.zeros_like(Sl), ltr)
+
+
+def synthesis_tor_ml(cfg: SHTConfig, im: int, Tl, ltr: int):
+    """Toroidal part of one order (``synthesis_tor_ml``, src/sphtor_transforms.jl:1143-1148): S = 0."""
+    return synthesis_sphtor_ml(cfg, im, torch.zeros_like(Tl), Tl, ltr)
+
+
+def synthesis_grad_ml(cfg: SHTConfig, im: int, Sl, ltr: int):
+    """``synthesis_grad_ml`` (src/sphtor_transforms.jl:1362-1364): the spheroidal synthesis."""
+    return synthesis_sph_ml(cfg, im, Sl, ltr)
+
+
 # --------------------------------------------------------------------------- QST
 def _qst_promoted(xs):
     """The three components in one precision, the widest of the inputs (``_qst_promoted``
@@ -459,3 +545,16 @@ def _adjoint_analysis_sphtor(cfg: SHTConfig, Sb, Tb):
                                                         _ptr(Vp), nf, _code(rdt), _stream(Sb.device)),
                "_adjoint_analysis_sphtor")
     return Vt, Vp
+
+
+def synthesis_qst_ml(cfg: SHTConfig, im: int, Ql, Sl, Tl, ltr: int):
+    """``synthesis_qst_ml(cfg, mval, Ql, Sl, Tl, ltr)`` (src/qst_transforms.jl:414-424):
+    the scalar and vector single-order syntheses in one promoted precision."""
+    Ql, Sl, Tl = _qst_promoted((Ql, Sl, Tl))
+    return (synthesis_packed_ml(cfg, im, Ql, ltr),) + tuple(synthesis_sphtor_ml(cfg, im, Sl, Tl, ltr))
+
+
+def analysis_qst_ml(cfg: SHTConfig, im: int, Vr_m, Vt_m, Vp_m, ltr: int):
+    """``analysis_qst_ml(cfg, mval, Vr_m, Vt_m, Vp_m, ltr)`` (src/qst_transforms.jl:386-397)."""
+    Vr_m, Vt_m, Vp_m = _qst_promoted((Vr_m, Vt_m, Vp_m))
+    return (analysis_packed_ml(cfg, im, Vr_m, ltr),) + tuple(analysis_sphtor_ml(cfg, im, Vt_m, Vp_m, ltr))

@@ -467,3 +467,46 @@ def test_regular_grid_round_trips(oracle, grid):
     Vt, Vp = cfg.synthesis_sphtor(S, T)
     S2, T2 = cfg.analysis_sphtor(Vt, Vp)
     assert np.abs(S2 - S).max() < 1e-12 and np.abs(T2 - T).max() < 1e-12
+
+
+# --------------------------------------------------------------------------- single order
+@pytest.mark.parametrize("fid", ["scalar_ml", "sphtor_ml", "qst_ml", "analysis_scalar_ml", "analysis_sphtor_ml",
+                                 "analysis_qst_ml"])
+def test_fixture_single_order(oracle, fid):
+    """SHTns 3.7 single-order transforms (stored order im, degrees m..ltr), with the
+    fixtures' fixed_mode_scale on the Fourier column as the reference's test applies
+    it (test/parity/shtns37_fixtures.jl:88-92, 240-247): regular grid with Schmidt,
+    real_norm and mres = 2, the pole grid for sphtor_ml, Gauss for the analyses."""
+    fx = FX.fixture(fid)
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    im, ltr, sc = fx["stored_im"], fx["ltr"], fx["fixed_mode_scale"]
+    tol = (fx["atol"], fx["rtol"])
+    syn = fx.get("direction") != "analysis"
+    col = lambda k: np.asarray(p[k]).ravel()
+    if "coefficients" in p or "Q" in p:
+        q, v = (col("coefficients"), col("field")) if "coefficients" in p else (col("Q"), col("Vr"))
+        if syn:
+            assert FX.isapprox(cfg.synthesis_ml(im, q, ltr), sc * v, *tol)[0]
+        assert FX.isapprox(cfg.analysis_ml(im, sc * v, ltr), q, *tol)[0]
+    if "S" in p:
+        if syn:
+            Vt, Vp = cfg.synthesis_sphtor_ml(im, col("S"), col("T"), ltr)
+            assert FX.isapprox(Vt, sc * col("Vt"), *tol)[0] and FX.isapprox(Vp, sc * col("Vp"), *tol)[0]
+        S, T = cfg.analysis_sphtor_ml(im, sc * col("Vt"), sc * col("Vp"), ltr)
+        assert FX.isapprox(S, col("S"), *tol)[0] and FX.isapprox(T, col("T"), *tol)[0]
+
+
+def test_single_order_against_full_transform(oracle):
+    """With only order m present the complex grid of the full synthesis is
+    F_m(theta) exp(i m phi), and the single-order column is phi_inv_scale (nlon)
+    times F_m, i.e. nlon times the grid at phi = 0."""
+    lmax, nlat, nlon = 10, 12, 24
+    cfg = oracle.Config(lmax, nlat, nlon=nlon, norm="schmidt", real_norm=True)
+    for m in (0, 3):
+        a = np.zeros((lmax + 1, lmax + 1), dtype=np.complex128, order="F")
+        rng = np.random.default_rng(m)
+        a[m:, m] = rng.standard_normal(lmax + 1 - m) + 1j * rng.standard_normal(lmax + 1 - m)
+        z = cfg.synthesis(a, real_output=False)
+        col = cfg.synthesis_ml(m, a[m:, m], lmax)
+        assert np.allclose(col / nlon, z[:, 0], atol=1e-13, rtol=0)

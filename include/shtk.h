@@ -45,8 +45,10 @@ enum shtk_phi_scale { SHTK_PHI_DFT = 0, SHTK_PHI_QUAD = 1 };
 /* Latitude grids of create_gauss_config and create_regular_config (src/config.jl:633-655,
  * 1235-1330): Gauss-Legendre nodes; Fejer-1 midpoints theta = (i + 1/2) pi / nlat
  * (include_poles = false); Clenshaw-Curtis nodes theta = i pi / (nlat - 1) with both
- * poles (include_poles = true). */
-enum shtk_grid { SHTK_GRID_GAUSS = 0, SHTK_GRID_REGULAR = 1, SHTK_GRID_REGULAR_POLES = 2 };
+ * poles (include_poles = true); Driscoll-Healy nodes theta = i pi / nlat with the
+ * weights of driscoll_healy_weights (src/mathutils.jl:149-183; use_dh_weights = true):
+ * the north pole is a node, the south pole is not. */
+enum shtk_grid { SHTK_GRID_GAUSS = 0, SHTK_GRID_REGULAR = 1, SHTK_GRID_REGULAR_POLES = 2, SHTK_GRID_DRISCOLL_HEALY = 3 };
 enum shtk_dtype { SHTK_F64 = 0, SHTK_F32 = 1 };
 
 /* Contraction precision of the batched entry points (*_ex).
@@ -63,8 +65,7 @@ enum shtk_precision { SHTK_PREC_FP64 = 0, SHTK_PREC_SPLIT = 1 };
  * (src/SHTnsKit.jl:141-177).  Constraints as src/config.jl:465-481 and the
  * grid minimums of _min_nlat (src/config.jl:452-458): nlat >= lmax+1 (Gauss,
  * regular with poles, and at least 2 there), nlat >= lmax+2 (regular),
- * nlon >= 2*mmax+1, 0 <= mmax <= lmax, mres >= 1.  The Driscoll-Healy grid
- * (use_dh_weights) is not provided. */
+ * nlat = 2(lmax+1) (Driscoll-Healy), nlon >= 2*mmax+1, 0 <= mmax <= lmax, mres >= 1. */
 typedef struct shtk_desc {
     int lmax, mmax, mres, nlat, nlon;
     int norm;        /* enum shtk_norm */

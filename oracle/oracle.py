@@ -143,6 +143,18 @@ def clenshaw_curtis(n):
     return x, w
 
 
+def driscoll_healy(n):
+    """Driscoll-Healy nodes theta_j = pi j / n, j < n (north pole included, south
+    pole not), north first, and weights (sqrt(8)/n sqrt(2)) sin(pi j/n)
+    sum_{l=0}^{n/2-1} sin((2l+1) pi j / n) / (2l+1) (driscoll_healy_weights,
+    src/mathutils.jl:149-183; create_regular_config src/config.jl:1258-1273)."""
+    j = np.arange(n)
+    th = np.pi * j / n
+    l = np.arange(n // 2)
+    s = (np.sin(np.outer(j, 2 * l + 1) * np.pi / n) / (2 * l + 1)).sum(axis=1)
+    return np.cos(th), (math.sqrt(8.0) / n * math.sqrt(2.0)) * np.sin(th) * s
+
+
 class Config:
     """Grid configuration mirroring ``create_gauss_config`` (src/config.jl:633-655)
     and ``create_regular_config`` (src/config.jl:1235-1330, grids "regular" and
@@ -154,7 +166,10 @@ class Config:
         if nlon is None:
             nlon = 2 * lmax + 1
             nlon += nlon & 1
-        min_nlat = {"gauss": lmax + 1, "regular": lmax + 2, "regular_poles": max(2, lmax + 1)}[grid]
+        min_nlat = {"gauss": lmax + 1, "regular": lmax + 2, "regular_poles": max(2, lmax + 1),
+                    "driscoll_healy": 2 * (lmax + 1)}[grid]
+        if grid == "driscoll_healy" and nlat != 2 * (lmax + 1):
+            raise ValueError("DH weights require nlat = 2 (lmax + 1)")
         if not (0 <= mmax <= lmax) or nlat < min_nlat or nlon < 2 * mmax + 1 or mres < 1:
             raise ValueError("invalid spectral layout")
         self.lmax, self.mmax, self.mres, self.nlat, self.nlon = lmax, mmax, mres, nlat, nlon
@@ -165,7 +180,7 @@ class Config:
             self.x = np.ascontiguousarray(xa[::-1])
             self.w = np.ascontiguousarray(wa[::-1])
         else:
-            xa, wa = fejer1(nlat) if grid == "regular" else clenshaw_curtis(nlat)
+            xa, wa = {"regular": fejer1, "regular_poles": clenshaw_curtis, "driscoll_healy": driscoll_healy}[grid](nlat)
             self.x = np.ascontiguousarray(xa)
             self.w = np.ascontiguousarray(wa)
         self._c = _CCfg(lmax, mmax, mres, nlat, nlon, NORMS[norm], int(cs_phase), int(real_norm),

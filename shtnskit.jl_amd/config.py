@@ -19,7 +19,7 @@ import numpy as np
 from . import _lib
 
 _NORMS = {"orthonormal": 0, "fourpi": 1, "schmidt": 2}
-_GRIDS = {"gauss": 0, "regular": 1, "regular_poles": 2}
+_GRIDS = {"gauss": 0, "regular": 1, "regular_poles": 2, "driscoll_healy": 3}
 
 
 def _min_nlat(grid_type: str, lmax: int) -> int:
@@ -70,12 +70,14 @@ class SHTConfig:
         if mres < 1:
             raise ValueError(f"mres must be ≥ 1, got {mres}")
         if grid_type not in _GRIDS:
-            raise ValueError(f"unsupported grid_type={grid_type}; this backend provides gauss, regular "
-                             "and regular_poles")
+            raise ValueError(f"unsupported grid_type={grid_type}; choose gauss, regular, regular_poles "
+                             "or driscoll_healy")
         if grid_type == "gauss" and nlat < lmax + 1:
             raise ValueError(f"Gauss–Legendre quadrature requires nlat ≥ lmax+1, got nlat={nlat} and lmax={lmax}")
         if grid_type != "gauss" and nlat < _min_nlat(grid_type, lmax):
             raise ValueError(f"nlat must be ≥ {_min_nlat(grid_type, lmax)} for regular grids")
+        if grid_type == "driscoll_healy" and (nlat % 2 or nlat != 2 * (lmax + 1)):
+            raise ValueError(f"DH weights require nlat=2*(lmax+1)={2 * (lmax + 1)}, got nlat={nlat}")
         if grid_type == "regular_poles" and nlat < 2:
             raise ValueError(f"pole-inclusive grids need nlat ≥ 2 (got nlat={nlat}); "
                              "use grid_type='regular' for a single-latitude grid")
@@ -177,15 +179,15 @@ def create_regular_config(lmax: int, nlat: int, *, include_poles: bool = False, 
     """``create_regular_config(lmax, nlat; ..., include_poles, precompute_plm, use_dh_weights)``.
 
     ``include_poles = False``: Fejer-1 midpoints ``theta_i = (i + 1/2) pi / nlat``;
-    ``True``: Clenshaw-Curtis nodes ``theta_i = i pi / (nlat - 1)``.  Legendre rows
-    are always computed on the fly, so ``precompute_plm`` has no effect.  The
-    Driscoll-Healy grid (``use_dh_weights = True``) is not provided.
+    ``True``: Clenshaw-Curtis nodes ``theta_i = i pi / (nlat - 1)``; with
+    ``use_dh_weights`` the Driscoll-Healy nodes ``theta_i = i pi / nlat`` (north pole
+    included, south pole not) and weights, ``nlat = 2 (lmax + 1)``.  Legendre rows
+    are always computed on the fly, so ``precompute_plm`` has no effect.
     """
     if use_dh_weights and not include_poles:
         raise ValueError("Driscoll-Healy weights require include_poles=true")
-    if use_dh_weights:
-        raise NotImplementedError("Driscoll-Healy grids are not provided by this backend")
-    return SHTConfig(lmax, nlat, grid_type="regular_poles" if include_poles else "regular", **kwargs)
+    grid = "driscoll_healy" if use_dh_weights else ("regular_poles" if include_poles else "regular")
+    return SHTConfig(lmax, nlat, grid_type=grid, **kwargs)
 
 
 def create_config(lmax: int, *, mmax: int | None = None, mres: int = 1, nlat: int | None = None,

@@ -261,13 +261,14 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
         }
         __syncthreads();
         fft_lds<+1>(zs + row * RS, a, lt, tr);
-        GT *gf = (GT *)a.grid + (size_t)f * nlat * N + lat;
-        if (live) {
+        const int nrow = a.nrow;
+        GT *gf = (GT *)a.grid + (size_t)f * nrow * N + lat;
+        if (live && lat < nrow) {
             const double sc = fft_row_scale(a, lat);
             for (int e = tid; e < N * L; e += nt) {
                 const int j = e / L;
                 const double2 v = z[zp(j)];
-                gf[(size_t)j * nlat] = (GT)((k < G ? v.x : v.y) * sc);
+                gf[(size_t)j * nrow] = (GT)((k < G ? v.x : v.y) * sc);
             }
         }
         __syncthreads();
@@ -299,14 +300,15 @@ __global__ __launch_bounds__(1024) void k_fft_synth_c(const FftArgs a, int nfiel
         }
         __syncthreads();
         fft_lds<+1>(zs + row * RS, a, lt, tr);
-        GT *gf = (GT *)a.grid + 2 * ((size_t)f * nlat * N + lat);
-        if (live) {
+        const int nrow = a.nrow;
+        GT *gf = (GT *)a.grid + 2 * ((size_t)f * nrow * N + lat);
+        if (live && lat < nrow) {
             const double sc = fft_row_scale(a, lat);
             for (int e = tid; e < N * G; e += nt) {
                 const int j = e / G;
                 const double2 v = z[zp(j)];
-                gf[2 * (size_t)j * nlat] = (GT)(v.x * sc);
-                gf[2 * (size_t)j * nlat + 1] = (GT)(v.y * sc);
+                gf[2 * (size_t)j * nrow] = (GT)(v.x * sc);
+                gf[2 * (size_t)j * nrow + 1] = (GT)(v.y * sc);
             }
         }
         __syncthreads();
@@ -332,7 +334,9 @@ __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfield
         const int f = ug / T.ntile, lat = (ug % T.ntile) * L + k;
         const bool live = lat < nlat;
         const double sc = live ? fft_row_scale(a, lat) : 0.0;
-        const GT *gf = (const GT *)a.grid + (size_t)f * nlat * N + lat;
+        const int nrow = a.nrow;
+        const bool has_row = live && lat < nrow;  // rows past nrow read as zero
+        const GT *gf = (const GT *)a.grid + (size_t)f * nrow * N + lat;
         double2 *z = zs + gk * RS;
         constexpr int PF = 8;  // loads in flight per thread
         for (int e0 = 0; e0 < N * L; e0 += PF * nt) {
@@ -340,7 +344,7 @@ __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfield
 #pragma unroll
             for (int q = 0; q < PF; ++q) {
                 const int j = (e0 + tid + q * nt) / L;
-                ld[q] = live && j < N ? (double)gf[(size_t)j * nlat] : 0.0;
+                ld[q] = has_row && j < N ? (double)gf[(size_t)j * nrow] : 0.0;
             }
 #pragma unroll
             for (int q = 0; q < PF; ++q) {

@@ -116,7 +116,8 @@ struct FftArgs {
     unsigned long long radix; // 4 bits per pass, first pass in the low bits
     const double2 *tw;        // e^{-2 pi i t / nlon}
     double2 *F;               // Fourier workspace [field][im][nlat]
-    void *grid;               // grid array (nlat, nlon, nfields)
+    void *grid;               // grid array (nrow, nlon, nfields)
+    int nrow;                 // grid rows: nlat, or nlat - 1 when the last node (south pole) has no row
     int io_f32;
     int mode;          // grid row factor: 0 none, 1 1/sin(theta), 2 sin(theta) (Robert form)
     const double *s;   // [np] sin(theta) for Robert scaling
@@ -131,6 +132,11 @@ struct shtk_plan {
     int xcd_map = 0, synth_chains = 4, fft_tile = 0;  // launch-shape knobs
     bool canonical = true;
     bool poles = false;  // nodes at theta = 0 and pi (the first pair)
+    bool pole_weights = false;  // the pole nodes carry nonzero quadrature weights
+    // Grid rows of the user arrays.  The Driscoll-Healy grid runs as the symmetric
+    // grid of nlat + 1 nodes theta = i pi / nlat (d.nlat = nlat + 1 inside the plan)
+    // whose last node, the south pole, has weight 0 and no row.
+    int nrow = 0;
     double inv_scale = 0, scale_phi = 0;
     std::vector<double> hx, hw;  // north-first nodes and weights (nlat)
     std::vector<long long> hcoff;  // host copy of coff: first table entry of each order
@@ -311,6 +317,7 @@ struct MlArgs {
     const void *in0, *in1;
     void *out0, *out1;
     double *acc;         // analysis: FP64 sums [field][l - m][4], cleared first
+    int nrow;            // rows of the Fourier columns (nlat, or nlat - 1: no south-pole row)
     long long ld_spec, ld_grid;
 };
 hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st);

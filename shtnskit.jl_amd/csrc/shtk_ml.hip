@@ -140,7 +140,7 @@ __global__ __launch_bounds__(64) void k_ml_synth(const MlArgs a) {
     const int ps = a.nlat - 1 - p;
     st_c2(o0, p, g * n0.x, g * n0.y);
     if (a.vector) st_c2(o1, p, g * n1.x, g * n1.y);
-    if (ps != p) {
+    if (ps != p && ps < a.nrow) {
         st_c2(o0, ps, g * s0.x, g * s0.y);
         if (a.vector) st_c2(o1, ps, g * s1.x, g * s1.y);
     }
@@ -163,12 +163,13 @@ __global__ __launch_bounds__(64) void k_ml_anal(const MlArgs a) {
     double2 vN0 = make_double2(0.0, 0.0), vS0 = vN0, vN1 = vN0, vS1 = vN0;
     {
         const IT *in0 = (const IT *)a.in0 + 2 * (size_t)f * a.ld_grid;
+        const bool south = ps != p && ps < a.nrow;
         vN0 = ld_c(in0, p);
-        if (ps != p) vS0 = ld_c(in0, ps);
+        if (south) vS0 = ld_c(in0, ps);
         if (a.vector) {
             const IT *in1 = (const IT *)a.in1 + 2 * (size_t)f * a.ld_grid;
             vN1 = ld_c(in1, p);
-            if (ps != p) vS1 = ld_c(in1, ps);
+            if (south) vS1 = ld_c(in1, ps);
         }
         const double sc = wl * rs;
         vN0.x *= sc; vN0.y *= sc; vS0.x *= sc; vS0.y *= sc;

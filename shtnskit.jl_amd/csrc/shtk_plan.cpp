@@ -139,6 +139,24 @@ void clenshaw_curtis_nodes(int n, std::vector<double> &x, std::vector<double> &w
     }
 }
 
+// Driscoll-Healy nodes theta_j = pi j / n (j < n) and weights (sqrt(8)/n sqrt(2))
+// sin(pi j / n) sum_{l=0}^{n/2-1} sin((2l+1) pi j / n) / (2l+1), the formula of
+// driscoll_healy_weights (src/mathutils.jl:149-183) with the nodes of
+// create_regular_config (src/config.jl:1267-1273).  Node n, the south pole, is
+// appended with weight 0 so that the node set is symmetric about the equator.
+void driscoll_healy_nodes(int n, std::vector<double> &x, std::vector<double> &w) {
+    x.assign(n + 1, 0.0);
+    w.assign(n + 1, 0.0);
+    const double nf = std::sqrt(8.0) / n * std::sqrt(2.0);
+    for (int j = 0; j < n; ++j) {
+        x[j] = std::cos(M_PI * j / n);
+        double sum = 0.0;
+        for (int l = 0; l <= n / 2 - 1; ++l) sum += std::sin((2.0 * l + 1.0) * M_PI * j / n) / (2.0 * l + 1.0);
+        w[j] = nf * std::sin(M_PI * j / n) * sum;
+    }
+    x[n] = -1.0;
+}
+
 double coef_scale(const shtk_desc &d, int l, int m) {
     double nrm = 1.0;
     if (d.norm == SHTK_NORM_FOURPI) nrm = std::sqrt(4.0 * M_PI);
@@ -178,8 +196,11 @@ int validate(const shtk_desc *d) {
     if (d->lmax < 0) return fail(SHTK_EINVAL, "lmax must be >= 0");
     if (d->mmax < 0 || d->mmax > d->lmax) return fail(SHTK_EINVAL, "mmax must satisfy 0 <= mmax <= lmax");
     if (d->mres < 1) return fail(SHTK_EINVAL, "mres must be >= 1");
-    if (d->grid < SHTK_GRID_GAUSS || d->grid > SHTK_GRID_REGULAR_POLES)
-        return fail(SHTK_EINVAL, "unknown grid type (gauss, regular or regular with poles)");
+    if (d->grid < SHTK_GRID_GAUSS || d->grid > SHTK_GRID_DRISCOLL_HEALY)
+        return fail(SHTK_EINVAL, "unknown grid type (gauss, regular, regular with poles, Driscoll-Healy)");
+    if (d->grid == SHTK_GRID_DRISCOLL_HEALY && (d->nlat % 2 != 0 || d->nlat != 2 * (d->lmax + 1)))
+        return fail(SHTK_EINVAL, "DH weights require nlat=2*(lmax+1)=" + std::to_string(2 * (d->lmax + 1)) +
+                                     ", got nlat=" + std::to_string(d->nlat));
     if (d->grid == SHTK_GRID_GAUSS && d->nlat < d->lmax + 1)
         return fail(SHTK_EINVAL, "Gauss-Legendre quadrature requires nlat >= lmax+1");
     if (d->grid == SHTK_GRID_REGULAR && d->nlat < d->lmax + 2)
@@ -190,7 +211,8 @@ int validate(const shtk_desc *d) {
     if (d->norm < 0 || d->norm > 2) return fail(SHTK_EINVAL, "unknown normalization");
     if (d->phi_scale != SHTK_PHI_DFT && d->phi_scale != SHTK_PHI_QUAD)
         return fail(SHTK_EINVAL, "unknown phi scaling mode");
-    if (d->nlat > 8192) return fail(SHTK_EUNSUPPORTED, "nlat > 8192 is not supported by this backend");
+    if (d->nlat + (d->grid == SHTK_GRID_DRISCOLL_HEALY ? 1 : 0) > 8192)
+        return fail(SHTK_EUNSUPPORTED, "nlat > 8192 (8191 for Driscoll-Healy grids) is not supported by this backend");
     return SHTK_OK;
 }
 
@@ -304,6 +326,7 @@ shtk::FftArgs fft_args(const shtk_plan *p) {
     f.tw = p->tw;
     f.s = p->s;
     f.tile_pairs = p->fft_tile;
+    f.nrow = p->nrow;
     return f;
 }
 
@@ -718,6 +741,9 @@ int ensure_vshadow(shtk_plan *p) {
     s->scale_phi = p->scale_phi;
     s->hx = p->hx;
     s->hw = p->hw;
+    s->nrow = p->nrow;
+    s->poles = p->poles;
+    s->pole_weights = p->pole_weights;
     const int rc = plan_tables(s);
     if (rc) {
         free_plan(s);
@@ -1369,6 +1395,8 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     shtk_plan *p = new (std::nothrow) shtk_plan();
     if (!p) return fail(SHTK_ENOMEM, "host allocation failed");
     p->d = *desc;
+    p->nrow = desc->nlat;
+    if (desc->grid == SHTK_GRID_DRISCOLL_HEALY) p->d.nlat = desc->nlat + 1;  // plus the south pole
     const shtk_desc &d = p->d;
     if (hipGetDevice(&p->device) != hipSuccess) {
         delete p;
@@ -1393,11 +1421,13 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
         p->hw.assign(wa.rbegin(), wa.rend());
     } else {
         if (d.grid == SHTK_GRID_REGULAR) fejer1_nodes(d.nlat, xa, wa);
-        else clenshaw_curtis_nodes(d.nlat, xa, wa);
+        else if (d.grid == SHTK_GRID_REGULAR_POLES) clenshaw_curtis_nodes(d.nlat, xa, wa);
+        else driscoll_healy_nodes(p->nrow, xa, wa);
         p->hx = xa;
         p->hw = wa;
     }
-    p->poles = d.grid == SHTK_GRID_REGULAR_POLES;
+    p->poles = d.grid == SHTK_GRID_REGULAR_POLES || d.grid == SHTK_GRID_DRISCOLL_HEALY;
+    p->pole_weights = d.grid == SHTK_GRID_REGULAR_POLES;
     rc = plan_tables(p);
     if (rc) {
         free_plan(p);
@@ -1462,7 +1492,7 @@ int shtk_plan_destroy(shtk_plan *plan) {
 
 int shtk_plan_nodes(const shtk_plan *plan, double *x, double *w) {
     if (!plan) return fail(SHTK_EINVAL, "plan is NULL");
-    for (int i = 0; i < plan->d.nlat; ++i) {
+    for (int i = 0; i < plan->nrow; ++i) {
         if (x) x[i] = plan->hx[i];
         if (w) w[i] = plan->hw[i];
     }
@@ -1553,7 +1583,7 @@ int shtk_analysis_sphtor(shtk_plan *p, const void *Vt, const void *Vp, void *S, 
     RC_TRY(norm_ltr(p, ltr, &ltr));
     // Robert-form rows cannot be divided by sin(theta) at weighted pole nodes when
     // order 1 is stored (_validate_robert_analysis, src/config.jl:608-624)
-    if (p->poles && p->d.robert_form && p->d.mres == 1 && p->d.mmax >= 1 && ltr >= 1)
+    if (p->pole_weights && p->d.robert_form && p->d.mres == 1 && p->d.mmax >= 1 && ltr >= 1)
         return fail(SHTK_EINVAL, "Robert-form components with m=1 cannot be unweighted at nonzero-weight pole "
                                  "nodes; use a Gauss or pole-free regular grid");
     Job j{1, 1, p->d.robert_form ? p->winvs2 : p->winvs, 1.0, 0, 1, 0, ltr, "leg_anal_sphtor", "fft_anal"};
@@ -1623,7 +1653,8 @@ static int ml_args(shtk_plan *p, int im, int ltr, int nfields, int dtype, int ve
     A.sectk = p->sectk + (size_t)im * p->np;
     A.conv = p->canonical ? nullptr : p->convscale + (size_t)(p->d.lmax + 1) * m;
     A.ld_spec = ltr - m + 1;
-    A.ld_grid = p->d.nlat;
+    A.ld_grid = p->nrow;
+    A.nrow = p->nrow;
     return SHTK_OK;
 }
 
@@ -1650,7 +1681,7 @@ static int ml_anal(shtk_plan *p, int im, const void *V0, const void *V1, void *S
     if (!S || !V0 || (vector && (!T || !V1))) return fail(SHTK_EINVAL, "NULL array argument");
     // Robert-form rows cannot be divided by sin(theta) at weighted pole nodes for
     // order 1 (_validate_robert_analysis with m, src/config.jl:608-624)
-    if (vector && p->poles && p->d.robert_form && a.m == 1 && a.ltr >= 1)
+    if (vector && p->pole_weights && p->d.robert_form && a.m == 1 && a.ltr >= 1)
         return fail(SHTK_EINVAL, "Robert-form components with m=1 cannot be unweighted at nonzero-weight pole "
                                  "nodes; use a Gauss or pole-free regular grid");
     std::lock_guard<std::mutex> lk(p->mu);

@@ -46,12 +46,13 @@ function _check(op::Symbol, status::Cint)
 end
 
 _norm_code(cfg) = cfg.norm === :orthonormal ? 0 : cfg.norm === :fourpi ? 1 : 2
-# shtk_grid: Gauss-Legendre, Fejer midpoints, Clenshaw-Curtis with poles (no Driscoll-Healy)
+# shtk_grid: Gauss-Legendre, Fejer midpoints, Clenshaw-Curtis with poles, Driscoll-Healy
 function _grid_code(cfg)
     g = cfg.grid_type
     (g === :gauss || g === :gauss_fly) && return 0
     g === :regular && return 1
     g === :regular_poles && return 2
+    g === :driscoll_healy && return 3
     throw(BackendUnavailableError(:plan, "grid_type $(g) is not provided by the MI355X backend"))
 end
 _desc(cfg::SHTConfig) = ShtkDesc(cfg.lmax, cfg.mmax, cfg.mres, cfg.nlat, cfg.nlon, _norm_code(cfg),

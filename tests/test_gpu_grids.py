@@ -16,7 +16,8 @@ GRIDS = ["regular", "regular_poles"]
 
 
 def _pair(shtk, oracle, lmax, nlat, grid, **kw):
-    cfg = shtk.create_regular_config(lmax, nlat, include_poles=grid == "regular_poles", **kw)
+    cfg = shtk.create_regular_config(lmax, nlat, include_poles=grid != "regular",
+                                     use_dh_weights=grid == "driscoll_healy", **kw)
     return cfg, oracle.Config(lmax, nlat, grid=grid, **kw)
 
 
@@ -192,3 +193,55 @@ def test_round_trip_exact_grids(shtk, oracle, cuda, grid):
     Vt, Vp = shtk.synthesis_sphtor_batch(cfg, to_dev(S), to_dev(T))
     S2, T2 = shtk.analysis_sphtor_batch(cfg, Vt, Vp)
     assert rel_l2(to_host(S2), S) < 1e-12 and rel_l2(to_host(T2), T) < 1e-12
+
+
+# --------------------------------------------------------------------------- Driscoll-Healy
+DH_CASES = [(3, 10, None, 1, False), (40, 96, None, 1, False), (40, 96, None, 1, True), (60, 128, 60, 2, False),
+            (255, 512, None, 1, False)]
+
+
+@pytest.mark.parametrize("lmax,nlon,mmax,mres,robert", DH_CASES)
+def test_driscoll_healy_match_oracle(shtk, oracle, cuda, lmax, nlon, mmax, mres, robert):
+    """Driscoll-Healy grid (nlat = 2 (lmax + 1), north pole a node, south pole not):
+    the plan runs the symmetric grid of nlat + 1 nodes whose south-pole row does
+    not exist in the arrays.  Scalar and vector, FP64 and split scalar, against
+    the oracle; Robert-form analysis is allowed (the pole weight is zero)."""
+    nlat = 2 * (lmax + 1)
+    mm = lmax if mmax is None else mmax
+    cfg, ocfg = _pair(shtk, oracle, lmax, nlat, "driscoll_healy", nlon=nlon, mmax=mm, mres=mres,
+                      robert_form=robert)
+    assert cfg.x.shape == (nlat,) and np.abs(cfg.x - ocfg.x).max() < 5e-16 and np.abs(cfg.w - ocfg.w).max() < 1e-15
+    alm = _alm(oracle, lmax, mm, mres, 5, nf=2)
+    f_r = ocfg.synthesis(alm)
+    assert rel_l2(to_host(shtk.synthesis_batch(cfg, to_dev(alm))), f_r) < TOL
+    assert rel_l2(to_host(shtk.synthesis_batch(cfg, to_dev(alm), precision="split")), f_r) < 1e-11
+    rng = np.random.default_rng(8)
+    g = np.asfortranarray(rng.standard_normal((nlat, nlon, 2)))
+    a_r = ocfg.analysis(g)
+    assert rel_l2(to_host(shtk.analysis_batch(cfg, to_dev(g))), a_r) < TOL
+    assert rel_l2(to_host(shtk.analysis_batch(cfg, to_dev(g), precision="split")), a_r) < 1e-11
+    S = _alm(oracle, lmax, mm, mres, 21, nf=2, lmin=1)
+    T = _alm(oracle, lmax, mm, mres, 22, nf=2, lmin=1)
+    Vt_r, Vp_r = ocfg.synthesis_sphtor(S, T)
+    Vt, Vp = shtk.synthesis_sphtor_batch(cfg, to_dev(S), to_dev(T))
+    assert rel_l2(to_host(Vt), Vt_r) < TOL and rel_l2(to_host(Vp), Vp_r) < TOL
+    gt = np.asfortranarray(rng.standard_normal((nlat, nlon, 2)))
+    gp = np.asfortranarray(rng.standard_normal((nlat, nlon, 2)))
+    S_r, T_r = ocfg.analysis_sphtor(gt, gp)
+    S2, T2 = shtk.analysis_sphtor_batch(cfg, to_dev(gt), to_dev(gp))
+    assert rel_l2(to_host(S2), S_r) < TOL and rel_l2(to_host(T2), T_r) < TOL
+    # single order on the same grid
+    im = 1 if mres == 1 else 0
+    q = S[im * mres:, im * mres, 0]
+    col = shtk.synthesis_packed_ml(cfg, im, torch.from_numpy(np.ascontiguousarray(q)).to("cuda:0"), lmax)
+    assert rel_l2(col.cpu().numpy(), ocfg.synthesis_ml(im, q, lmax)) < TOL
+
+
+def test_driscoll_healy_round_trip(shtk, oracle, cuda):
+    lmax = 127
+    cfg, _ = _pair(shtk, oracle, lmax, 2 * (lmax + 1), "driscoll_healy", nlon=256)
+    alm = _alm(oracle, lmax, lmax, 1, 9, nf=2)
+    back = to_host(shtk.analysis_batch(cfg, shtk.synthesis_batch(cfg, to_dev(alm))))
+    assert rel_l2(back, alm) < 1e-12
+    with pytest.raises(ValueError):
+        shtk.create_regular_config(lmax, 2 * lmax, include_poles=True, use_dh_weights=True)

@@ -510,3 +510,35 @@ def test_single_order_against_full_transform(oracle):
         z = cfg.synthesis(a, real_output=False)
         col = cfg.synthesis_ml(m, a[m:, m], lmax)
         assert np.allclose(col / nlon, z[:, 0], atol=1e-13, rtol=0)
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32, 64])
+def test_driscoll_healy_rule(oracle, n):
+    """Driscoll-Healy weights on theta_j = pi j / n integrate x^k exactly for
+    k <= n - 1 and not for k = n; the north pole is a node with weight 0 and the
+    south pole is not a node."""
+    x, w = oracle.driscoll_healy(n)
+    assert x[0] == 1.0 and w[0] == 0.0 and x[-1] > -1.0
+    assert np.allclose(x, np.cos(np.pi * np.arange(n) / n), atol=1e-16)
+    for k in range(n):
+        exact = 2.0 / (k + 1) if k % 2 == 0 else 0.0
+        assert abs(np.dot(w, x ** k) - exact) < 2e-14
+    if n <= 16:  # the degree-n error shrinks fast with n
+        assert abs(np.dot(w, x ** n) - 2.0 / (n + 1)) > 1e-12
+
+
+def test_driscoll_healy_round_trip(oracle):
+    """nlat = 2 (lmax + 1) Driscoll-Healy nodes: analysis inverts synthesis, scalar
+    and vector (Robert form included: the pole weight is zero)."""
+    lmax = 11
+    for robert in (False, True):
+        cfg = oracle.Config(lmax, 2 * (lmax + 1), nlon=24, grid="driscoll_healy", robert_form=robert)
+        alm = oracle.random_alm(lmax, lmax, 2, 1.0)
+        alm[:, 0] = alm[:, 0].real
+        assert np.abs(cfg.analysis(cfg.synthesis(alm)) - alm).max() < 1e-13
+        S = oracle.random_alm(lmax, lmax, 4, 1.0, lmin=1)
+        T = oracle.random_alm(lmax, lmax, 5, 1.0, lmin=1)
+        S[:, 0] = S[:, 0].real
+        T[:, 0] = T[:, 0].real
+        S2, T2 = cfg.analysis_sphtor(*cfg.synthesis_sphtor(S, T))
+        assert np.abs(S2 - S).max() < 1e-12 and np.abs(T2 - T).max() < 1e-12

@@ -353,6 +353,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup-seconds", type=float, default=0.5,
+                    help="the plain warmup repeats the step (past --warmup steps) until this much time has "
+                         "passed: a GPU fresh from idle runs its first tens of milliseconds of work slowly")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-split", action="store_true", help="skip the batched split-precision line")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-lmax batch line")
@@ -393,8 +396,13 @@ def main() -> int:
     def step():
         return shtk.analysis(cfg, shtk.synthesis(cfg, alm))
 
-    for _ in range(args.warmup):
+    t_w = time.perf_counter()
+    n_w = 0
+    while n_w < args.warmup or time.perf_counter() - t_w < args.warmup_seconds:
         out = step()
+        n_w += 1
+        if n_w % 50 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
     rt_err = float(torch.linalg.vector_norm(out - alm) / torch.linalg.vector_norm(alm))
     no_ev, with_ev, kern, _ = timed_steps(torch, _lib, cfg, step, args.steps, stream)
@@ -408,6 +416,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_steps_run": n_w,
         "ms_per_step": round(ms_per_step, 5),
         "ms_per_step_with_kernel_events": round(with_ev / args.steps, 5),
         "higher_is_better": True,

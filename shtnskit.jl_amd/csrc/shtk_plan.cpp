@@ -264,8 +264,9 @@ hipEvent_t pool_event(shtk_plan *p) {
         p->event_pool.pop_back();
         return e;
     }
+    // timing markers only: no system-scope fence (it adds microseconds around every kernel)
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 

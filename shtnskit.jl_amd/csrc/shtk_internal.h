@@ -130,6 +130,7 @@ struct shtk_plan {
     shtk_desc d{};
     int np = 0, nm = 0, seg = 128, nseg_total = 0;
     int xcd_map = 0, synth_chains = 4, fft_tile = 0;  // launch-shape knobs
+    int oz_nh_force = 0;  // split analysis pair halves per block (0: by grid size)
     bool canonical = true;
     bool poles = false;  // nodes at theta = 0 and pi (the first pair)
     bool pole_weights = false;  // the pole nodes carry nonzero quadrature weights
@@ -324,9 +325,9 @@ hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st);
 hipError_t launch_ml_anal(const MlArgs &a, int nfields, hipStream_t st);
 // Pole rows of order 1 for the vector transforms on grids with nodes at the
 // poles (one block per field): synthesis rewrites them from the prologue
-// records in a.in0 (robert = 1: zeros every order's pole rows), analysis adds
-// the weighted pole terms to the order-1 outputs in a.out0 / a.out1.
-hipError_t launch_vpole_synth(const LegArgs &a, int nfields, int robert, hipStream_t st);
+// records in a.in0 times wscale (robert = 1: zeros every order's pole rows),
+// analysis adds the pole terms times wpole to the order-1 outputs in a.out0 / a.out1.
+hipError_t launch_vpole_synth(const LegArgs &a, int nfields, int robert, double wscale, hipStream_t st);
 hipError_t launch_vpole_anal(const LegArgs &a, int nfields, double wpole, hipStream_t st);
 // Complex-valued synthesis FFT: bins m = 0 .. mmax of F -> complex grid rows.
 hipError_t launch_fft_cplx(const FftArgs &a, int nfields, hipStream_t st);

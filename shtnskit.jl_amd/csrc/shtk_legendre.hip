@@ -705,7 +705,7 @@ __device__ __forceinline__ void block_sum256(double (*red)[256], double *v) {
     for (int k = 0; k < NV; ++k) v[k] = red[k][0];
 }
 
-__global__ __launch_bounds__(256) void k_vpole_synth(const LegArgs a, int robert) {
+__global__ __launch_bounds__(256) void k_vpole_synth(const LegArgs a, int robert, double wscale) {
     __shared__ double red[8][256];
     const int f = blockIdx.x, nlat = a.nlat;
     if (robert) {
@@ -734,7 +734,7 @@ __global__ __launch_bounds__(256) void k_vpole_synth(const LegArgs a, int robert
     }
     block_sum256<8>(red, v);
     if (threadIdx.x == 0) {
-        const double gm = a.g * mfac_of(a.mfac_mode, 1);
+        const double gm = a.g * mfac_of(a.mfac_mode, 1) * wscale;
         const size_t fo = ((size_t)f * a.nm + 1) * nlat;
         // north: Vt = -sum h S' + i sum h T', Vp = -i sum h S' - sum h T'
         a.F0[fo] = make_double2(gm * (-v[0] - v[3]), gm * (v[2] - v[1]));
@@ -1272,8 +1272,8 @@ hipError_t launch_vspec_prep(const void *S, const void *T, double *rec, int in_f
     return hipGetLastError();
 }
 
-hipError_t launch_vpole_synth(const LegArgs &a, int nfields, int robert, hipStream_t st) {
-    hipLaunchKernelGGL(k_vpole_synth, dim3((unsigned)nfields), dim3(256), 0, st, a, robert);
+hipError_t launch_vpole_synth(const LegArgs &a, int nfields, int robert, double wscale, hipStream_t st) {
+    hipLaunchKernelGGL(k_vpole_synth, dim3((unsigned)nfields), dim3(256), 0, st, a, robert, wscale);
     return hipGetLastError();
 }
 

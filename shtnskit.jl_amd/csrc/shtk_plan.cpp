@@ -424,7 +424,9 @@ struct OzWs {
 // when np > 256, so that lmax <= 1023 needs one block per order and no
 // atomics); ksteps of 64 pairs cover whole blocks, so every LDS position of a
 // block has a field operand (zero beyond np).
-int oz_nh(const shtk_plan *p) { return p->np > 256 ? 2 : 1; }
+// pair halves per split-analysis block: 512 pairs (two halves) for grids of more than
+// 256 pairs unless SHTK_OZ_NH (1 or 2) chose otherwise at plan creation
+int oz_nh(const shtk_plan *p) { return p->oz_nh_force ? p->oz_nh_force : (p->np > 256 ? 2 : 1); }
 int oz_npb(const shtk_plan *p) { return (p->np + 256 * oz_nh(p) - 1) / (256 * oz_nh(p)); }
 int oz_nks(const shtk_plan *p) { return 4 * oz_nh(p) * oz_npb(p); }
 
@@ -553,11 +555,14 @@ struct Job {
     const char *leg_name, *fft_name;
     int prec = SHTK_PREC_FP64;
     int cplx_out = 0;  // scalar synthesis to a complex grid (real_output = false)
-    int pole_fix = 0;  // forward vector transform: order-1 pole limits on grids with poles
+    // vector transforms on grids with poles, order-1 pole limits: 1 forward synthesis or
+    // analysis (quadrature weight in the analysis), 2 adjoint direction (the weight moves to
+    // the synthesis side: adjoint analysis; none on the analysis side: adjoint synthesis)
+    int pole_fix = 0;
 };
 
-// Vector transforms on grids with nodes at the poles: the FP64 forward pair
-// takes the pole limits of order 1; the split and adjoint forms are not provided.
+// Vector transforms on grids with nodes at the poles: the FP64 forward pair and its
+// adjoints take the pole limits of order 1; the split forms are not provided.
 int pole_vector_check(const shtk_plan *p, const char *what) {
     if (!p->poles) return SHTK_OK;
     return fail(SHTK_EUNSUPPORTED, std::string(what) + " is not provided on grids with nodes at the poles");
@@ -625,8 +630,10 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
         HIP_TRY(timed(p, j.leg_name, st,
                       [&] { return shtk::launch_leg_synth(a, nf, j.vector, p->synth_chains, st); }),
                 "Legendre synthesis launch");
-        if (j.pole_fix && p->poles && (p->d.robert_form || (p->d.mres == 1 && p->d.mmax >= 1)))
-            HIP_TRY(shtk::launch_vpole_synth(a, nf, p->d.robert_form, st), "pole rows launch");
+        const bool zero_rows = j.pole_fix == 1 && p->d.robert_form;  // Robert rows are sin(theta) V
+        if (j.pole_fix && p->poles && (zero_rows || (p->d.mres == 1 && p->d.mmax >= 1)))
+            HIP_TRY(shtk::launch_vpole_synth(a, nf, zero_rows, j.pole_fix == 2 ? p->hw[0] * p->scale_phi : 1.0, st),
+                    "pole rows launch");
     }
     void *grids[2] = {g0, g1};
     for (int q = 0; q <= j.vector; ++q) {
@@ -706,7 +713,8 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
     HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_leg_anal(a, nf, j.vector, st); }),
             "Legendre analysis launch");
     if (j.pole_fix && p->poles && p->d.mres == 1 && p->d.mmax >= 1)
-        HIP_TRY(shtk::launch_vpole_anal(a, nf, p->hw[0] * p->scale_phi, st), "pole rows launch");
+        HIP_TRY(shtk::launch_vpole_anal(a, nf, j.pole_fix == 2 ? 1.0 : p->hw[0] * p->scale_phi, st),
+                "pole rows launch");
     if (!p->canonical) {
         void *dst[2] = {s0, s1};
         for (int q = 0; q <= j.vector; ++q)
@@ -737,6 +745,7 @@ int ensure_vshadow(shtk_plan *p) {
     s->synth_chains = p->synth_chains;
     s->fft_tile = p->fft_tile;
     s->oz_run_len = p->oz_run_len;
+    s->oz_nh_force = p->oz_nh_force;
     s->canonical = true;
     s->inv_scale = p->inv_scale;
     s->scale_phi = p->scale_phi;
@@ -1412,6 +1421,10 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     p->synth_chains = env_int("SHTK_SYNTH_CHAINS", p->seg == 64 ? 2 : 4);
     p->fft_tile = env_int("SHTK_FFT_PAIRS", 0);
     p->oz_run_len = std::max(1, env_int("SHTK_OZ_RUN", 8));
+    {
+        const int nh = env_int("SHTK_OZ_NH", 0);
+        p->oz_nh_force = (nh == 1 || nh == 2) ? nh : 0;
+    }
     p->canonical = d.norm == SHTK_NORM_ORTHONORMAL && !d.real_norm && d.cs_phase;
     p->inv_scale = d.phi_scale == SHTK_PHI_QUAD ? d.nlon / (2.0 * M_PI) : (double)d.nlon;
     p->scale_phi = (2.0 * M_PI / d.nlon) * d.nlon / p->inv_scale;
@@ -1723,17 +1736,20 @@ int shtk_analysis_sphtor_ml(shtk_plan *p, int im, const void *Vt, const void *Vp
 int shtk_adjoint_synthesis_sphtor(shtk_plan *p, const void *Vtb, const void *Vpb, void *Sb, void *Tb, int nfields,
                                   int dtype, int real_output, void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {Vtb, Vpb, Sb, Tb}));
-    RC_TRY(pole_vector_check(p, "the adjoint vector synthesis"));
     Job j{1, 0, p->invs, p->inv_scale / p->d.nlon, real_output ? 1 : 0, 0, p->d.robert_form ? 2 : 0, p->d.lmax,
           "leg_anal_sphtor", "fft_anal"};
+    j.pole_fix = 2;
     return run_to_spec(p, j, Vtb, Vpb, Sb, Tb, nfields, dtype, stream);
 }
 
 int shtk_adjoint_analysis_sphtor(shtk_plan *p, const void *Sb, const void *Tb, void *Vtb, void *Vpb, int nfields,
                                  int dtype, void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {Vtb, Vpb, Sb, Tb}));
-    RC_TRY(pole_vector_check(p, "the adjoint vector analysis"));
+    if (p->pole_weights && p->d.robert_form && p->d.mres == 1 && p->d.mmax >= 1)
+        return fail(SHTK_EINVAL, "Robert-form components with m=1 cannot be unweighted at nonzero-weight pole "
+                                 "nodes; use a Gauss or pole-free regular grid");
     Job j{1, 1, p->winvs, 1.0, 2, 1, p->d.robert_form ? 1 : 0, p->d.lmax, "leg_synth_sphtor", "fft_synth"};
+    j.pole_fix = 2;
     return run_to_grid(p, j, Sb, Tb, Vtb, Vpb, nfields, dtype, stream);
 }
 

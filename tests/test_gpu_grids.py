@@ -125,8 +125,30 @@ def test_pole_grid_f32_and_truncation(shtk, oracle, cuda):
     assert rel_l2(to_host(Vt), Vt_r) < TOL and rel_l2(to_host(Vp), Vp_r) < TOL
 
 
+@pytest.mark.parametrize("grid,robert", [("regular_poles", False), ("driscoll_healy", False),
+                                         ("driscoll_healy", True), ("regular", True)])
+def test_vector_adjoints_pole_grids(shtk, oracle, cuda, grid, robert):
+    """Vector adjoints (the rrule pullbacks) on grids with poles: the order-1 pole
+    terms move to the other side of the transform (no weight in the adjoint
+    synthesis, the quadrature weight in the adjoint analysis)."""
+    lmax, nlon = 40, 96
+    nlat = 2 * (lmax + 1) if grid == "driscoll_healy" else 81
+    cfg, ocfg = _pair(shtk, oracle, lmax, nlat, grid, nlon=nlon, robert_form=robert)
+    S = _alm(oracle, lmax, lmax, 1, 31, lmin=1)
+    T = _alm(oracle, lmax, lmax, 1, 32, lmin=1)
+    rng = np.random.default_rng(6)
+    yt = np.asfortranarray(rng.standard_normal((nlat, nlon)))
+    yp = np.asfortranarray(rng.standard_normal((nlat, nlon)))
+    Sb_r, Tb_r = ocfg.adjoint_synthesis_sphtor(yt, yp)
+    Sb, Tb = shtk._adjoint_synthesis_sphtor(cfg, to_dev(yt), to_dev(yp))
+    assert rel_l2(to_host(Sb), Sb_r) < TOL and rel_l2(to_host(Tb), Tb_r) < TOL
+    vt_r, vp_r = ocfg.adjoint_analysis_sphtor(S, T)
+    vt, vp = shtk._adjoint_analysis_sphtor(cfg, to_dev(S), to_dev(T))
+    assert rel_l2(to_host(vt), vt_r.real) < TOL and rel_l2(to_host(vp), vp_r.real) < TOL
+
+
 def test_pole_grid_unsupported_forms(shtk, oracle, cuda):
-    """Split vector transforms and the vector adjoints are refused on pole grids."""
+    """Split vector transforms are refused on pole grids."""
     cfg, _ = _pair(shtk, oracle, 8, 17, "regular_poles")
     S = to_dev(_alm(oracle, 8, 8, 1, 1, lmin=1))
     g = torch.zeros((17, cfg.nlon), dtype=torch.float64, device="cuda:0").t().contiguous().t()

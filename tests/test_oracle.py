@@ -542,3 +542,25 @@ def test_driscoll_healy_round_trip(oracle):
         T[:, 0] = T[:, 0].real
         S2, T2 = cfg.analysis_sphtor(*cfg.synthesis_sphtor(S, T))
         assert np.abs(S2 - S).max() < 1e-12 and np.abs(T2 - T).max() < 1e-12
+
+
+@pytest.mark.parametrize("grid", ["regular_poles", "driscoll_healy"])
+def test_adjoint_dot_product_pole_grids(oracle, grid):
+    """The vector adjoints on grids with nodes at the poles (order-1 pole limits)
+    satisfy the same dot-product identities as on Gauss grids."""
+    lmax = 11
+    nlat = 2 * (lmax + 1) if grid == "driscoll_healy" else 23
+    cfg = oracle.Config(lmax, nlat, nlon=24, grid=grid)
+    rng = np.random.default_rng(3)
+    S = oracle.random_alm(lmax, lmax, 4, 1.0, lmin=1)
+    T = oracle.random_alm(lmax, lmax, 5, 1.0, lmin=1)
+    yt = np.asfortranarray(rng.standard_normal((nlat, 24)))
+    yp = np.asfortranarray(rng.standard_normal((nlat, 24)))
+    Vt, Vp = cfg.synthesis_sphtor(S, T)
+    Sb, Tb = cfg.adjoint_synthesis_sphtor(yt, yp)
+    lhs = np.sum(Vt * yt) + np.sum(Vp * yp)
+    assert abs(lhs - _inner_real(Sb, S) - _inner_real(Tb, T)) < 1e-11 * abs(lhs)
+    Sa, Ta = cfg.analysis_sphtor(yt, yp)
+    vt, vp = cfg.adjoint_analysis_sphtor(S, T)
+    lhs = _inner_real(Sa, S) + _inner_real(Ta, T)
+    assert abs(lhs - np.sum(yt * vt.real) - np.sum(yp * vp.real)) < 1e-11 * abs(lhs)

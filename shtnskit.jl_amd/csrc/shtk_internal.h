@@ -320,6 +320,7 @@ struct MlArgs {
     double *acc;         // analysis: FP64 sums [field][l - m][4], cleared first
     int nrow;            // rows of the Fourier columns (nlat, or nlat - 1: no south-pole row)
     long long ld_spec, ld_grid;
+    int fourier_f64;     // the Fourier columns are FP64 whatever io_f32 says (plan workspace)
 };
 hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st);
 hipError_t launch_ml_anal(const MlArgs &a, int nfields, hipStream_t st);

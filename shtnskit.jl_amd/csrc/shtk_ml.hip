@@ -232,7 +232,11 @@ __global__ __launch_bounds__(256) void k_ml_fin(const MlArgs a) {
 
 hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st) {
     const dim3 grid((unsigned)((a.np + 63) / 64), (unsigned)nfields);
-    if (a.io_f32)
+    if (a.io_f32 && a.fourier_f64)
+        hipLaunchKernelGGL((k_ml_synth<float, double>), grid, dim3(64), 0, st, a);
+    else if (a.fourier_f64)
+        hipLaunchKernelGGL((k_ml_synth<double, double>), grid, dim3(64), 0, st, a);
+    else if (a.io_f32)
         hipLaunchKernelGGL((k_ml_synth<float, float>), grid, dim3(64), 0, st, a);
     else
         hipLaunchKernelGGL((k_ml_synth<double, double>), grid, dim3(64), 0, st, a);
@@ -241,7 +245,14 @@ hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st) {
 
 hipError_t launch_ml_anal(const MlArgs &a, int nfields, hipStream_t st) {
     const dim3 grid((unsigned)((a.np + 63) / 64), (unsigned)nfields);
-    if (a.io_f32) {
+    if (a.fourier_f64) {
+        hipLaunchKernelGGL(k_ml_anal<double>, grid, dim3(64), 0, st, a);
+        if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
+        if (a.io_f32)
+            hipLaunchKernelGGL(k_ml_fin<float>, dim3((unsigned)nfields), dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL(k_ml_fin<double>, dim3((unsigned)nfields), dim3(256), 0, st, a);
+    } else if (a.io_f32) {
         hipLaunchKernelGGL(k_ml_anal<float>, grid, dim3(64), 0, st, a);
         if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
         hipLaunchKernelGGL(k_ml_fin<float>, dim3((unsigned)nfields), dim3(256), 0, st, a);

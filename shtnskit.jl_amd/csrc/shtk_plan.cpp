@@ -230,6 +230,7 @@ struct DeviceGuard {
 // Workspace layout: [F0][F1][spec0][spec1], each region 256-byte aligned.
 struct WsLayout {
     size_t f_bytes, spec_bytes, total;
+    size_t zonal_off;  // vector: FP64 sums of the order-0 rewrite of the analysis
 };
 
 WsLayout ws_layout(const shtk_plan *p, int nf, int vector) {
@@ -242,6 +243,8 @@ WsLayout ws_layout(const shtk_plan *p, int nf, int vector) {
     L.total = (vector ? 2 : 1) * (L.f_bytes + L.spec_bytes);
     // vector synthesis records: 8 doubles per coefficient (k_vspec_prep)
     if (vector) L.total = std::max(L.total, 2 * L.f_bytes + 4 * L.spec_bytes);
+    L.zonal_off = L.total;
+    if (vector) L.total += al(sizeof(double) * 4 * (size_t)nf * (p->d.lmax + 1));
     return L;
 }
 
@@ -568,6 +571,35 @@ int pole_vector_check(const shtk_plan *p, const char *what) {
     return fail(SHTK_EUNSUPPORTED, std::string(what) + " is not provided on grids with nodes at the poles");
 }
 
+// Order 0 of the FP64 vector transforms, rewritten by the single-order kernels:
+// near the poles D_l = l x P_l - g_l P_{l-1} cancels for m = 0 (the terms are ~l P,
+// the difference ~sin^2), so dP_l/dtheta of order 0 comes from the differentiated
+// recurrence of _zonal_dtheta_row! (src/legendre.jl:331-338) instead.  Columns are
+// the plan's Fourier rows of order 0 (field stride nm * nlat) and the order-0
+// coefficients (field stride spec_fstride).
+shtk::MlArgs zonal_args(const shtk_plan *p, int ltr, long long spec_fstride) {
+    shtk::MlArgs A{};
+    A.lmax = p->d.lmax;
+    A.m = 0;
+    A.ltr = ltr;
+    A.nlat = p->d.nlat;
+    A.np = p->np;
+    A.vector = 1;
+    A.robert = p->d.robert_form;
+    A.x = p->x;
+    A.s = p->s;
+    A.w = p->wscaled;
+    A.ab = p->ab + p->hcoff[0];
+    A.gtab = p->gtab + p->hcoff[0];
+    A.sectv = p->sectv;
+    A.sectk = p->sectk;
+    A.ld_spec = spec_fstride;
+    A.ld_grid = (long long)p->nm * p->d.nlat;
+    A.nrow = p->d.nlat;
+    A.fourier_f64 = 1;
+    return A;
+}
+
 // spectral -> grid: [convert] -> Legendre synthesis -> inverse phi FFT
 int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void *g0, void *g1, int nf, int dtype,
                 void *stream) {
@@ -630,6 +662,18 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
         HIP_TRY(timed(p, j.leg_name, st,
                       [&] { return shtk::launch_leg_synth(a, nf, j.vector, p->synth_chains, st); }),
                 "Legendre synthesis launch");
+        if (j.vector && j.pole_fix == 1 && j.ltr >= 1) {
+            shtk::MlArgs z = zonal_args(p, j.ltr, a.spec_fstride);
+            z.io_f32 = dtype == SHTK_F32;
+            z.conv = p->canonical ? nullptr : p->convscale;  // order 0: index l
+            z.g = j.g;
+            z.in0 = s0;
+            z.in1 = s1;
+            z.out0 = F0;
+            z.out1 = F1;
+            HIP_TRY(timed(p, "leg_synth_sphtor_m0", st, [&] { return shtk::launch_ml_synth(z, nf, st); }),
+                    "order-0 rows launch");
+        }
         const bool zero_rows = j.pole_fix == 1 && p->d.robert_form;  // Robert rows are sin(theta) V
         if (j.pole_fix && p->poles && (zero_rows || (p->d.mres == 1 && p->d.mmax >= 1)))
             HIP_TRY(shtk::launch_vpole_synth(a, nf, zero_rows, j.pole_fix == 2 ? p->hw[0] * p->scale_phi : 1.0, st),
@@ -712,6 +756,21 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
     }
     HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_leg_anal(a, nf, j.vector, st); }),
             "Legendre analysis launch");
+    if (j.vector && j.pole_fix == 1 && j.ltr >= 1) {
+        // the convention map of order 0 follows with the rest (non-canonical plans)
+        shtk::MlArgs z = zonal_args(p, j.ltr, a.spec_fstride);
+        z.io_f32 = a.io_f32;
+        z.g = j.g;
+        z.in0 = F0;
+        z.in1 = F1;
+        z.out0 = a.out0;
+        z.out1 = a.out1;
+        z.acc = (double *)(ws + L.zonal_off);
+        const size_t acc_bytes = sizeof(double) * 4 * (size_t)nf * (j.ltr + 1);
+        HIP_TRY(shtk::launch_zero(z.acc, acc_bytes, st), "accumulator clear");
+        HIP_TRY(timed(p, "leg_anal_sphtor_m0", st, [&] { return shtk::launch_ml_anal(z, nf, st); }),
+                "order-0 rows launch");
+    }
     if (j.pole_fix && p->poles && p->d.mres == 1 && p->d.mmax >= 1)
         HIP_TRY(shtk::launch_vpole_anal(a, nf, j.pole_fix == 2 ? 1.0 : p->hw[0] * p->scale_phi, st),
                 "pole rows launch");

@@ -158,3 +158,22 @@ def test_split_scalar_complex_output(shtk, oracle, cuda):
     ref = ocfg.synthesis(alm, real_output=False)
     got = to_host(shtk.synthesis_batch(cfg, to_dev(alm), real_output=False, precision="split"))
     assert rel_l2(got, ref) < 1e-11
+
+
+@pytest.mark.parametrize("robert", [False, True])
+def test_sphtor_zonal_lmax1023(shtk, oracle, cuda, robert):
+    """Order 0 alone at lmax 1023: dP/dtheta of order 0 must keep its accuracy near
+    the poles (the differentiated recurrence of _zonal_dtheta_row!), which the
+    mixed random spectra of the other tests would dilute."""
+    lmax, nlat, nlon = 1023, 1024, 2048
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon, robert_form=robert)
+    ocfg = oracle.Config(lmax, nlat, nlon=nlon, robert_form=robert)
+    S, T = _st(oracle, lmax, lmax, 41)
+    S[:, 1:] = 0
+    T[:, 1:] = 0
+    Vt_r, Vp_r = ocfg.synthesis_sphtor(S, T)
+    Vt, Vp = shtk.synthesis_sphtor(cfg, to_dev(S), to_dev(T))
+    assert rel_l2(to_host(Vt), Vt_r) < TOL and rel_l2(to_host(Vp), Vp_r) < TOL
+    S2, T2 = shtk.analysis_sphtor(cfg, to_dev(Vt_r), to_dev(Vp_r))
+    S_r, T_r = ocfg.analysis_sphtor(Vt_r, Vp_r)
+    assert rel_l2(to_host(S2), S_r) < TOL and rel_l2(to_host(T2), T_r) < TOL

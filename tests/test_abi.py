@@ -49,3 +49,42 @@ def test_config_mirror_validation(shtk):
         shtk.create_gauss_config(10, 11, nlon=20)
     cfg = shtk.create_gauss_config(10, 11)
     assert cfg.nlon == 22 and cfg.nlm == 66
+
+
+def test_grid_descriptor_validation_without_gpu(shtk):
+    """Grid minimums of _min_nlat (src/config.jl:452-458) in the C ABI and the mirror."""
+    import pytest
+
+    lib = shtk.load_library()
+    from shtnskit_jl_amd._lib import ShtkDesc, SHTK_EINVAL
+
+    h = ctypes.c_void_p()
+    for lmax, nlat, grid in ((10, 11, 1),     # regular needs lmax + 2
+                             (10, 10, 2),     # pole grid needs max(2, lmax + 1)
+                             (0, 1, 2),
+                             (10, 20, 3),     # Driscoll-Healy needs 2 (lmax + 1)
+                             (10, 22, 4)):    # unknown grid
+        bad = ShtkDesc(lmax, lmax, 1, nlat, 2 * lmax + 2, 0, 1, 0, 0, 0, grid)
+        assert lib.shtk_plan_create(ctypes.byref(bad), ctypes.byref(h)) == SHTK_EINVAL, (lmax, nlat, grid)
+    with pytest.raises(ValueError):
+        shtk.create_regular_config(10, 11)
+    with pytest.raises(ValueError):
+        shtk.create_regular_config(10, 21, include_poles=True, use_dh_weights=True)
+    with pytest.raises(ValueError):
+        shtk.create_regular_config(10, 22, use_dh_weights=True)  # DH needs include_poles
+    assert shtk.create_regular_config(10, 12).grid_type == "regular"
+    assert shtk.create_regular_config(10, 11, include_poles=True).grid_type == "regular_poles"
+    assert shtk.create_regular_config(10, 22, include_poles=True, use_dh_weights=True).grid_type == "driscoll_healy"
+
+
+def test_create_config_defaults(shtk):
+    """create_config (src/config.jl:1376-1415): the default nlat = lmax + 2 becomes
+    max(2 lmax + 1, 2) on equiangular grids and 2 (lmax + 1) for Driscoll-Healy."""
+    assert shtk.create_config(10).nlat == 12 and shtk.create_config(10).grid_type == "gauss"
+    assert shtk.create_config(10, grid_type="regular").nlat == 21
+    c = shtk.create_config(10, grid_type="regular_poles")
+    assert c.nlat == 21 and c.grid_type == "regular_poles"
+    c = shtk.create_config(10, grid_type="driscoll_healy")
+    assert c.nlat == 22 and c.grid_type == "driscoll_healy"
+    assert shtk.create_config(10, grid_type="regular", nlat=5).nlat == 12  # raised to the minimum
+    assert shtk.create_config(10, nlon=4).nlon == 21

@@ -317,7 +317,7 @@ struct MlArgs {
     double g;            // synthesis: phi inverse scale; analysis: 1
     const void *in0, *in1;
     void *out0, *out1;
-    double *acc;         // analysis: FP64 sums [field][l - m][4], cleared first
+    double *acc;         // analysis: FP64 partials [field][pair block of 64][l - m][4]
     int nrow;            // rows of the Fourier columns (nlat, or nlat - 1: no south-pole row)
     long long ld_spec, ld_grid;
     int fourier_f64;     // the Fourier columns are FP64 whatever io_f32 says (plan workspace)

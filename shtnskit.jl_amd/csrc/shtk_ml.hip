@@ -11,8 +11,9 @@
 // l x P_l - g_l P_{l-1}.  South values follow from the parity (-1)^(l-m).
 // Vector rows use dP/dtheta = D/s and P/s; at exact poles (s = 0) order 1 takes
 // the limits -h_l (north), h_l = sqrt((2l+1) l (l+1) / (4 pi)) / 2, and other
-// orders 0 (src/kernels.jl:72-99).  Analysis sums reduce over the wavefront and
-// are added to an FP64 accumulator; a second kernel applies the convention
+// orders 0 (src/kernels.jl:72-99).  The per-degree operands come through LDS in
+// chunks of 64 degrees.  Analysis sums reduce over the wavefront and are stored
+// as per-block FP64 partials; a second kernel adds them, applies the convention
 // map (and 1/(l(l+1)) for the vector pair) and writes the output dtype.
 #include "shtk_internal.h"
 
@@ -27,16 +28,16 @@ __device__ __forceinline__ double ml_pole_h(int l) {
     return 0.5 * 0.28209479177387814 * sqrt((2.0 * l + 1.0) * (double)l * (double)(l + 1));
 }
 
-// Recurrence state of one lane; step() moves from degree l to l + 1.
+// Recurrence state of one lane; step() moves from degree l to l + 1 with the
+// coefficients co = (a_{l+1}, b_{l+1}).
 struct MlRec {
     double x, cur, prev;
     int k;
-    __device__ __forceinline__ void step(const MlArgs &a, int l) {
-        const double2 co = a.ab[l + 1 - a.m];
+    __device__ __forceinline__ void step(double2 co, int l, int m) {
         const double nx = fma(co.x * x, cur, -(co.y * prev));
         prev = cur;
         cur = nx;
-        if (k > 0 && ((l - a.m) & (SCALE_CHECK - 1)) == SCALE_CHECK - 1 && fmax(fabs(cur), fabs(prev)) > kHiM) {
+        if (k > 0 && ((l - m) & (SCALE_CHECK - 1)) == SCALE_CHECK - 1 && fmax(fabs(cur), fabs(prev)) > kHiM) {
             cur *= kDownM;
             prev *= kDownM;
             --k;
@@ -57,25 +58,27 @@ __device__ __forceinline__ void st_c2(T *p, size_t i, double re, double im) {
 
 // Order 0: dP_l/dtheta from the differentiated recurrence d_l = sqrt((2l+1)/(2l-1))
 // (x d_{l-1} - l s P_{l-1}) (_zonal_dtheta_row!, src/legendre.jl:331-338), which
-// keeps its accuracy near the poles where l x P_l - g_l P_{l-1} cancels.
-__device__ __forceinline__ double ml_zonal_step(int l, double x, double s, double Pm1, double d) {
-    return sqrt((2.0 * l + 1.0) / (2.0 * l - 1.0)) * fma(x, d, -((double)l * s * Pm1));
+// keeps its accuracy near the poles where l x P_l - g_l P_{l-1} cancels.  With
+// g_l = l sqrt((2l+1)/(2l-1)) of order 0: d_l = (g_l / l) x d_{l-1} - g_l s P_{l-1};
+// gl = g_l / l comes staged with the chunk.
+__device__ __forceinline__ double ml_zonal_step(double g, double gl, double x, double s, double Pm1, double d) {
+    return fma(gl * x, d, -(g * s * Pm1));
 }
 
 // (dn, qn): dP/dtheta and P/sin(theta) at the north node of the lane (Robert
 // form synthesis: D and P themselves, zero at the poles); dz: the order-0
-// derivative of this degree.
-__device__ __forceinline__ void ml_vrow(const MlArgs &a, int l, double P, double Pm1, double x, double s,
+// derivative of this degree; g: g_l of the order.
+__device__ __forceinline__ void ml_vrow(int m, int l, double g, double P, double Pm1, double x, double s,
                                         double invs, double dz, bool pole, bool robert_syn, double &dn,
                                         double &qn) {
-    const double D = a.m == 0 ? s * dz : fma((double)l * x, P, -(a.gtab[l - a.m] * Pm1));
-    if (a.m == 0 && !robert_syn) {
+    const double D = m == 0 ? s * dz : fma((double)l * x, P, -(g * Pm1));
+    if (m == 0 && !robert_syn) {
         dn = dz;
         qn = P * invs;
         return;
     }
     if (pole) {
-        const double v = (a.m == 1 && !robert_syn) ? -ml_pole_h(l) : 0.0;
+        const double v = (m == 1 && !robert_syn) ? -ml_pole_h(l) : 0.0;
         dn = v;
         qn = v;
     } else if (robert_syn) {
@@ -87,14 +90,44 @@ __device__ __forceinline__ void ml_vrow(const MlArgs &a, int l, double P, double
     }
 }
 
+// Per-degree operands of a chunk of ML_CH degrees, staged in LDS by the block's
+// wave (one load per lane instead of a dependent global load per degree).
+constexpr int ML_CH = 64;
+struct MlStage {
+    double2 ab[ML_CH];  // (a_{l+1}, b_{l+1})
+    double g[ML_CH];    // g_l
+    double gl[ML_CH];   // g_l / l (order 0)
+    double c[ML_CH];    // convention factor (synthesis)
+    double2 q0[ML_CH], q1[ML_CH];  // coefficients (synthesis)
+};
+
+template <typename IT>
+__device__ __forceinline__ void ml_stage(const MlArgs &a, MlStage &S, int c0, int n, const IT *in0, const IT *in1,
+                                         bool coefs) {
+    const int t = threadIdx.x;
+    if (t < n) {
+        const int l = c0 + t, m = a.m;
+        S.ab[t] = a.ab[l + 1 - m];  // l + 1 <= lmax + 1: inside the order's table
+        S.g[t] = a.gtab[l - m];
+        S.gl[t] = l > 0 ? S.g[t] / (double)l : 0.0;
+        if (coefs) {
+            S.c[t] = a.conv ? a.conv[l] : 1.0;
+            S.q0[t] = ld_c(in0, (size_t)(l - m));
+            if (a.vector) S.q1[t] = ld_c(in1, (size_t)(l - m));
+        }
+    }
+}
+
 }  // namespace
 
 template <typename IT, typename OT>
 __global__ __launch_bounds__(64) void k_ml_synth(const MlArgs a) {
-    const int p = blockIdx.x * 64 + threadIdx.x, f = blockIdx.y;
-    if (p >= a.np) return;  // no cross-lane operations in this kernel
+    __shared__ MlStage S;
+    const int p0 = blockIdx.x * 64 + threadIdx.x, f = blockIdx.y;
+    const bool act = p0 < a.np;
+    const int p = act ? p0 : 0;
     const int m = a.m;
-    MlRec r{a.x[p], a.sectv[p], 0.0, a.sectk[p]};
+    MlRec r{a.x[p], act ? a.sectv[p] : 0.0, 0.0, act ? a.sectk[p] : 0};
     const double s = a.s[p];
     const bool pole = s == 0.0;
     const double invs = pole ? 0.0 : 1.0 / s;
@@ -103,37 +136,47 @@ __global__ __launch_bounds__(64) void k_ml_synth(const MlArgs a) {
     double2 n0 = make_double2(0.0, 0.0), s0 = n0, n1 = n0, s1 = n0;
     const double mm = (double)m;
     double dz = 0.0;
-    for (int l = m; l <= a.ltr; ++l) {
-        const double live = r.k == 0 ? 1.0 : 0.0;
-        const double P = r.cur * live, Pm1 = r.prev * live;
-        if (m == 0 && l > 0) dz = ml_zonal_step(l, r.x, s, Pm1, dz);
-        const double c = a.conv ? a.conv[l] : 1.0;
-        const double sg = ((l - m) & 1) ? -1.0 : 1.0;
-        if (!a.vector) {
-            const double2 q = ld_c(in0, (size_t)(l - m));
-            const double v = c * P;
-            n0.x = fma(v, q.x, n0.x);
-            n0.y = fma(v, q.y, n0.y);
-            s0.x = fma(sg * v, q.x, s0.x);
-            s0.y = fma(sg * v, q.y, s0.y);
-        } else if (l >= 1) {
-            double dn, qn;
-            ml_vrow(a, l, P, Pm1, r.x, s, invs, dz, pole, a.robert, dn, qn);
-            const double2 S = ld_c(in0, (size_t)(l - m)), T = ld_c(in1, (size_t)(l - m));
-            const double Sr = c * S.x, Si = c * S.y, Tr = c * T.x, Ti = c * T.y;
-            const double mq = mm * qn;
-            // Vt = d S - i m q T, Vp = i m q S + d T; south: d -> -sg d, q -> sg q
-            n0.x += dn * Sr + mq * Ti;
-            n0.y += dn * Si - mq * Tr;
-            n1.x += -mq * Si + dn * Tr;
-            n1.y += mq * Sr + dn * Ti;
-            s0.x += sg * (-dn * Sr + mq * Ti);
-            s0.y += sg * (-dn * Si - mq * Tr);
-            s1.x += sg * (-mq * Si - dn * Tr);
-            s1.y += sg * (mq * Sr - dn * Ti);
+    for (int c0 = m; c0 <= a.ltr; c0 += ML_CH) {
+        const int n = min(ML_CH, a.ltr + 1 - c0);
+        __syncthreads();
+        ml_stage(a, S, c0, n, in0, in1, true);
+        __syncthreads();
+#pragma unroll 4
+        for (int t = 0; t < n; ++t) {
+            const int l = c0 + t;
+            const double live = r.k == 0 ? 1.0 : 0.0;
+            const double P = r.cur * live, Pm1 = r.prev * live;
+            const double g = S.g[t];
+            if (m == 0 && l > 0) dz = ml_zonal_step(g, S.gl[t], r.x, s, Pm1, dz);
+            const double c = S.c[t];
+            const double sg = ((l - m) & 1) ? -1.0 : 1.0;
+            if (!a.vector) {
+                const double2 q = S.q0[t];
+                const double v = c * P;
+                n0.x = fma(v, q.x, n0.x);
+                n0.y = fma(v, q.y, n0.y);
+                s0.x = fma(sg * v, q.x, s0.x);
+                s0.y = fma(sg * v, q.y, s0.y);
+            } else if (l >= 1) {
+                double dn, qn;
+                ml_vrow(m, l, g, P, Pm1, r.x, s, invs, dz, pole, a.robert, dn, qn);
+                const double2 Sq = S.q0[t], Tq = S.q1[t];
+                const double Sr = c * Sq.x, Si = c * Sq.y, Tr = c * Tq.x, Ti = c * Tq.y;
+                const double mq = mm * qn;
+                // Vt = d S - i m q T, Vp = i m q S + d T; south: d -> -sg d, q -> sg q
+                n0.x += dn * Sr + mq * Ti;
+                n0.y += dn * Si - mq * Tr;
+                n1.x += -mq * Si + dn * Tr;
+                n1.y += mq * Sr + dn * Ti;
+                s0.x += sg * (-dn * Sr + mq * Ti);
+                s0.y += sg * (-dn * Si - mq * Tr);
+                s1.x += sg * (-mq * Si - dn * Tr);
+                s1.y += sg * (mq * Sr - dn * Ti);
+            }
+            if (l < a.ltr) r.step(S.ab[t], l, m);
         }
-        if (l < a.ltr) r.step(a, l);
     }
+    if (!act) return;
     const double g = a.g;  // Robert form: the sums above already carry sin(theta)
     OT *o0 = (OT *)a.out0 + 2 * (size_t)f * a.ld_grid;
     OT *o1 = a.vector ? (OT *)a.out1 + 2 * (size_t)f * a.ld_grid : nullptr;
@@ -146,12 +189,17 @@ __global__ __launch_bounds__(64) void k_ml_synth(const MlArgs a) {
     }
 }
 
+// Analysis: the lanes' terms of 16 degrees go to an LDS tile [4 * degree + q][lane];
+// lane r then adds row r and stores it as this block's partial, acc[field][block]
+// [l - m][4]; k_ml_fin adds the blocks' partials.
 template <typename IT>
 __global__ __launch_bounds__(64) void k_ml_anal(const MlArgs a) {
+    __shared__ MlStage S;
+    __shared__ double red[64][65];
     const int p0 = blockIdx.x * 64 + threadIdx.x, f = blockIdx.y;
     const bool act = p0 < a.np;
     const int p = act ? p0 : 0;
-    const int m = a.m;
+    const int m = a.m, nl = a.ltr - m + 1;
     MlRec r{a.x[p], act ? a.sectv[p] : 0.0, 0.0, act ? a.sectk[p] : 0};
     const double s = a.s[p];
     const bool pole = s == 0.0;
@@ -175,49 +223,65 @@ __global__ __launch_bounds__(64) void k_ml_anal(const MlArgs a) {
         vN0.x *= sc; vN0.y *= sc; vS0.x *= sc; vS0.y *= sc;
         vN1.x *= sc; vN1.y *= sc; vS1.x *= sc; vS1.y *= sc;
     }
-    double *acc = a.acc + 4 * (size_t)f * (a.ltr - m + 1);
+    double *acc = a.acc + 4 * ((size_t)f * gridDim.x + blockIdx.x) * nl;
     const double mm = (double)m;
     double dz = 0.0;
-    for (int l = m; l <= a.ltr; ++l) {
-        const double live = r.k == 0 ? 1.0 : 0.0;
-        const double P = r.cur * live, Pm1 = r.prev * live;
-        if (m == 0 && l > 0) dz = ml_zonal_step(l, r.x, s, Pm1, dz);
-        const double sg = ((l - m) & 1) ? -1.0 : 1.0;
-        double v[4];
-        if (!a.vector) {
-            v[0] = P * (vN0.x + sg * vS0.x);
-            v[1] = P * (vN0.y + sg * vS0.y);
-            v[2] = v[3] = 0.0;
-        } else {
-            double dn = 0.0, qn = 0.0;
-            if (l >= 1) ml_vrow(a, l, P, Pm1, r.x, s, invs, dz, pole, false, dn, qn);
-            // north (d, q) and south (-sg d, sg q) rows:
-            // S += d Vt - i m q Vp,  T += i m q Vt + d Vp
-            const double mq = mm * qn;
-            const double tdx = vN0.x - sg * vS0.x, tdy = vN0.y - sg * vS0.y;  // Vt under d
-            const double tqx = vN0.x + sg * vS0.x, tqy = vN0.y + sg * vS0.y;  // Vt under q
-            const double pdx = vN1.x - sg * vS1.x, pdy = vN1.y - sg * vS1.y;  // Vp under d
-            const double pqx = vN1.x + sg * vS1.x, pqy = vN1.y + sg * vS1.y;  // Vp under q
-            v[0] = dn * tdx + mq * pqy;
-            v[1] = dn * tdy - mq * pqx;
-            v[2] = -mq * tqy + dn * pdx;
-            v[3] = mq * tqx + dn * pdy;
-        }
-        const int nv = a.vector ? 4 : 2;
-        for (int q = 0; q < nv; ++q) {
-            double t = v[q];
+    for (int c0 = m; c0 <= a.ltr; c0 += ML_CH) {
+        const int n = min(ML_CH, a.ltr + 1 - c0);
+        __syncthreads();
+        ml_stage<IT>(a, S, c0, n, nullptr, nullptr, false);
+        __syncthreads();
+#pragma unroll 4
+        for (int t = 0; t < n; ++t) {
+            const int l = c0 + t;
+            const double live = r.k == 0 ? 1.0 : 0.0;
+            const double P = r.cur * live, Pm1 = r.prev * live;
+            const double g = S.g[t];
+            if (m == 0 && l > 0) dz = ml_zonal_step(g, S.gl[t], r.x, s, Pm1, dz);
+            const double sg = ((l - m) & 1) ? -1.0 : 1.0;
+            double v[4];
+            if (!a.vector) {
+                v[0] = P * (vN0.x + sg * vS0.x);
+                v[1] = P * (vN0.y + sg * vS0.y);
+                v[2] = v[3] = 0.0;
+            } else {
+                double dn = 0.0, qn = 0.0;
+                if (l >= 1) ml_vrow(m, l, g, P, Pm1, r.x, s, invs, dz, pole, false, dn, qn);
+                // north (d, q) and south (-sg d, sg q) rows:
+                // S += d Vt - i m q Vp,  T += i m q Vt + d Vp
+                const double mq = mm * qn;
+                const double tdx = vN0.x - sg * vS0.x, tdy = vN0.y - sg * vS0.y;  // Vt under d
+                const double tqx = vN0.x + sg * vS0.x, tqy = vN0.y + sg * vS0.y;  // Vt under q
+                const double pdx = vN1.x - sg * vS1.x, pdy = vN1.y - sg * vS1.y;  // Vp under d
+                const double pqx = vN1.x + sg * vS1.x, pqy = vN1.y + sg * vS1.y;  // Vp under q
+                v[0] = dn * tdx + mq * pqy;
+                v[1] = dn * tdy - mq * pqx;
+                v[2] = -mq * tqy + dn * pdx;
+                v[3] = mq * tqx + dn * pdy;
+            }
+            const int row = 4 * (t & 15);
 #pragma unroll
-            for (int msk = 32; msk >= 1; msk >>= 1) t += __shfl_xor(t, msk);
-            if (threadIdx.x == 0) atomicAdd(acc + 4 * (size_t)(l - m) + q, t);
+            for (int q = 0; q < 4; ++q) red[row + q][threadIdx.x] = v[q];
+            if ((t & 15) == 15 || t == n - 1) {
+                __syncthreads();
+                const int cnt = 4 * ((t & 15) + 1), lfirst = l - (t & 15);
+                if ((int)threadIdx.x < cnt) {
+                    double u = 0.0;
+#pragma unroll 8
+                    for (int i = 0; i < 64; ++i) u += red[threadIdx.x][i];
+                    acc[4 * (size_t)(lfirst - m) + threadIdx.x] = u;
+                }
+                __syncthreads();
+            }
+            if (l < a.ltr) r.step(S.ab[t], l, m);
         }
-        if (l < a.ltr) r.step(a, l);
     }
 }
 
 template <typename OT>
 __global__ __launch_bounds__(256) void k_ml_fin(const MlArgs a) {
-    const int f = blockIdx.x, m = a.m, nl = a.ltr - m + 1;
-    const double *acc = a.acc + 4 * (size_t)f * nl;
+    const int f = blockIdx.x, m = a.m, nl = a.ltr - m + 1, nb = (a.np + 63) / 64;
+    const double *acc = a.acc + 4 * (size_t)f * nb * nl;
     OT *o0 = (OT *)a.out0 + 2 * (size_t)f * a.ld_spec;
     OT *o1 = a.vector ? (OT *)a.out1 + 2 * (size_t)f * a.ld_spec : nullptr;
     for (int t = threadIdx.x; t < nl; t += 256) {
@@ -225,8 +289,12 @@ __global__ __launch_bounds__(256) void k_ml_fin(const MlArgs a) {
         double sc = a.g;
         if (a.conv) sc /= a.conv[l];
         if (a.vector) sc = l > 0 ? sc / ((double)l * (l + 1)) : 0.0;
-        st_c2(o0, t, sc * acc[4 * t], sc * acc[4 * t + 1]);
-        if (a.vector) st_c2(o1, t, sc * acc[4 * t + 2], sc * acc[4 * t + 3]);
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < nb; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] += acc[4 * ((size_t)b * nl + t) + q];
+        st_c2(o0, t, sc * v[0], sc * v[1]);
+        if (a.vector) st_c2(o1, t, sc * v[2], sc * v[3]);
     }
 }
 

@@ -244,7 +244,7 @@ WsLayout ws_layout(const shtk_plan *p, int nf, int vector) {
     // vector synthesis records: 8 doubles per coefficient (k_vspec_prep)
     if (vector) L.total = std::max(L.total, 2 * L.f_bytes + 4 * L.spec_bytes);
     L.zonal_off = L.total;
-    if (vector) L.total += al(sizeof(double) * 4 * (size_t)nf * (p->d.lmax + 1));
+    if (vector) L.total += al(sizeof(double) * 4 * (size_t)nf * ((p->np + 63) / 64) * (p->d.lmax + 1));
     return L;
 }
 
@@ -766,8 +766,6 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
         z.out0 = a.out0;
         z.out1 = a.out1;
         z.acc = (double *)(ws + L.zonal_off);
-        const size_t acc_bytes = sizeof(double) * 4 * (size_t)nf * (j.ltr + 1);
-        HIP_TRY(shtk::launch_zero(z.acc, acc_bytes, st), "accumulator clear");
         HIP_TRY(timed(p, "leg_anal_sphtor_m0", st, [&] { return shtk::launch_ml_anal(z, nf, st); }),
                 "order-0 rows launch");
     }
@@ -1760,7 +1758,7 @@ static int ml_anal(shtk_plan *p, int im, const void *V0, const void *V1, void *S
     std::lock_guard<std::mutex> lk(p->mu);
     DeviceGuard dg(p->device);
     hipStream_t st = (hipStream_t)stream;
-    const size_t acc_bytes = sizeof(double) * 4 * (size_t)nfields * a.ld_spec;
+    const size_t acc_bytes = sizeof(double) * 4 * (size_t)nfields * ((p->np + 63) / 64) * a.ld_spec;
     RC_TRY(ensure_ws(p, acc_bytes));
     a.g = 1.0;
     a.in0 = V0;
@@ -1768,7 +1766,6 @@ static int ml_anal(shtk_plan *p, int im, const void *V0, const void *V1, void *S
     a.out0 = S;
     a.out1 = T;
     a.acc = (double *)p->ws;
-    HIP_TRY(shtk::launch_zero(a.acc, acc_bytes, st), "accumulator clear");
     HIP_TRY(shtk::launch_ml_anal(a, nfields, st), "single-order analysis launch");
     return SHTK_OK;
 }

@@ -327,9 +327,10 @@ hipError_t launch_ml_anal(const MlArgs &a, int nfields, hipStream_t st);
 // Pole rows of order 1 for the vector transforms on grids with nodes at the
 // poles (one block per field): synthesis rewrites them from the prologue
 // records in a.in0 times wscale (robert = 1: zeros every order's pole rows),
-// analysis adds the pole terms times wpole to the order-1 outputs in a.out0 / a.out1.
+// analysis adds the pole terms times wpole to the order-1 outputs in a.out0 / a.out1
+// (conv: the outputs are in the plan's convention map, or nullptr).
 hipError_t launch_vpole_synth(const LegArgs &a, int nfields, int robert, double wscale, hipStream_t st);
-hipError_t launch_vpole_anal(const LegArgs &a, int nfields, double wpole, hipStream_t st);
+hipError_t launch_vpole_anal(const LegArgs &a, int nfields, double wpole, const double *conv, hipStream_t st);
 // Complex-valued synthesis FFT: bins m = 0 .. mmax of F -> complex grid rows.
 hipError_t launch_fft_cplx(const FftArgs &a, int nfields, hipStream_t st);
 // Split-precision vector transforms: (S, T) -> coefficients of 2 nf scalar

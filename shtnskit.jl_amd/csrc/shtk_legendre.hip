@@ -746,9 +746,10 @@ __global__ __launch_bounds__(256) void k_vpole_synth(const LegArgs a, int robert
 }
 
 // Analysis: adds the weighted pole rows of order 1 to S_l1, T_l1 (after the
-// Legendre analysis wrote them), S += w [dP Vt - i P/s Vp], T += w [i P/s Vt + dP Vp].
+// Legendre analysis wrote them), S += w [dP Vt - i P/s Vp], T += w [i P/s Vt + dP Vp];
+// conv (optional): the outputs are in the plan's convention, divide by it.
 template <typename OT>
-__global__ __launch_bounds__(256) void k_vpole_anal(const LegArgs a, double wpole) {
+__global__ __launch_bounds__(256) void k_vpole_anal(const LegArgs a, double wpole, const double *conv) {
     const int f = blockIdx.x, nlat = a.nlat;
     const size_t fo = ((size_t)f * a.nm + 1) * nlat;
     const double2 tN = a.F0[fo], pN = a.F1[fo], tS = a.F0[fo + nlat - 1], pS = a.F1[fo + nlat - 1];
@@ -761,6 +762,7 @@ __global__ __launch_bounds__(256) void k_vpole_anal(const LegArgs a, double wpol
         const double dN = -h, qN = -h, dS = -hs, qS = hs;  // dP/dtheta and P/sin(theta)
         double sc = gm;
         if (a.ll1_mode) sc /= (double)l * (l + 1);
+        if (conv) sc /= conv[(size_t)(a.lmax + 1) + l];
         const double sr = dN * tN.x + qN * pN.y + dS * tS.x + qS * pS.y;
         const double si = dN * tN.y - qN * pN.x + dS * tS.y - qS * pS.x;
         const double tr = -qN * tN.y + dN * pN.x - qS * tS.y + dS * pS.x;
@@ -1277,11 +1279,11 @@ hipError_t launch_vpole_synth(const LegArgs &a, int nfields, int robert, double 
     return hipGetLastError();
 }
 
-hipError_t launch_vpole_anal(const LegArgs &a, int nfields, double wpole, hipStream_t st) {
+hipError_t launch_vpole_anal(const LegArgs &a, int nfields, double wpole, const double *conv, hipStream_t st) {
     if (a.io_f32)
-        hipLaunchKernelGGL(k_vpole_anal<float>, dim3((unsigned)nfields), dim3(256), 0, st, a, wpole);
+        hipLaunchKernelGGL(k_vpole_anal<float>, dim3((unsigned)nfields), dim3(256), 0, st, a, wpole, conv);
     else
-        hipLaunchKernelGGL(k_vpole_anal<double>, dim3((unsigned)nfields), dim3(256), 0, st, a, wpole);
+        hipLaunchKernelGGL(k_vpole_anal<double>, dim3((unsigned)nfields), dim3(256), 0, st, a, wpole, conv);
     return hipGetLastError();
 }
 

@@ -564,13 +564,6 @@ struct Job {
     int pole_fix = 0;
 };
 
-// Vector transforms on grids with nodes at the poles: the FP64 forward pair and its
-// adjoints take the pole limits of order 1; the split forms are not provided.
-int pole_vector_check(const shtk_plan *p, const char *what) {
-    if (!p->poles) return SHTK_OK;
-    return fail(SHTK_EUNSUPPORTED, std::string(what) + " is not provided on grids with nodes at the poles");
-}
-
 // Order 0 of the FP64 vector transforms, rewritten by the single-order kernels:
 // near the poles D_l = l x P_l - g_l P_{l-1} cancels for m = 0 (the terms are ~l P,
 // the difference ~sin^2), so dP_l/dtheta of order 0 comes from the differentiated
@@ -770,7 +763,7 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
                 "order-0 rows launch");
     }
     if (j.pole_fix && p->poles && p->d.mres == 1 && p->d.mmax >= 1)
-        HIP_TRY(shtk::launch_vpole_anal(a, nf, j.pole_fix == 2 ? 1.0 : p->hw[0] * p->scale_phi, st),
+        HIP_TRY(shtk::launch_vpole_anal(a, nf, j.pole_fix == 2 ? 1.0 : p->hw[0] * p->scale_phi, nullptr, st),
                 "pole rows launch");
     if (!p->canonical) {
         void *dst[2] = {s0, s1};
@@ -866,6 +859,10 @@ int run_vsplit_to_grid(shtk_plan *p, const Job &j, const void *S, const void *T,
     a.nm_run = nm_lo;
     HIP_TRY(timed(p, "leg_synth_sphtor_low", st, [&] { return shtk::launch_leg_synth(a, nf, 1, p->synth_chains, st); }),
             "Legendre synthesis launch");
+    // grids with poles: order-1 pole rows from the limits (the phi transform divides
+    // those rows by 1), Robert form: every pole row zero
+    if (p->poles && (p->d.robert_form || (p->d.mres == 1 && p->d.mmax >= 1)))
+        HIP_TRY(shtk::launch_vpole_synth(a, nf, p->d.robert_form, 1.0, st), "pole rows launch");
     void *grids[2] = {Vt, Vp};
     for (int q = 0; q < 2; ++q) {
         shtk::FftArgs fa = fft_args(p);
@@ -914,6 +911,22 @@ int run_vsplit_to_spec(shtk_plan *p, const Job &j, const void *Vt, const void *V
     HIP_TRY(shtk::launch_vsplit_out(G, S, T, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale, j.conv_div,
                                     p->d.lmax, p->d.mmax, p->d.mres, j.ltr, g_fs, out_fs, nf, st),
             "vector coefficient combination");
+    // grids with poles: the weights w/sin(theta) dropped the pole rows; order 1 takes
+    // their limit terms, added to the outputs in the plan's convention
+    if (p->poles && p->d.mres == 1 && p->d.mmax >= 1) {
+        shtk::LegArgs a = base_args(p, j.ltr);
+        a.F0 = F0;
+        a.F1 = F1;
+        a.g = j.g;
+        a.ll1_mode = j.ll1_mode;
+        a.mfac_mode = j.mfac_mode;
+        a.out0 = S;
+        a.out1 = T;
+        a.io_f32 = dtype == SHTK_F32;
+        a.spec_fstride = out_fs;
+        HIP_TRY(shtk::launch_vpole_anal(a, nf, p->hw[0] * p->scale_phi, p->canonical ? nullptr : p->convscale, st),
+                "pole rows launch");
+    }
     return SHTK_OK;
 }
 
@@ -1669,7 +1682,6 @@ int shtk_synthesis_sphtor_ex(shtk_plan *p, const void *S, const void *T, void *V
     if (precision != SHTK_PREC_SPLIT) return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
     RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
     if (p->d.lmax > 4094) return fail(SHTK_EUNSUPPORTED, "split-precision vector transforms support lmax <= 4094");
-    RC_TRY(pole_vector_check(p, "split-precision vector synthesis"));
     RC_TRY(norm_ltr(p, ltr, &ltr));
     // the contraction gives sin(theta) V; the phi FFT divides by sin(theta) unless Robert form
     Job j{1, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, p->d.robert_form ? 0 : 1, ltr, "oz_synth_sphtor",
@@ -1685,8 +1697,10 @@ int shtk_analysis_sphtor_ex(shtk_plan *p, const void *Vt, const void *Vp, void *
     if (precision != SHTK_PREC_SPLIT) return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
     RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
     if (p->d.lmax > 4094) return fail(SHTK_EUNSUPPORTED, "split-precision vector transforms support lmax <= 4094");
-    RC_TRY(pole_vector_check(p, "split-precision vector analysis"));
     RC_TRY(norm_ltr(p, ltr, &ltr));
+    if (p->pole_weights && p->d.robert_form && p->d.mres == 1 && p->d.mmax >= 1 && ltr >= 1)
+        return fail(SHTK_EINVAL, "Robert-form components with m=1 cannot be unweighted at nonzero-weight pole "
+                                 "nodes; use a Gauss or pole-free regular grid");
     Job j{1, 1, p->d.robert_form ? p->winvs2 : p->winvs, 1.0, 0, 1, 0, ltr, "oz_anal_sphtor", "fft_anal"};
     j.prec = precision;
     return run_vsplit_to_spec(p, j, Vt, Vp, S, T, nfields, dtype, stream);

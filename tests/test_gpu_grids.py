@@ -101,11 +101,11 @@ def test_sphtor_match_oracle(shtk, oracle, cuda, grid, lmax, nlat, nlon, mmax, m
     S_r, T_r = ocfg.analysis_sphtor(gt, gp)
     S2, T2 = shtk.analysis_sphtor_batch(cfg, to_dev(gt), to_dev(gp))
     assert rel_l2(to_host(S2), S_r) < TOL and rel_l2(to_host(T2), T_r) < TOL
-    if grid == "regular":
-        Vts, Vps = shtk.synthesis_sphtor_batch(cfg, to_dev(S), to_dev(T), precision="split")
-        Ss, Ts = shtk.analysis_sphtor_batch(cfg, to_dev(gt), to_dev(gp), precision="split")
-        assert rel_l2(to_host(Vts), Vt_r) < 1e-11 and rel_l2(to_host(Vps), Vp_r) < 1e-11
-        assert rel_l2(to_host(Ss), S_r) < 1e-11 and rel_l2(to_host(Ts), T_r) < 1e-11
+    # split precision on both grids (pole rows from the limits on the pole grid)
+    Vts, Vps = shtk.synthesis_sphtor_batch(cfg, to_dev(S), to_dev(T), precision="split")
+    Ss, Ts = shtk.analysis_sphtor_batch(cfg, to_dev(gt), to_dev(gp), precision="split")
+    assert rel_l2(to_host(Vts), Vt_r) < 1e-11 and rel_l2(to_host(Vps), Vp_r) < 1e-11
+    assert rel_l2(to_host(Ss), S_r) < 1e-11 and rel_l2(to_host(Ts), T_r) < 1e-11
 
 
 def test_pole_grid_f32_and_truncation(shtk, oracle, cuda):
@@ -147,15 +147,29 @@ def test_vector_adjoints_pole_grids(shtk, oracle, cuda, grid, robert):
     assert rel_l2(to_host(vt), vt_r.real) < TOL and rel_l2(to_host(vp), vp_r.real) < TOL
 
 
-def test_pole_grid_unsupported_forms(shtk, oracle, cuda):
-    """Split vector transforms are refused on pole grids."""
-    cfg, _ = _pair(shtk, oracle, 8, 17, "regular_poles")
-    S = to_dev(_alm(oracle, 8, 8, 1, 1, lmin=1))
-    g = torch.zeros((17, cfg.nlon), dtype=torch.float64, device="cuda:0").t().contiguous().t()
-    with pytest.raises(shtk.BackendUnavailableError):
-        shtk.synthesis_sphtor(cfg, S, S, precision="split")
-    with pytest.raises(shtk.BackendUnavailableError):
-        shtk.analysis_sphtor(cfg, g, g, precision="split")
+@pytest.mark.parametrize("norm,rn,robert", [("schmidt", True, False), ("orthonormal", False, True)])
+def test_pole_grid_split_vector(shtk, oracle, cuda, norm, rn, robert):
+    """Split vector transforms on the pole grid: non-canonical conventions (the pole
+    terms are added in the plan's convention) and Robert form (zero pole rows;
+    the analysis raises with order 1 stored)."""
+    lmax, nlat, nlon = 63, 127, 128
+    cfg, ocfg = _pair(shtk, oracle, lmax, nlat, "regular_poles", nlon=nlon, norm=norm, real_norm=rn,
+                      robert_form=robert)
+    S = _alm(oracle, lmax, lmax, 1, 51, nf=2, lmin=1)
+    T = _alm(oracle, lmax, lmax, 1, 52, nf=2, lmin=1)
+    Vt_r, Vp_r = ocfg.synthesis_sphtor(S, T)
+    Vt, Vp = shtk.synthesis_sphtor_batch(cfg, to_dev(S), to_dev(T), precision="split")
+    assert rel_l2(to_host(Vt), Vt_r) < 1e-11 and rel_l2(to_host(Vp), Vp_r) < 1e-11
+    rng = np.random.default_rng(9)
+    gt = np.asfortranarray(rng.standard_normal((nlat, nlon, 2)))
+    gp = np.asfortranarray(rng.standard_normal((nlat, nlon, 2)))
+    if robert:
+        with pytest.raises(ValueError):
+            shtk.analysis_sphtor_batch(cfg, to_dev(gt), to_dev(gp), precision="split")
+        return
+    S_r, T_r = ocfg.analysis_sphtor(gt, gp)
+    S2, T2 = shtk.analysis_sphtor_batch(cfg, to_dev(gt), to_dev(gp), precision="split")
+    assert rel_l2(to_host(S2), S_r) < 1e-11 and rel_l2(to_host(T2), T_r) < 1e-11
 
 
 @pytest.mark.parametrize("fid", ["packed_storage", "qst_batch", "sphtor_l", "qst_l"])
@@ -252,6 +266,10 @@ def test_driscoll_healy_match_oracle(shtk, oracle, cuda, lmax, nlon, mmax, mres,
     S_r, T_r = ocfg.analysis_sphtor(gt, gp)
     S2, T2 = shtk.analysis_sphtor_batch(cfg, to_dev(gt), to_dev(gp))
     assert rel_l2(to_host(S2), S_r) < TOL and rel_l2(to_host(T2), T_r) < TOL
+    Vts, Vps = shtk.synthesis_sphtor_batch(cfg, to_dev(S), to_dev(T), precision="split")
+    assert rel_l2(to_host(Vts), Vt_r) < 1e-11 and rel_l2(to_host(Vps), Vp_r) < 1e-11
+    S3, T3 = shtk.analysis_sphtor_batch(cfg, to_dev(gt), to_dev(gp), precision="split")
+    assert rel_l2(to_host(S3), S_r) < 1e-11 and rel_l2(to_host(T3), T_r) < 1e-11
     # single order on the same grid
     im = 1 if mres == 1 else 0
     q = S[im * mres:, im * mres, 0]

@@ -156,7 +156,10 @@ int shtk_synthesis_sphtor(shtk_plan *plan, const void *S, const void *T, void *V
 
 /* Vector analysis (Vt, Vp) -> (S, T): analysis_sphtor / analysis_sphtor_batch
  * (src/sphtor_transforms.jl:252-289, src/batch_transforms.jl:689-725), behind
- * _gpu_adapter_analysis_sphtor (src/sphtor_transforms.jl:298-302). */
+ * _gpu_adapter_analysis_sphtor (src/sphtor_transforms.jl:298-302).  On grids with
+ * nodes at the poles the order-1 rows use the pole limits (src/kernels.jl:72-99);
+ * Robert form with order 1 stored and nonzero pole weights (Clenshaw-Curtis) is
+ * SHTK_EINVAL, as _validate_robert_analysis (src/config.jl:608-624). */
 int shtk_analysis_sphtor(shtk_plan *plan, const void *Vt, const void *Vp, void *S, void *T, int nfields,
                          int dtype, int ltr, void *stream);
 

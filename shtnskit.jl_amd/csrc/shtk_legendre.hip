@@ -1152,13 +1152,38 @@ __global__ __launch_bounds__(256) void k_spec_qscale(const IT *in, double *out, 
     const int im = blockIdx.x % nm, f = blockIdx.x / nm, m = im * mres;
     const double *cm = qcc + coff[im] - m;
     const long long base = (long long)f * per_field + (long long)(lmax + 1) * m;
-    for (int l = m + (int)threadIdx.x; l <= lmax; l += 256) {
-        const long long q = base + l;
+    auto factor = [&](int l, long long q) {
         double s = cm[l];
         if (scale) {
             const double cs = scale[q - (long long)f * per_field];
             s = divide ? s / cs : s * cs;
         }
+        return s;
+    };
+    // the first 4 x 256 degrees: every load issued before the first store
+    constexpr int U = 4;
+    double vr[U], vi[U], sc[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const int l = m + (int)threadIdx.x + 256 * k;
+        const long long q = base + l;
+        const bool ok = l <= lmax;
+        vr[k] = ok ? (double)in[2 * q] : 0.0;
+        vi[k] = ok ? (double)in[2 * q + 1] : 0.0;
+        sc[k] = ok ? factor(l, q) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const int l = m + (int)threadIdx.x + 256 * k;
+        const long long q = base + l;
+        if (l <= lmax) {
+            out[2 * q] = sc[k] * vr[k];
+            out[2 * q + 1] = sc[k] * vi[k];
+        }
+    }
+    for (int l = m + (int)threadIdx.x + 256 * U; l <= lmax; l += 256) {
+        const long long q = base + l;
+        const double s = factor(l, q);
         out[2 * q] = s * (double)in[2 * q];
         out[2 * q + 1] = s * (double)in[2 * q + 1];
     }

@@ -143,10 +143,18 @@ int shtk_synthesis_ex(shtk_plan *plan, const void *alm, void *f, int nfields, in
 
 /* Batched scalar analysis with a contraction precision: the batch analysis of
  * src/batch_transforms.jl:337-429 (one Legendre row per (m, latitude) shared by
- * all fields, :403-418).  SHTK_PREC_SPLIT accumulates with device atomics, so
- * its last bits may differ from run to run. */
+ * all fields, :403-418).  SHTK_PREC_SPLIT stores every coefficient once when
+ * nlat <= 1024 (bit-reproducible); larger grids add the partial sums of several
+ * latitude blocks with device atomics, so their last bits may differ from run to run. */
 int shtk_analysis_ex(shtk_plan *plan, const void *f, void *alm, int nfields, int dtype, int ltr, int precision,
                      void *stream);
+
+/* Scalar analysis of complex fields: f (nlat, nlon, nfields) complex (interleaved
+ * re/im of dtype) -> alm.  analysis with a complex eltype (src/core_transforms.jl:
+ * 169-207): the m-loop reads the bins m >= 0 of the complex FFT (no Hermitian
+ * fold).  precision as shtk_analysis_ex. */
+int shtk_analysis_cplx(shtk_plan *plan, const void *f, void *alm, int nfields, int dtype, int ltr, int precision,
+                       void *stream);
 
 /* Vector synthesis (S, T) -> (Vt, Vp): synthesis_sphtor / synthesis_sphtor_batch
  * (src/sphtor_transforms.jl:162-241, src/batch_transforms.jl:745-815), behind

@@ -51,6 +51,7 @@ def lib():
         _lib.ora_coef_scale.restype = d
         _lib.ora_synthesis.argtypes = [p, p, p, i, i, i]
         _lib.ora_analysis.argtypes = [p, p, p, i, i]
+        _lib.ora_analysis_cplx.argtypes = [p, p, p, i, i]
         _lib.ora_synthesis_sphtor.argtypes = [p, p, p, p, p, i, i, i]
         _lib.ora_analysis_sphtor.argtypes = [p, p, p, p, p, i, i]
         _lib.ora_adjoint_synthesis.argtypes = [p, p, p, i, i]
@@ -286,6 +287,14 @@ class Config:
         nf = self._nf(f, 2)
         alm = self._spec(nf)
         lib().ora_analysis(self.cref, _ptr(f), _ptr(alm), nf, ltr)
+        return alm if f.ndim == 3 else alm[:, :, 0]
+
+    def analysis_cplx(self, f, ltr=-1):
+        """Analysis of complex fields (bins m >= 0 of the complex FFT)."""
+        f = np.asfortranarray(f, dtype=np.complex128)
+        nf = self._nf(f, 2)
+        alm = self._spec(nf)
+        lib().ora_analysis_cplx(self.cref, _ptr(f), _ptr(alm), nf, ltr)
         return alm if f.ndim == 3 else alm[:, :, 0]
 
     def synthesis_sphtor(self, S, T, real_output=True, ltr=-1):

@@ -455,13 +455,31 @@ int ora_synthesis(const ora_cfg *c, const double *alm, double *out, int nf, int 
 }
 
 /* Scalar analysis of real fields: core_transforms.jl:169-207 (nf=1) and the
- * batch form batch_transforms.jl:372-429 (row shared, scale applied last). */
+ * batch form batch_transforms.jl:372-429 (row shared, scale applied last).
+ * ora_analysis_cplx takes complex fields (interleaved re/im): the same m-loop
+ * over the bins m >= 0 of the complex FFT (core_transforms.jl:196-206 with a
+ * complex eltype). */
+static int analysis_fourier(const ora_cfg *c, cplx *F, double *alm, int nf, int ltr);
+
 int ora_analysis(const ora_cfg *c, const double *f, double *alm, int nf, int ltr) {
+    size_t nspat = (size_t)c->nlat * c->nlon;
+    cplx *F = (cplx *)malloc(sizeof(cplx) * nspat * nf);
+    for (size_t t = 0; t < nspat * nf; ++t) F[t] = f[t];
+    return analysis_fourier(c, F, alm, nf, ltr);
+}
+
+int ora_analysis_cplx(const ora_cfg *c, const double *f, double *alm, int nf, int ltr) {
+    size_t nspat = (size_t)c->nlat * c->nlon;
+    cplx *F = (cplx *)malloc(sizeof(cplx) * nspat * nf);
+    for (size_t t = 0; t < nspat * nf; ++t) F[t] = f[2 * t] + I * f[2 * t + 1];
+    return analysis_fourier(c, F, alm, nf, ltr);
+}
+
+/* Takes ownership of F (grid values, transformed and freed here). */
+static int analysis_fourier(const ora_cfg *c, cplx *F, double *alm, int nf, int ltr) {
     int lmax = c->lmax, mmax = c->mmax, nlat = c->nlat, nlon = c->nlon;
     if (ltr < 0 || ltr > lmax) ltr = lmax;
     size_t nalm = (size_t)(lmax + 1) * (mmax + 1), nspat = (size_t)nlat * nlon;
-    cplx *F = (cplx *)malloc(sizeof(cplx) * nspat * nf);
-    for (size_t t = 0; t < nspat * nf; ++t) F[t] = f[t];
     for (int k = 0; k < nf; ++k) fft_phi(F + nspat * k, nlat, nlon, 0);
     memset(alm, 0, sizeof(double) * 2 * nalm * nf);
     double scale_phi = analysis_phi_scale(c);

@@ -20,6 +20,40 @@ from . import _lib
 from .config import SHTConfig
 from .transforms import _PRECISIONS, _code, _colmajor, _empty_colmajor, _ptr, _stream
 
+_SPEC_DTYPES = (torch.complex64, torch.complex128)
+_GRID_DTYPES = (torch.float32, torch.float64)
+
+
+def check_group_arrays(cfgs, nfields, xs, kind, op):
+    """Validates the per-group arrays of an uneven batch call and returns them
+    column-major: one array per group of shape (lmax+1, mmax+1, nf) (kind
+    "spec", complex64/complex128) or (nlat, nlon, nf) (kind "grid",
+    float32/float64), every group of the same precision and on one device.
+    The batch kernels read every group with the one dtype they are given, so a
+    mismatch is an error, never a conversion."""
+    xs = list(xs)
+    if len(xs) != len(cfgs):
+        raise ValueError(f"{op}: expected {len(cfgs)} arrays, got {len(xs)}")
+    allowed = _SPEC_DTYPES if kind == "spec" else _GRID_DTYPES
+    out = []
+    for g, (c, nf, x) in enumerate(zip(cfgs, nfields, xs)):
+        want = ((c.lmax + 1, c.mmax + 1) if kind == "spec" else (c.nlat, c.nlon)) + (nf,)
+        if tuple(x.shape) != want:
+            raise ValueError(f"{op}: array of shape {tuple(x.shape)} where {want} is expected")
+        if x.dtype not in allowed:
+            what = "complex64 or complex128 coefficients" if kind == "spec" else "float32 or float64 grids"
+            raise ValueError(f"{op}: group {g} has dtype {x.dtype}; the batch takes {what}")
+        if x.dtype != xs[0].dtype:
+            raise ValueError(f"{op}: group {g} has dtype {x.dtype}, group 0 {xs[0].dtype}; "
+                             "every group must have the same precision")
+    for x in xs:
+        if not x.is_cuda:
+            raise ValueError(f"{op}: arrays must be device tensors")
+        if x.device != xs[0].device:
+            raise ValueError(f"{op}: every array must be on one device")
+        out.append(_colmajor(x))
+    return out
+
 
 class UnevenBatch:
     """``UnevenBatch(cfgs, nfields, precision="split", graph=False)``: group g holds
@@ -44,22 +78,12 @@ class UnevenBatch:
         if graph:
             _lib.check(lib.shtk_batch_set_graph(ctypes.c_void_p(self._b), 1), "UnevenBatch")
 
-    def _arrays(self, xs, shape_of, op):
-        if len(xs) != len(self.cfgs):
-            raise ValueError(f"{op}: expected {len(self.cfgs)} arrays, got {len(xs)}")
-        out = []
-        for c, nf, x in zip(self.cfgs, self.nfields, xs):
-            want = shape_of(c) + (nf,)
-            if tuple(x.shape) != want:
-                raise ValueError(f"{op}: array of shape {tuple(x.shape)} where {want} is expected")
-            if not x.is_cuda:
-                raise ValueError(f"{op}: arrays must be device tensors")
-            out.append(_colmajor(x))
-        return out
+    def _arrays(self, xs, kind, op):
+        return check_group_arrays(self.cfgs, self.nfields, xs, kind, op)
 
     def synthesis(self, alms: Sequence[torch.Tensor]) -> list:
         """Coefficients (lmax+1, mmax+1, nf) per group -> grids (nlat, nlon, nf)."""
-        alms = self._arrays(alms, lambda c: (c.lmax + 1, c.mmax + 1), "synthesis")
+        alms = self._arrays(alms, "spec", "synthesis")
         rdt = torch.float32 if alms[0].dtype == torch.complex64 else torch.float64
         outs = [_empty_colmajor((c.nlat, c.nlon, nf), rdt, alms[0].device) for c, nf in zip(self.cfgs, self.nfields)]
         ins = (ctypes.c_void_p * len(alms))(*[_ptr(a).value for a in alms])
@@ -70,7 +94,7 @@ class UnevenBatch:
 
     def analysis(self, fields: Sequence[torch.Tensor]) -> list:
         """Grids (nlat, nlon, nf) per group -> coefficients (lmax+1, mmax+1, nf)."""
-        fields = self._arrays(fields, lambda c: (c.nlat, c.nlon), "analysis")
+        fields = self._arrays(fields, "grid", "analysis")
         rdt = fields[0].dtype
         cdt = torch.complex64 if rdt == torch.float32 else torch.complex128
         outs = [_empty_colmajor((c.lmax + 1, c.mmax + 1, nf), cdt, fields[0].device)
@@ -89,8 +113,10 @@ class UnevenBatch:
     def autotune(self, alms: Sequence[torch.Tensor], fields: Sequence[torch.Tensor], reps: int = 3) -> dict:
         """Times every candidate launch shape on these arrays (round trips that
         overwrite them) and keeps the fastest; returns the choice."""
-        alms = self._arrays(alms, lambda c: (c.lmax + 1, c.mmax + 1), "autotune")
-        fields = self._arrays(fields, lambda c: (c.nlat, c.nlon), "autotune")
+        alms = self._arrays(alms, "spec", "autotune")
+        fields = self._arrays(fields, "grid", "autotune")
+        if (alms[0].dtype == torch.complex64) != (fields[0].dtype == torch.float32):
+            raise ValueError("autotune: coefficients and grids must have the same precision")
         rdt = fields[0].dtype
         ins = (ctypes.c_void_p * len(alms))(*[_ptr(a).value for a in alms])
         outs = (ctypes.c_void_p * len(fields))(*[_ptr(f).value for f in fields])

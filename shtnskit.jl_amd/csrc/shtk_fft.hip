@@ -12,8 +12,10 @@
 // order).  Two real rows travel as one complex row z = f_a + i f_b (latitudes
 // k and 2G-1-k of the tile), so a single complex FFT of length N serves both
 // rows (no separate real-FFT pass).  The rows stay resident in LDS (16 N bytes
-// each) through a Stockham autosort FFT with radix 8/4/2/3/5 passes; twiddles
-// come from a per-plan table.  Consecutive tiles are mapped to the same XCD so
+// each) through a Stockham autosort FFT with radix 8/4/2/3/5 passes and generic
+// passes for other small primes; lengths with larger prime factors run
+// Bluestein's chirp convolution inside the same LDS row (of the convolution
+// length).  Twiddles come from per-plan tables.  Consecutive tiles are mapped to the same XCD so
 // the L2 merges the strided longitude-major stores of neighbouring rows.
 #include <algorithm>
 
@@ -151,21 +153,97 @@ __device__ __forceinline__ void stockham_pass(double2 *z, int N, int Ns, const d
     __syncthreads();
 }
 
+// Generic radix-R pass (R a prime above 5): every output is the R-term sum
+// out(j, r) = sum_s z[j + s nb] W^(s (k + r Ns) stride), k = j mod Ns, with the
+// twiddles read from the table (exact angles, no accumulated rounding).  A
+// thread computes up to FFT_VPT outputs (nt >= N / FFT_VPT).
 template <int SIGN>
-__device__ void fft_lds(double2 *z, const FftArgs &a, int tid, int nt) {
-    const int N = a.nlon;
+__device__ void stockham_gen(double2 *z, int N, int Ns, int R, const double2 *tw, int tid, int nt) {
+    const int nb = N / R, stride = N / (Ns * R);
+    double2 out[FFT_VPT];
+#pragma unroll
+    for (int q = 0; q < FFT_VPT; ++q) {
+        const int e = tid + q * nt;
+        out[q] = make_double2(0.0, 0.0);
+        if (e < N) {
+            const int j = e % nb, r = e / nb, k = j % Ns;
+            const int step = (k + r * Ns) * stride;  // < N
+            double2 acc = z[zp(j)];
+            int t = 0;
+            for (int s = 1; s < R; ++s) {
+                t += step;
+                if (t >= N) t -= N;
+                double2 w = tw[t];
+                if (SIGN > 0) w.y = -w.y;
+                const double2 v = z[zp(j + s * nb)];
+                acc = make_double2(fma(v.x, w.x, fma(-v.y, w.y, acc.x)), fma(v.x, w.y, fma(v.y, w.x, acc.y)));
+            }
+            out[q] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < FFT_VPT; ++q) {
+        const int e = tid + q * nt;
+        if (e < N) {
+            const int j = e % nb, r = e / nb, k = j % Ns;
+            z[zp((j / Ns) * Ns * R + k + r * Ns)] = out[q];
+        }
+    }
+    __syncthreads();
+}
+
+// All passes of a length-N transform of the LDS row z (unnormalised).
+template <int SIGN>
+__device__ void fft_passes(double2 *z, int N, int nrad, unsigned long long radix, const double2 *tw, int tid,
+                           int nt) {
     int Ns = 1;
-    for (int q = 0; q < a.nrad; ++q) {
-        const int R = (int)((a.radix >> (4 * q)) & 15ull);
+    for (int q = 0; q < nrad; ++q) {
+        const int R = (int)((radix >> (FFT_RADIX_BITS * q)) & ((1ull << FFT_RADIX_BITS) - 1));
         switch (R) {
-            case 8: stockham_pass<8, SIGN>(z, N, Ns, a.tw, tid, nt); break;
-            case 4: stockham_pass<4, SIGN>(z, N, Ns, a.tw, tid, nt); break;
-            case 2: stockham_pass<2, SIGN>(z, N, Ns, a.tw, tid, nt); break;
-            case 3: stockham_pass<3, SIGN>(z, N, Ns, a.tw, tid, nt); break;
-            default: stockham_pass<5, SIGN>(z, N, Ns, a.tw, tid, nt); break;
+            case 8: stockham_pass<8, SIGN>(z, N, Ns, tw, tid, nt); break;
+            case 4: stockham_pass<4, SIGN>(z, N, Ns, tw, tid, nt); break;
+            case 2: stockham_pass<2, SIGN>(z, N, Ns, tw, tid, nt); break;
+            case 3: stockham_pass<3, SIGN>(z, N, Ns, tw, tid, nt); break;
+            case 5: stockham_pass<5, SIGN>(z, N, Ns, tw, tid, nt); break;
+            default: stockham_gen<SIGN>(z, N, Ns, R, tw, tid, nt); break;
         }
         Ns *= R;
     }
+}
+
+// Length-nlon DFT of the LDS row z (entries at zp(j), j < nlon; the row holds
+// nfft + nfft/16 + 1 slots).  Direct passes, or Bluestein: chirp, zero-pad to
+// nfft, cyclic convolution with the kernel spectrum (forward FFT, pointwise
+// product, inverse FFT), chirp.  The caller's barrier precedes; one follows.
+template <int SIGN>
+__device__ void fft_lds(double2 *z, const FftArgs &a, int tid, int nt) {
+    if (!a.blue) {
+        fft_passes<SIGN>(z, a.nlon, a.nrad, a.radix, a.tw, tid, nt);
+        return;
+    }
+    const int N = a.nlon, M = a.nfft;
+    for (int j = tid; j < M; j += nt) {
+        if (j < N) {
+            double2 c = a.chirp[j];
+            if (SIGN > 0) c.y = -c.y;
+            z[zp(j)] = cmul(z[zp(j)], c);
+        } else {
+            z[zp(j)] = make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    fft_passes<-1>(z, M, a.nrad, a.radix, a.tw, tid, nt);
+    const double2 *bh = a.bhat + (SIGN > 0 ? M : 0);
+    for (int j = tid; j < M; j += nt) z[zp(j)] = cmul(z[zp(j)], bh[j]);
+    __syncthreads();
+    fft_passes<+1>(z, M, a.nrad, a.radix, a.tw, tid, nt);
+    for (int j = tid; j < N; j += nt) {
+        double2 c = a.chirp[j];
+        if (SIGN > 0) c.y = -c.y;
+        z[zp(j)] = cmul(z[zp(j)], c);
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -216,7 +294,7 @@ template <int G, typename GT>
 __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfields) {
     extern __shared__ double2 zs[];
     constexpr int L = 2 * G;
-    const int N = a.nlon, nlat = a.nlat, RS = N + (N >> 4) + 1;
+    const int N = a.nlon, nlat = a.nlat, RS = a.nfft + (a.nfft >> 4) + 1;
     const int tid = threadIdx.x, nt = blockDim.x, tr = nt / G;
     const int row = tid / tr, lt = tid % tr;
     const int k = tid % L;                // lane role within its group of 2G
@@ -282,7 +360,7 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
 template <int G, typename GT>
 __global__ __launch_bounds__(1024) void k_fft_synth_c(const FftArgs a, int nfields) {
     extern __shared__ double2 zs[];
-    const int N = a.nlon, nlat = a.nlat, RS = N + (N >> 4) + 1;
+    const int N = a.nlon, nlat = a.nlat, RS = a.nfft + (a.nfft >> 4) + 1;
     const int tid = threadIdx.x, nt = blockDim.x, tr = nt / G;
     const int row = tid / tr, lt = tid % tr;
     const int k = tid % G;  // latitude of this lane within the tile
@@ -315,13 +393,52 @@ __global__ __launch_bounds__(1024) void k_fft_synth_c(const FftArgs a, int nfiel
     }
 }
 
+// Complex-valued analysis (analysis of a complex field, src/core_transforms.jl:
+// 169-207 with a complex eltype): each latitude's complex row is transformed
+// and the bins m = 0 .. mmax are kept, the columns the m-loop reads
+// (Fph[:, m+1]).  Tiles of G adjacent latitudes, one LDS row each.
+template <int G, typename GT>
+__global__ __launch_bounds__(1024) void k_fft_anal_c(const FftArgs a, int nfields) {
+    extern __shared__ double2 zs[];
+    const int N = a.nlon, nlat = a.nlat, RS = a.nfft + (a.nfft >> 4) + 1;
+    const int tid = threadIdx.x, nt = blockDim.x, tr = nt / G;
+    const int row = tid / tr, lt = tid % tr;
+    const int k = tid % G;  // latitude of this lane within the tile
+    const int ntile = (nlat + G - 1) / G, total = ntile * nfields;
+    for (int u = blockIdx.x; u < total; u += gridDim.x) {
+        const int f = u / ntile, lat = (u % ntile) * G + k;
+        const bool live = lat < nlat;
+        const int nrow = a.nrow;
+        const bool has_row = live && lat < nrow;  // rows past nrow read as zero
+        const double sc = live ? fft_row_scale(a, lat) : 0.0;
+        const GT *gf = (const GT *)a.grid + 2 * ((size_t)f * nrow * N + lat);
+        double2 *z = zs + k * RS;
+        for (int e = tid; e < N * G; e += nt) {
+            const int j = e / G;
+            z[zp(j)] = has_row ? make_double2((double)gf[2 * (size_t)j * nrow] * sc,
+                                              (double)gf[2 * (size_t)j * nrow + 1] * sc)
+                               : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        fft_lds<-1>(zs + row * RS, a, lt, tr);
+        if (live) {
+            double2 *Ff = a.F + (size_t)f * a.nm * nlat + lat;
+            for (int e = tid; e < a.nm * G; e += nt) {
+                const int im = e / G;
+                Ff[(size_t)im * nlat] = z[zp(im * a.mres)];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Analysis: grid rows -> Fourier data, F_a = (Z_m + conj Z_{N-m}) / 2,
 // F_b = (Z_m - conj Z_{N-m}) / (2i) of the packed row Z = f_a + i f_b.
 template <int G, typename GT>
 __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfields) {
     extern __shared__ double2 zs[];
     constexpr int L = 2 * G;
-    const int N = a.nlon, nlat = a.nlat, RS = N + (N >> 4) + 1;
+    const int N = a.nlon, nlat = a.nlat, RS = a.nfft + (a.nfft >> 4) + 1;
     const int tid = threadIdx.x, nt = blockDim.x, tr = nt / G;
     const int row = tid / tr, lt = tid % tr;
     const int k = tid % L;
@@ -377,7 +494,7 @@ namespace {
 template <int G>
 hipError_t launch_fft_g(const FftArgs &a, int nfields, int inverse, int tr, hipStream_t st) {
     const int nt = G * tr;
-    const size_t lds = sizeof(double2) * (size_t)G * (a.nlon + (a.nlon >> 4) + 1);
+    const size_t lds = sizeof(double2) * (size_t)G * (a.nfft + (a.nfft >> 4) + 1);
     const void *fn = inverse ? (a.io_f32 ? (const void *)k_fft_synth_t<G, float> : (const void *)k_fft_synth_t<G, double>)
                              : (a.io_f32 ? (const void *)k_fft_anal_t<G, float> : (const void *)k_fft_anal_t<G, double>);
     if (lds > 64 * 1024) {
@@ -405,53 +522,58 @@ hipError_t launch_fft_g(const FftArgs &a, int nfields, int inverse, int tr, hipS
 }
 }  // namespace
 
-hipError_t launch_fft_cplx(const FftArgs &a, int nfields, hipStream_t st) {
-    if (a.nrad <= 0) return hipErrorNotSupported;
-    const int N = a.nlon;
-    const int tr = (std::max(64, (N + FFT_VPT - 1) / FFT_VPT) + 63) / 64 * 64;
-    const size_t row = sizeof(double2) * (size_t)(N + (N >> 4) + 1);
-    const int G = (4 * row <= 160 * 1024 && 4 * tr <= 1024) ? 4 : (2 * row <= 160 * 1024 && 2 * tr <= 1024) ? 2 : 1;
-    if (row > 160 * 1024 || tr > 1024) return hipErrorNotSupported;
+namespace {
+// Threads per LDS row: nfft / FFT_VPT rounded up to whole wavefronts (every
+// pass has at most FFT_VPT values per thread); 0 if the row does not fit.
+int fft_row_threads(const FftArgs &a, size_t *row_bytes) {
+    const int NF = a.nfft;
+    *row_bytes = sizeof(double2) * (size_t)(NF + (NF >> 4) + 1);
+    const int tr = (std::max(64, (NF + FFT_VPT - 1) / FFT_VPT) + 63) / 64 * 64;
+    if (NF < 1 || *row_bytes > 160 * 1024 || tr > 1024) return 0;
+    return tr;
+}
+
+template <int G, typename GT>
+hipError_t launch_cplx_g(const FftArgs &a, int nfields, int inverse, int tr, size_t row, hipStream_t st) {
     const size_t lds = row * G;
-    const int total = (a.nlat + G - 1) / G * nfields;
-    const int grid = std::max(1, std::min(256 * (int)std::max<size_t>(1, (160 * 1024) / lds), total));
-    const void *fns[3][2] = {{(const void *)k_fft_synth_c<1, double>, (const void *)k_fft_synth_c<1, float>},
-                             {(const void *)k_fft_synth_c<2, double>, (const void *)k_fft_synth_c<2, float>},
-                             {(const void *)k_fft_synth_c<4, double>, (const void *)k_fft_synth_c<4, float>}};
-    const int gi = G == 4 ? 2 : G == 2 ? 1 : 0;
+    const void *fn = inverse ? (const void *)k_fft_synth_c<G, GT> : (const void *)k_fft_anal_c<G, GT>;
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(fns[gi][a.io_f32 ? 1 : 0], hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    const dim3 gd(grid), bd(G * tr);
-    if (G == 4) {
-        if (a.io_f32) hipLaunchKernelGGL((k_fft_synth_c<4, float>), gd, bd, lds, st, a, nfields);
-        else hipLaunchKernelGGL((k_fft_synth_c<4, double>), gd, bd, lds, st, a, nfields);
-    } else if (G == 2) {
-        if (a.io_f32) hipLaunchKernelGGL((k_fft_synth_c<2, float>), gd, bd, lds, st, a, nfields);
-        else hipLaunchKernelGGL((k_fft_synth_c<2, double>), gd, bd, lds, st, a, nfields);
-    } else {
-        if (a.io_f32) hipLaunchKernelGGL((k_fft_synth_c<1, float>), gd, bd, lds, st, a, nfields);
-        else hipLaunchKernelGGL((k_fft_synth_c<1, double>), gd, bd, lds, st, a, nfields);
-    }
+    const int total = (a.nlat + G - 1) / G * nfields;
+    const int grid = std::max(1, std::min(256 * (int)std::max<size_t>(1, (160 * 1024) / lds), total));
+    if (inverse) hipLaunchKernelGGL((k_fft_synth_c<G, GT>), dim3(grid), dim3(G * tr), lds, st, a, nfields);
+    else hipLaunchKernelGGL((k_fft_anal_c<G, GT>), dim3(grid), dim3(G * tr), lds, st, a, nfields);
     return hipGetLastError();
 }
 
+template <typename GT>
+hipError_t launch_cplx_t(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
+    size_t row = 0;
+    const int tr = fft_row_threads(a, &row);
+    if (!tr) return hipErrorNotSupported;
+    if (4 * row <= 160 * 1024 && 4 * tr <= 1024) return launch_cplx_g<4, GT>(a, nfields, inverse, tr, row, st);
+    if (2 * row <= 160 * 1024 && 2 * tr <= 1024) return launch_cplx_g<2, GT>(a, nfields, inverse, tr, row, st);
+    return launch_cplx_g<1, GT>(a, nfields, inverse, tr, row, st);
+}
+}  // namespace
+
+hipError_t launch_fft_cplx(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
+    return a.io_f32 ? launch_cplx_t<float>(a, nfields, inverse, st) : launch_cplx_t<double>(a, nfields, inverse, st);
+}
+
 hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
-    if (a.nrad <= 0) return hipErrorNotSupported;
-    const int N = a.nlon;
-    // threads per row: N / FFT_VPT (every Stockham pass has at most FFT_VPT values per thread)
-    const int tr = (std::max(64, (N + FFT_VPT - 1) / FFT_VPT) + 63) / 64 * 64;
-    const size_t row = sizeof(double2) * (size_t)(N + (N >> 4) + 1);
+    size_t row = 0;
+    const int tr = fft_row_threads(a, &row);
+    if (!tr) return hipErrorNotSupported;
     // widest tile that fits LDS and still gives every CU work (a.tile_pairs > 0 forces it)
     const long long units4 = (long long)((a.nlat + 7) / 8) * nfields, units2 = (long long)((a.nlat + 3) / 4) * nfields;
     int G = a.tile_pairs;
     if (G <= 0) G = units4 >= 512 ? 4 : units2 >= 256 ? 2 : 1;
     if (G >= 4 && 4 * row <= 160 * 1024 && 4 * tr <= 1024) return launch_fft_g<4>(a, nfields, inverse, tr, st);
     if (G >= 2 && 2 * row <= 160 * 1024 && 2 * tr <= 1024) return launch_fft_g<2>(a, nfields, inverse, tr, st);
-    if (row <= 160 * 1024 && tr <= 1024) return launch_fft_g<1>(a, nfields, inverse, tr, st);
-    return hipErrorNotSupported;
+    return launch_fft_g<1>(a, nfields, inverse, tr, st);
 }
 
 }  // namespace shtk

@@ -110,11 +110,32 @@ struct SeedArgs {
     int *pfirst, *pfast;
 };
 
+// FFT factorisations: FFT_RADIX_BITS bits per pass, first pass in the low bits.
+// Radices 2, 3, 4, 5 and 8 have dedicated butterflies; other primes up to
+// FFT_GEN_MAX run a generic pass (R table twiddles per output).  Lengths whose
+// other prime factors would cost more take Bluestein's algorithm: a circular
+// convolution of length nfft >= 2 nlon - 1 (2^a 3^b 5^c) through two FFTs.
+constexpr int FFT_RADIX_BITS = 6;
+constexpr int FFT_MAX_PASSES = 10;
+constexpr int FFT_GEN_MAX = 61;     // largest prime of the generic pass
+constexpr int FFT_GEN_BUDGET = 48;  // largest sum of generic primes before Bluestein is used
+constexpr int FFT_MAX_LEN = 8192;   // LDS FFT length limit (8 values per thread, 1024 threads)
+
+struct FftPlan {
+    bool ok = false;    // nlon is supported
+    bool blue = false;  // Bluestein
+    int nfft = 0;       // LDS transform length: nlon, or the convolution length
+    std::vector<int> radix;   // passes of the nlon (direct) or nfft (Bluestein) transform
+};
+
 struct FftArgs {
     int nlat, nlon, np, nm, mres, mmax;
     int nrad;
-    unsigned long long radix; // 4 bits per pass, first pass in the low bits
-    const double2 *tw;        // e^{-2 pi i t / nlon}
+    unsigned long long radix; // FFT_RADIX_BITS bits per pass (of the length-nfft transform)
+    const double2 *tw;        // e^{-2 pi i t / nfft}
+    int nfft;                 // LDS row length: nlon, or the Bluestein convolution length
+    int blue;                 // Bluestein: chirp[nlon] = e^{-pi i t^2 / nlon}, bhat[2][nfft] = kernel spectra / nfft
+    const double2 *chirp, *bhat;
     double2 *F;               // Fourier workspace [field][im][nlat]
     void *grid;               // grid array (nrow, nlon, nfields)
     int nrow;                 // grid rows: nlat, or nlat - 1 when the last node (south pole) has no row
@@ -157,8 +178,10 @@ struct shtk_plan {
     double *seedq0 = nullptr, *seedq1 = nullptr;
     int *seedk = nullptr, *pfirst = nullptr, *pfast = nullptr, *segm = nullptr;
     double *convscale = nullptr;  // [(lmax+1)*(mmax+1)] configured -> canonical multiplier (non-canonical only)
-    double2 *tw = nullptr;
-    std::vector<int> radix;  // FFT factorization of nlon (empty: unsupported length)
+    double2 *tw = nullptr;     // e^{-2 pi i t / nfft}, t < nfft
+    double2 *chirp = nullptr;  // Bluestein chirp and kernel spectra (see FftArgs)
+    double2 *bhat = nullptr;
+    shtk::FftPlan fft;
 
     // benchmark instrumentation (shtk_profile_enable / shtk_profile_read)
     bool profiling = false;
@@ -189,7 +212,18 @@ struct shtk_plan {
     size_t oz_ws_bytes = 0;
     // companion plan of degree lmax + 1 on the same nodes (split-precision vector transforms)
     shtk_plan *vshadow = nullptr;
-    std::mutex mu;
+    // Serialises calls on this plan (host side); recursive because the uneven
+    // batches hold it across calls of the single-configuration entry points.
+    std::recursive_mutex mu;
+    // Device-side ordering of calls that share the workspaces: the last call's
+    // stream and an event recorded at its end (a call on another stream waits
+    // for it first).
+    hipEvent_t order_ev = nullptr;
+    hipStream_t order_st = nullptr;
+    bool order_used = false;
+    // Incremented whenever ws or oz_ws is reallocated: captured graphs that hold
+    // the old addresses are stale once it changes.
+    unsigned long long ws_gen = 0;
 };
 
 namespace shtk {
@@ -331,8 +365,10 @@ hipError_t launch_ml_anal(const MlArgs &a, int nfields, hipStream_t st);
 // (conv: the outputs are in the plan's convention map, or nullptr).
 hipError_t launch_vpole_synth(const LegArgs &a, int nfields, int robert, double wscale, hipStream_t st);
 hipError_t launch_vpole_anal(const LegArgs &a, int nfields, double wpole, const double *conv, hipStream_t st);
-// Complex-valued synthesis FFT: bins m = 0 .. mmax of F -> complex grid rows.
-hipError_t launch_fft_cplx(const FftArgs &a, int nfields, hipStream_t st);
+// FFTs of complex grid rows: synthesis (inverse = 1) turns the bins m = 0 .. mmax
+// of F into complex rows; analysis (inverse = 0) keeps the bins m = 0 .. mmax of
+// each complex row's forward transform.
+hipError_t launch_fft_cplx(const FftArgs &a, int nfields, int inverse, hipStream_t st);
 // Split-precision vector transforms: (S, T) -> coefficients of 2 nf scalar
 // fields of degree lmax + 1, and the analysis sums of those back to (S, T).
 hipError_t launch_vsplit_coef(const void *S, const void *T, double *C, int in_f32, const double *scale, int divide,

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -166,27 +167,62 @@ double coef_scale(const shtk_desc &d, int l, int m) {
     return nrm * rs * ph;
 }
 
-std::vector<int> factor_radix(int n) {
-    std::vector<int> r;
-    const int cand[] = {8, 4, 2, 3, 5};
-    for (int c : cand)
+// Pass radices of one FFT length: 8, 4, 2, 3, 5 first, then the other primes
+// up to FFT_GEN_MAX (generic passes, whose sum is returned in *gen_sum).
+bool factor_len(int n, std::vector<int> &r, int *gen_sum) {
+    r.clear();
+    *gen_sum = 0;
+    for (int c : {8, 4, 2, 3, 5})
         while (n % c == 0) {
             r.push_back(c);
             n /= c;
         }
-    if (n != 1 || r.size() > 16) r.clear();
-    return r;
+    for (int q = 7; n > 1 && q <= shtk::FFT_GEN_MAX; q += 2)
+        while (n % q == 0) {
+            r.push_back(q);
+            *gen_sum += q;
+            n /= q;
+        }
+    return n == 1 && (int)r.size() <= shtk::FFT_MAX_PASSES;
+}
+
+// The longitude FFT of length n: direct mixed-radix passes when n's prime
+// factors above 5 are few and small, otherwise Bluestein's algorithm with the
+// shortest 2^a 3^b 5^c convolution length >= 2n - 1.  FFTW (the reference's
+// FFT, src/fftutils.jl:195-238) accepts every length; so does this plan up to
+// FFT_MAX_LEN (direct) and FFT_MAX_LEN / 2 (Bluestein).
+shtk::FftPlan make_fft_plan(int n) {
+    shtk::FftPlan f;
+    if (n < 1 || n > shtk::FFT_MAX_LEN) return f;
+    int gs = 0;
+    if (factor_len(n, f.radix, &gs) && gs <= shtk::FFT_GEN_BUDGET) {
+        f.ok = true;
+        f.nfft = n;
+        return f;
+    }
+    for (int m = 2 * n - 1; m <= shtk::FFT_MAX_LEN; ++m) {
+        int t = m;
+        for (int c : {2, 3, 5})
+            while (t % c == 0) t /= c;
+        if (t != 1 || !factor_len(m, f.radix, &gs)) continue;
+        f.ok = f.blue = true;
+        f.nfft = m;
+        return f;
+    }
+    f.radix.clear();
+    return f;
 }
 
 void free_plan(shtk_plan *p) {
     if (!p) return;
     void *ptrs[] = {p->x,      p->s,      p->wscaled, p->winvs,  p->winvs2, p->invs,  p->qc,  p->ab, p->gtab,
                     p->qd,     p->coff,   p->segbase, p->seedq0, p->seedq1, p->seedk, p->pfirst,
-                    p->pfast,  p->segm,   p->convscale, p->tw,   p->ws,     p->sectv, p->sectk, p->oz_q0,
+                    p->pfast,  p->segm,   p->convscale, p->tw, p->chirp, p->bhat, p->ws,     p->sectv, p->sectk, p->oz_q0,
                     p->oz_q1,  p->oz_qk,  p->oz_ep,     p->oz_chbase, p->oz_ws, p->oz_emaxl, p->oz_epl,
                     p->oz_asc, p->oz_chm, p->oz_run, p->qb, p->qcc};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    if (p->order_ev) (void)hipEventDestroy(p->order_ev);
     free_plan(p->vshadow);
     delete p;
 }
@@ -258,6 +294,7 @@ int ensure_ws(shtk_plan *p, size_t bytes) {
     }
     HIP_TRY(hipMalloc(&p->ws, bytes), "workspace allocation");
     p->ws_bytes = bytes;
+    ++p->ws_gen;
     return SHTK_OK;
 }
 
@@ -284,6 +321,36 @@ hipError_t timed(shtk_plan *p, const char *name, hipStream_t st, F &&launch) {
     if (a && b) p->pending.push_back({name, a, b});
     return e;
 }
+
+// Calls on one plan share its workspaces.  The host mutex orders their
+// enqueueing; on the device, a call issued on another stream than the previous
+// call first waits for the event recorded at the end of that call, so kernels
+// of two streams never overlap on the workspaces.  Work being captured into a
+// graph skips both steps (the graph's owner orders its launches).
+struct StreamOrder {
+    shtk_plan *p;
+    hipStream_t st;
+    bool active = false;
+    StreamOrder(shtk_plan *p_, hipStream_t s) : p(p_), st(s) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+        active = true;
+        if (p->order_used && p->order_st != st) (void)hipStreamWaitEvent(st, p->order_ev, 0);
+    }
+    ~StreamOrder() {
+        if (!active) return;
+        if (!p->order_ev && hipEventCreateWithFlags(&p->order_ev, hipEventDisableTiming) != hipSuccess) {
+            p->order_ev = nullptr;
+            return;
+        }
+        if (hipEventRecord(p->order_ev, st) == hipSuccess) {
+            p->order_st = st;
+            p->order_used = true;
+        }
+    }
+    StreamOrder(const StreamOrder &) = delete;
+    StreamOrder &operator=(const StreamOrder &) = delete;
+};
 
 shtk::LegArgs base_args(const shtk_plan *p, int ltr) {
     shtk::LegArgs a{};
@@ -324,10 +391,15 @@ shtk::FftArgs fft_args(const shtk_plan *p) {
     f.nm = p->nm;
     f.mres = p->d.mres;
     f.mmax = p->d.mmax;
-    f.nrad = (int)p->radix.size();
+    f.nrad = (int)p->fft.radix.size();
     f.radix = 0;
-    for (size_t q = 0; q < p->radix.size(); ++q) f.radix |= (unsigned long long)p->radix[q] << (4 * q);
+    for (size_t q = 0; q < p->fft.radix.size(); ++q)
+        f.radix |= (unsigned long long)p->fft.radix[q] << (shtk::FFT_RADIX_BITS * q);
     f.tw = p->tw;
+    f.nfft = p->fft.nfft;
+    f.blue = p->fft.blue ? 1 : 0;
+    f.chirp = p->chirp;
+    f.bhat = p->bhat;
     f.s = p->s;
     f.tile_pairs = p->fft_tile;
     f.nrow = p->nrow;
@@ -340,9 +412,11 @@ int check_common(shtk_plan *p, int nfields, int dtype, std::initializer_list<con
     if (dtype != SHTK_F64 && dtype != SHTK_F32) return fail(SHTK_EINVAL, "dtype must be SHTK_F64 or SHTK_F32");
     for (const void *q : ptrs)
         if (!q) return fail(SHTK_EINVAL, "array pointer is NULL");
-    if (p->radix.empty())
-        return fail(SHTK_EUNSUPPORTED, "nlon=" + std::to_string(p->d.nlon) +
-                                           " has a prime factor above 5; the LDS FFT supports 2^a 3^b 5^c");
+    if (!p->fft.ok)
+        return fail(SHTK_EUNSUPPORTED, "nlon=" + std::to_string(p->d.nlon) + " exceeds the LDS FFT's length limit (" +
+                                           std::to_string(shtk::FFT_MAX_LEN) + " with prime factors <= " +
+                                           std::to_string(shtk::FFT_GEN_MAX) + ", " +
+                                           std::to_string(shtk::FFT_MAX_LEN / 2) + " otherwise)");
     return SHTK_OK;
 }
 
@@ -463,6 +537,7 @@ int ensure_oz_ws(shtk_plan *p, size_t bytes) {
     }
     HIP_TRY(hipMalloc(&p->oz_ws, bytes), "split workspace allocation");
     p->oz_ws_bytes = bytes;
+    ++p->ws_gen;
     return SHTK_OK;
 }
 
@@ -558,6 +633,7 @@ struct Job {
     const char *leg_name, *fft_name;
     int prec = SHTK_PREC_FP64;
     int cplx_out = 0;  // scalar synthesis to a complex grid (real_output = false)
+    int cplx_in = 0;   // scalar analysis of a complex grid
     // vector transforms on grids with poles, order-1 pole limits: 1 forward synthesis or
     // analysis (quadrature weight in the analysis), 2 adjoint direction (the weight moves to
     // the synthesis side: adjoint analysis; none on the analysis side: adjoint synthesis)
@@ -596,9 +672,10 @@ shtk::MlArgs zonal_args(const shtk_plan *p, int ltr, long long spec_fstride) {
 // spectral -> grid: [convert] -> Legendre synthesis -> inverse phi FFT
 int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void *g0, void *g1, int nf, int dtype,
                 void *stream) {
-    std::lock_guard<std::mutex> lk(p->mu);
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
     DeviceGuard dg(p->device);
     hipStream_t st = (hipStream_t)stream;
+    StreamOrder so(p, st);
     const WsLayout L = ws_layout(p, nf, j.vector);
     RC_TRY(ensure_ws(p, L.total));
     char *ws = (char *)p->ws;
@@ -680,7 +757,7 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
         fa.io_f32 = dtype == SHTK_F32;
         fa.mode = j.fft_mode;
         if (j.cplx_out)
-            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft_cplx(fa, nf, st); }),
+            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft_cplx(fa, nf, 1, st); }),
                     "phi synthesis launch");
         else
             HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 1, st); }),
@@ -692,9 +769,10 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
 // grid -> spectral: forward phi FFT -> Legendre analysis -> [convert]
 int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void *s0, void *s1, int nf, int dtype,
                 void *stream) {
-    std::lock_guard<std::mutex> lk(p->mu);
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
     DeviceGuard dg(p->device);
     hipStream_t st = (hipStream_t)stream;
+    StreamOrder so(p, st);
     const WsLayout L = ws_layout(p, nf, j.vector);
     RC_TRY(ensure_ws(p, L.total));
     char *ws = (char *)p->ws;
@@ -707,7 +785,12 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
         fa.grid = (void *)grids[q];
         fa.io_f32 = dtype == SHTK_F32;
         fa.mode = j.fft_mode;
-        HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 0, st); }), "phi analysis launch");
+        if (j.cplx_in)
+            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft_cplx(fa, nf, 0, st); }),
+                    "phi analysis launch");
+        else
+            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 0, st); }),
+                    "phi analysis launch");
     }
     shtk::LegArgs a = base_args(p, j.ltr);
     a.F0 = F0;
@@ -821,9 +904,10 @@ int ensure_vshadow(shtk_plan *p) {
 // the contiguous [V_theta fields][V_phi fields] of the workspace.
 int run_vsplit_to_grid(shtk_plan *p, const Job &j, const void *S, const void *T, void *Vt, void *Vp, int nf,
                        int dtype, void *stream) {
-    std::lock_guard<std::mutex> lk(p->mu);
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
     DeviceGuard dg(p->device);
     hipStream_t st = (hipStream_t)stream;
+    StreamOrder so(p, st);
     const WsLayout L = ws_layout(p, nf, 1);
     RC_TRY(ensure_ws(p, L.total));
     RC_TRY(ensure_vshadow(p));
@@ -871,7 +955,7 @@ int run_vsplit_to_grid(shtk_plan *p, const Job &j, const void *S, const void *T,
         fa.io_f32 = dtype == SHTK_F32;
         fa.mode = j.fft_mode;
         if (j.cplx_out)
-            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft_cplx(fa, nf, st); }),
+            HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft_cplx(fa, nf, 1, st); }),
                     "phi synthesis launch");
         else
             HIP_TRY(timed(p, j.fft_name, st, [&] { return shtk::launch_fft(fa, nf, 1, st); }),
@@ -882,9 +966,10 @@ int run_vsplit_to_grid(shtk_plan *p, const Job &j, const void *S, const void *T,
 
 int run_vsplit_to_spec(shtk_plan *p, const Job &j, const void *Vt, const void *Vp, void *S, void *T, int nf,
                        int dtype, void *stream) {
-    std::lock_guard<std::mutex> lk(p->mu);
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
     DeviceGuard dg(p->device);
     hipStream_t st = (hipStream_t)stream;
+    StreamOrder so(p, st);
     const WsLayout L = ws_layout(p, nf, 1);
     RC_TRY(ensure_ws(p, L.total));
     RC_TRY(ensure_vshadow(p));
@@ -957,6 +1042,7 @@ struct BatchGraph {
     std::vector<const void *> ptrs;
     hipStream_t stream;
     hipGraphExec_t exec;
+    std::vector<unsigned long long> gens;  // workspace generation of every plan at capture
 };
 
 }  // namespace
@@ -1084,7 +1170,7 @@ int batch_issue(shtk_batch *b, int dir, const void *const *in, void *const *out,
 int batch_prepare(shtk_batch *b, int dtype, hipStream_t st) {
     for (int gi = 0; gi < b->ngroups; ++gi) {
         shtk_plan *p = b->plans[gi];
-        std::lock_guard<std::mutex> lk(p->mu);
+        std::lock_guard<std::recursive_mutex> lk(p->mu);
         RC_TRY(ensure_ws(p, ws_layout(p, b->nf[gi], 0).total));
         if (b->precision == SHTK_PREC_SPLIT) {
             RC_TRY(ensure_oz_tables(p, st));
@@ -1094,6 +1180,11 @@ int batch_prepare(shtk_batch *b, int dtype, hipStream_t st) {
     return SHTK_OK;
 }
 
+void batch_drop_graphs(shtk_batch *b) {
+    for (BatchGraph &gr : b->graphs) (void)hipGraphExecDestroy(gr.exec);
+    b->graphs.clear();
+}
+
 int batch_run(shtk_batch *b, int dir, const void *const *in, void *const *out, int dtype, void *stream) {
     if (!b) return fail(SHTK_EINVAL, "batch is NULL");
     if (!in || !out) return fail(SHTK_EINVAL, "array list is NULL");
@@ -1101,16 +1192,30 @@ int batch_run(shtk_batch *b, int dir, const void *const *in, void *const *out, i
     for (int gi = 0; gi < b->ngroups; ++gi)
         if (!in[gi] || !out[gi]) return fail(SHTK_EINVAL, "array pointer is NULL");
     std::lock_guard<std::mutex> lk(b->mu);
+    // the group plans stay locked for the whole call (their workspaces are in use,
+    // also by a plan handed out through shtk_batch_plan)
+    std::vector<std::unique_lock<std::recursive_mutex>> plan_locks;
+    for (shtk_plan *p : b->plans) plan_locks.emplace_back(p->mu);
     DeviceGuard dg(b->device);
     hipStream_t st = (hipStream_t)stream;
     if (b->use_graph && !st) return fail(SHTK_EINVAL, "graph capture needs a non-default stream");
     RC_TRY(batch_prepare(b, dtype, st));
+    std::vector<std::unique_ptr<StreamOrder>> order;
+    for (shtk_plan *p : b->plans) order.emplace_back(new StreamOrder(p, st));
     if (!b->use_graph) return batch_issue(b, dir, in, out, dtype, st);
     std::vector<const void *> key;
     for (int gi = 0; gi < b->ngroups; ++gi) {
         key.push_back(in[gi]);
         key.push_back(out[gi]);
     }
+    std::vector<unsigned long long> gens;
+    for (shtk_plan *p : b->plans) gens.push_back(p->ws_gen);
+    // a workspace moved since a graph was captured: every cached graph holds stale addresses
+    for (const BatchGraph &gr : b->graphs)
+        if (gr.gens != gens) {
+            batch_drop_graphs(b);
+            break;
+        }
     for (const BatchGraph &gr : b->graphs)
         if (gr.dir == dir && gr.dtype == dtype && gr.stream == st && gr.ptrs == key) {
             HIP_TRY(hipGraphLaunch(gr.exec, st), "graph launch");
@@ -1130,14 +1235,9 @@ int batch_run(shtk_batch *b, int dir, const void *const *in, void *const *out, i
     const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     HIP_TRY(ei, "graph instantiation");
-    b->graphs.push_back({dir, dtype, key, st, exec});
+    b->graphs.push_back({dir, dtype, key, st, exec, gens});
     HIP_TRY(hipGraphLaunch(exec, st), "graph launch");
     return SHTK_OK;
-}
-
-void batch_drop_graphs(shtk_batch *b) {
-    for (BatchGraph &gr : b->graphs) (void)hipGraphExecDestroy(gr.exec);
-    b->graphs.clear();
 }
 
 }  // namespace
@@ -1391,12 +1491,46 @@ static int plan_tables(shtk_plan *p) {
         for (int m = 0; m <= d.mmax; ++m)
             for (int l = m; l <= lmax; ++l) conv[(size_t)l + (size_t)(lmax + 1) * m] = coef_scale(d, l, m);
     }
-    std::vector<double2> tw(d.nlon);
-    for (int t = 0; t < d.nlon; ++t) {
-        const long double ang = 2.0L * 3.141592653589793238462643383279502884L * t / d.nlon;
-        tw[t] = make_double2((double)cosl(ang), -(double)sinl(ang));
+    p->fft = make_fft_plan(d.nlon);
+    const long double PI_L = 3.141592653589793238462643383279502884L;
+    std::vector<double2> tw, chirp, bhat;
+    if (p->fft.ok) {
+        const int M = p->fft.nfft, N = d.nlon;
+        tw.resize(M);
+        for (int t = 0; t < M; ++t) {
+            const long double ang = 2.0L * PI_L * t / M;
+            tw[t] = make_double2((double)cosl(ang), -(double)sinl(ang));
+        }
+        if (p->fft.blue) {
+            // Bluestein: X_k = conj(b_k) sum_n (x_n conj(b_n)) b_{k-n}, b_n = e^{i pi n^2 / N};
+            // the convolution kernel B (b_m at m and M - m) is even, so its spectrum is
+            // b_0 + 2 sum_{m>=1} b_m cos(2 pi m k / M); the inverse DFT uses conj(B).
+            chirp.resize(N);
+            std::vector<long double> br(N), bi(N);
+            for (int t = 0; t < N; ++t) {
+                const long long q = ((long long)t * t) % (2LL * N);
+                const long double ang = PI_L * q / N;
+                br[t] = cosl(ang);
+                bi[t] = sinl(ang);
+                chirp[t] = make_double2((double)br[t], -(double)bi[t]);
+            }
+            std::vector<long double> cosM(M);
+            for (int t = 0; t < M; ++t) cosM[t] = cosl(2.0L * PI_L * t / M);
+            bhat.resize(2 * (size_t)M);
+            for (int k = 0; k < M; ++k) {
+                long double re = br[0], im = bi[0];
+                long long idx = 0;
+                for (int m = 1; m < N; ++m) {
+                    idx += k;
+                    if (idx >= M) idx %= M;
+                    re += 2.0L * br[m] * cosM[idx];
+                    im += 2.0L * bi[m] * cosM[idx];
+                }
+                bhat[k] = make_double2((double)(re / M), (double)(im / M));
+                bhat[M + k] = make_double2((double)(re / M), -(double)(im / M));
+            }
+        }
     }
-    p->radix = factor_radix(d.nlon);
 
     RC_TRY(upload(&p->x, x, "plan tables"));
     RC_TRY(upload(&p->s, s, "plan tables"));
@@ -1426,6 +1560,8 @@ static int plan_tables(shtk_plan *p) {
     RC_TRY(upload(&p->segm, segm, "plan tables"));
     RC_TRY(upload(&p->convscale, conv, "plan tables"));
     RC_TRY(upload(&p->tw, tw, "plan tables"));
+    RC_TRY(upload(&p->chirp, chirp, "plan tables"));
+    RC_TRY(upload(&p->bhat, bhat, "plan tables"));
 
     // Segment seeds, computed once on the device.
     const size_t nseed = (size_t)nseg_total * np;
@@ -1523,14 +1659,14 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
 
 int shtk_profile_enable(shtk_plan *plan, int on) {
     if (!plan) return fail(SHTK_EINVAL, "plan is NULL");
-    std::lock_guard<std::mutex> lk(plan->mu);
+    std::lock_guard<std::recursive_mutex> lk(plan->mu);
     plan->profiling = on != 0;
     return SHTK_OK;
 }
 
 int shtk_profile_read(shtk_plan *plan, int max, char *names, double *total_ms, int *count, int *n_out) {
     if (!plan) return fail(SHTK_EINVAL, "plan is NULL");
-    std::lock_guard<std::mutex> lk(plan->mu);
+    std::lock_guard<std::recursive_mutex> lk(plan->mu);
     DeviceGuard g(plan->device);
     for (auto &pe : plan->pending) {
         float ms = 0.0f;
@@ -1621,6 +1757,20 @@ int shtk_analysis_ex(shtk_plan *p, const void *f, void *alm, int nfields, int dt
     RC_TRY(norm_ltr(p, ltr, &ltr));
     Job j{0, 1, p->wscaled, 1.0, 0, 0, 0, ltr, "leg_anal", "fft_anal"};
     j.prec = precision;
+    return run_to_spec(p, j, f, nullptr, alm, nullptr, nfields, dtype, stream);
+}
+
+int shtk_analysis_cplx(shtk_plan *p, const void *f, void *alm, int nfields, int dtype, int ltr, int precision,
+                       void *stream) {
+    RC_TRY(check_common(p, nfields, dtype, {alm, f}));
+    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT)
+        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
+    if (precision == SHTK_PREC_SPLIT && p->d.lmax > 4095)
+        return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    Job j{0, 1, p->wscaled, 1.0, 0, 0, 0, ltr, "leg_anal", "fft_anal"};
+    j.prec = precision;
+    j.cplx_in = 1;
     return run_to_spec(p, j, f, nullptr, alm, nullptr, nfields, dtype, stream);
 }
 
@@ -1748,8 +1898,9 @@ static int ml_synth(shtk_plan *p, int im, const void *S, const void *T, void *V0
     shtk::MlArgs a;
     RC_TRY(ml_args(p, im, ltr, nfields, dtype, vector, &a));
     if (!S || !V0 || (vector && (!T || !V1))) return fail(SHTK_EINVAL, "NULL array argument");
-    std::lock_guard<std::mutex> lk(p->mu);
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
     DeviceGuard dg(p->device);
+    StreamOrder so(p, (hipStream_t)stream);
     a.g = p->inv_scale;
     a.in0 = S;
     a.in1 = T;
@@ -1769,9 +1920,10 @@ static int ml_anal(shtk_plan *p, int im, const void *V0, const void *V1, void *S
     if (vector && p->pole_weights && p->d.robert_form && a.m == 1 && a.ltr >= 1)
         return fail(SHTK_EINVAL, "Robert-form components with m=1 cannot be unweighted at nonzero-weight pole "
                                  "nodes; use a Gauss or pole-free regular grid");
-    std::lock_guard<std::mutex> lk(p->mu);
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
     DeviceGuard dg(p->device);
     hipStream_t st = (hipStream_t)stream;
+    StreamOrder so(p, st);
     const size_t acc_bytes = sizeof(double) * 4 * (size_t)nfields * ((p->np + 63) / 64) * a.ld_spec;
     RC_TRY(ensure_ws(p, acc_bytes));
     a.g = 1.0;

@@ -120,7 +120,7 @@ def _spec_in(cfg: SHTConfig, alm, adapt: bool, op: str, batch: bool):
     return alm, rdt
 
 
-def _grid_in(cfg: SHTConfig, f, adapt: bool, op: str, batch: bool):
+def _grid_in(cfg: SHTConfig, f, adapt: bool, op: str, batch: bool, allow_complex: bool = False):
     f = _adapt(f, adapt, op)
     want = 3 if batch else 2
     if f.dim() != want:
@@ -131,9 +131,11 @@ def _grid_in(cfg: SHTConfig, f, adapt: bool, op: str, batch: bool):
         raise ValueError(f"{op}: second dim must be nlon={cfg.nlon}")
     if batch and f.shape[2] < 1:
         raise ValueError(f"{op} requires at least one field")
-    if f.is_complex():
-        raise BackendUnavailableError(op, "complex-valued fields are not handled by this backend")
     rdt = _real_dtype(f)
+    if f.is_complex():
+        if not allow_complex:
+            raise ValueError(f"{op}: the fields must be real-valued")
+        return _colmajor(f.to(_cplx(rdt))), rdt
     f = _colmajor(f.to(rdt))
     return f, rdt
 
@@ -183,17 +185,25 @@ def _synthesis(args, kw, batch, op):
 def _analysis(args, kw, batch, op):
     adapt, args = _split_device(args)
     cfg, f = args
-    kw.pop("use_rfft", None)
+    use_rfft = kw.pop("use_rfft", False)
     ltr = kw.pop("ltr", -1)
     prec = _precision(op, kw.pop("precision", "fp64"))
     _check_kw(op, True, kw.pop("fft_scratch", None))
     if kw:
         raise TypeError(f"{op}: unexpected keywords {sorted(kw)}")
-    f, rdt = _grid_in(cfg, f, adapt, op, batch)
+    f, rdt = _grid_in(cfg, f, adapt, op, batch, allow_complex=True)
     nf = _nf(f, batch)
     shape = (cfg.lmax + 1, cfg.mmax + 1) + ((nf,) if batch else ())
     out = _empty_colmajor(shape, _cplx(rdt), f.device)
     lib = _lib.load()
+    if f.is_complex():
+        # complex fields: the m-loop over the bins m >= 0 of the complex FFT
+        # (src/core_transforms.jl:169-207); use_rfft needs a real input (:181)
+        if use_rfft:
+            raise ValueError(f"{op}: use_rfft=true requires a real-valued input")
+        _lib.check(lib.shtk_analysis_cplx(ctypes.c_void_p(cfg.plan), _ptr(f), _ptr(out), nf, _code(rdt), int(ltr),
+                                          prec, _stream(f.device)), op)
+        return out
     _lib.check(lib.shtk_analysis_ex(ctypes.c_void_p(cfg.plan), _ptr(f), _ptr(out), nf, _code(rdt), int(ltr), prec,
                                     _stream(f.device)), op)
     return out

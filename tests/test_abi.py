@@ -88,3 +88,29 @@ def test_create_config_defaults(shtk):
     assert c.nlat == 22 and c.grid_type == "driscoll_healy"
     assert shtk.create_config(10, grid_type="regular", nlat=5).nlat == 12  # raised to the minimum
     assert shtk.create_config(10, nlon=4).nlon == 21
+
+
+def test_uneven_batch_rejects_mixed_arrays(shtk):
+    """The batch kernels read every group with one dtype: mismatched precisions,
+    real coefficients or complex grids are rejected before any device call."""
+    import pytest
+    import torch
+
+    from shtnskit_jl_amd.batch import check_group_arrays
+
+    cfgs = [shtk.create_gauss_config(7, 8, nlon=16), shtk.create_gauss_config(3, 4, nlon=8)]
+    nfs = [2, 1]
+    good = [torch.zeros((8, 8, 2), dtype=torch.complex128), torch.zeros((4, 4, 1), dtype=torch.complex128)]
+    mixed = [good[0], good[1].to(torch.complex64)]
+    real = [torch.zeros((8, 8, 2), dtype=torch.float64), good[1]]
+    with pytest.raises(ValueError, match="same precision"):
+        check_group_arrays(cfgs, nfs, mixed, "spec", "synthesis")
+    with pytest.raises(ValueError, match="complex64 or complex128"):
+        check_group_arrays(cfgs, nfs, real, "spec", "synthesis")
+    grids = [torch.zeros((8, 16, 2), dtype=torch.complex128), torch.zeros((4, 8, 1), dtype=torch.float64)]
+    with pytest.raises(ValueError, match="float32 or float64"):
+        check_group_arrays(cfgs, nfs, grids, "grid", "analysis")
+    with pytest.raises(ValueError, match="expected 2 arrays"):
+        check_group_arrays(cfgs, nfs, good[:1], "spec", "synthesis")
+    with pytest.raises(ValueError, match="device tensors"):
+        check_group_arrays(cfgs, nfs, good, "spec", "synthesis")

@@ -98,7 +98,7 @@ def test_dtheta_rows_against_finite_differences(oracle):
 
 
 # --------------------------------------------------------------------------- FFT
-@pytest.mark.parametrize("n", [8, 10, 12, 101, 256, 2048, 1000])
+@pytest.mark.parametrize("n", [8, 10, 12, 14, 22, 101, 202, 256, 608, 2002, 2042, 2048, 1000])
 def test_fft_matches_numpy(oracle, n):
     rng = np.random.default_rng(n)
     v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
@@ -467,6 +467,23 @@ def test_regular_grid_round_trips(oracle, grid):
     Vt, Vp = cfg.synthesis_sphtor(S, T)
     S2, T2 = cfg.analysis_sphtor(Vt, Vp)
     assert np.abs(S2 - S).max() < 1e-12 and np.abs(T2 - T).max() < 1e-12
+
+
+def test_fixture_scalar_complex_analysis(oracle):
+    """Analysis of a complex field (the m-loop over the bins m >= 0 of the complex
+    FFT) returns the m >= 0 half of the SHTns complex fixture's coefficients (the
+    grid integrates the band-limited products exactly), and it is linear:
+    analysis_cplx(fr + i fi) = analysis(fr) + i analysis(fi)."""
+    fx = FX.fixture("scalar_complex_full")
+    p = FX.payloads(fx)
+    cfg = _cfg(oracle, fx)
+    A, _ = cplx_split(fx["lmax"], p["coefficients"])
+    a = cfg.analysis_cplx(p["field"])
+    assert np.abs(a - A).max() < 1e-12
+    rng = np.random.default_rng(3)
+    fr = rng.standard_normal((fx["nlat"], fx["nphi"]))
+    fi = rng.standard_normal((fx["nlat"], fx["nphi"]))
+    assert np.abs(cfg.analysis_cplx(fr + 1j * fi) - (cfg.analysis(fr) + 1j * cfg.analysis(fi))).max() < 1e-14
 
 
 # --------------------------------------------------------------------------- single order

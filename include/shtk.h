@@ -128,6 +128,19 @@ size_t shtk_workspace_size(const shtk_plan *plan, int nfields, int vector);
 int shtk_synthesis(shtk_plan *plan, const void *alm, void *f, int nfields, int dtype, int real_output,
                    int ltr, void *stream);
 
+/* Scalar synthesis that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as shtk_synthesis (FP64 contraction), with a
+ * fixed workspace: a device array of SHTK_STATIC_SCRATCH_BYTES per GPU, held
+ * until the process ends (set up by shtk_plan_create).  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (other streams, other
+ * plans) race; a call whose workspace, shtk_workspace_size(plan, nfields, 0),
+ * exceeds the array returns SHTK_EINVAL (one field at lmax 1023 needs 33.6 MB). */
+#define SHTK_STATIC_SCRATCH_BYTES (64u << 20)
+int shtk_synthesis_static(shtk_plan *plan, const void *alm, void *f, int nfields, int dtype, int real_output,
+                          int ltr, void *stream);
+
 /* Scalar analysis: real f (nlat, nlon, nfields) -> alm.  Replaces analysis /
  * analysis_batch (src/core_transforms.jl:169-207, src/batch_transforms.jl:312-429)
  * behind _gpu_adapter_analysis (src/device_utils.jl:87-89). */

@@ -19,7 +19,8 @@ SHTK_PREC_FP64, SHTK_PREC_SPLIT = 0, 1
 # Every symbol include/shtk.h declares.
 EXPORTS = (
     "shtk_plan_create", "shtk_plan_destroy", "shtk_plan_nodes", "shtk_workspace_size",
-    "shtk_synthesis", "shtk_analysis", "shtk_synthesis_ex", "shtk_analysis_ex", "shtk_analysis_cplx", "shtk_synthesis_sphtor", "shtk_analysis_sphtor",
+    "shtk_synthesis", "shtk_analysis", "shtk_synthesis_ex", "shtk_analysis_ex", "shtk_analysis_cplx",
+    "shtk_synthesis_static", "shtk_synthesis_sphtor", "shtk_analysis_sphtor",
     "shtk_synthesis_sphtor_ex", "shtk_analysis_sphtor_ex",
     "shtk_synthesis_ml", "shtk_analysis_ml", "shtk_synthesis_sphtor_ml", "shtk_analysis_sphtor_ml",
     "shtk_adjoint_synthesis", "shtk_adjoint_analysis", "shtk_adjoint_synthesis_sphtor",
@@ -71,6 +72,7 @@ def load():
     lib.shtk_synthesis_ex.argtypes = [p, p, p, i, i, i, i, i, p]
     lib.shtk_analysis_ex.argtypes = [p, p, p, i, i, i, i, p]
     lib.shtk_analysis_cplx.argtypes = [p, p, p, i, i, i, i, p]
+    lib.shtk_synthesis_static.argtypes = [p, p, p, i, i, i, i, p]
     lib.shtk_synthesis_sphtor.argtypes = [p, p, p, p, p, i, i, i, i, p]
     lib.shtk_analysis_sphtor.argtypes = [p, p, p, p, p, i, i, i, p]
     lib.shtk_synthesis_sphtor_ex.argtypes = [p, p, p, p, p, i, i, i, i, i, p]

@@ -25,6 +25,15 @@ extern "C" {
 static int plan_tables(shtk_plan *p);  // defined with the entry points
 }
 
+// Workspace of shtk_synthesis_static: a file-scope device array, one copy per
+// GPU at its own address, held until the process ends.  In host code the
+// array's name is only a stand-in for the symbol; its device address on each
+// GPU comes from hipGetSymbolAddress (static_scratch_setup, run once per device
+// by shtk_plan_create with that device current) and only that address is ever
+// passed to kernels.  Every call on a GPU shares that GPU's array.
+constexpr size_t kStaticScratchBytes = SHTK_STATIC_SCRATCH_BYTES;
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+
 namespace {
 
 thread_local std::string g_err;
@@ -52,6 +61,24 @@ int hip_fail(hipError_t e, const char *what) {
         int rc_ = (expr);        \
         if (rc_) return rc_;     \
     } while (0)
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // device address of `scratch` per device (nullptr: not set up)
+
+int static_scratch_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return SHTK_OK;
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "static scratch address");
+    g_scratch_ptr[device] = ptr;
+    return SHTK_OK;
+}
+
+void *static_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
 
 template <typename T>
 int upload(T **dst, const std::vector<T> &src, const char *what) {
@@ -338,11 +365,7 @@ struct StreamOrder {
         if (p->order_used && p->order_st != st) (void)hipStreamWaitEvent(st, p->order_ev, 0);
     }
     ~StreamOrder() {
-        if (!active) return;
-        if (!p->order_ev && hipEventCreateWithFlags(&p->order_ev, hipEventDisableTiming) != hipSuccess) {
-            p->order_ev = nullptr;
-            return;
-        }
+        if (!active || !p->order_ev) return;  // the event is made with the plan
         if (hipEventRecord(p->order_ev, st) == hipSuccess) {
             p->order_st = st;
             p->order_used = true;
@@ -670,15 +693,23 @@ shtk::MlArgs zonal_args(const shtk_plan *p, int ltr, long long spec_fstride) {
 }
 
 // spectral -> grid: [convert] -> Legendre synthesis -> inverse phi FFT
+// ext_ws: a caller-provided workspace of ext_bytes (nothing is allocated), or
+// nullptr for the plan's own (grown on demand).
 int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void *g0, void *g1, int nf, int dtype,
-                void *stream) {
+                void *stream, void *ext_ws = nullptr, size_t ext_bytes = 0) {
     std::lock_guard<std::recursive_mutex> lk(p->mu);
     DeviceGuard dg(p->device);
     hipStream_t st = (hipStream_t)stream;
     StreamOrder so(p, st);
     const WsLayout L = ws_layout(p, nf, j.vector);
-    RC_TRY(ensure_ws(p, L.total));
-    char *ws = (char *)p->ws;
+    if (ext_ws) {
+        if (L.total > ext_bytes)
+            return fail(SHTK_EINVAL, "the call needs " + std::to_string(L.total) + " bytes of workspace; the static "
+                                     "scratch holds " + std::to_string(ext_bytes));
+    } else {
+        RC_TRY(ensure_ws(p, L.total));
+    }
+    char *ws = ext_ws ? (char *)ext_ws : (char *)p->ws;
     double2 *F0 = (double2 *)ws, *F1 = (double2 *)(ws + L.f_bytes);
     char *spec = ws + (j.vector ? 2 : 1) * L.f_bytes;
     shtk::LegArgs a = base_args(p, j.ltr);
@@ -1618,6 +1649,18 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
         delete p;
         return fail(SHTK_EHIP, "hipGetDevice failed (no ROCm device)");
     }
+    if (hipEventCreateWithFlags(&p->order_ev, hipEventDisableTiming) != hipSuccess) {
+        delete p;
+        return fail(SHTK_EHIP, "hipEventCreateWithFlags failed");
+    }
+    {
+        const int rc_s = static_scratch_setup(p->device);
+        if (rc_s) {
+            (void)hipEventDestroy(p->order_ev);
+            delete p;
+            return rc_s;
+        }
+    }
     p->np = (d.nlat + 1) / 2;
     p->nm = d.mmax / d.mres + 1;
     p->seg = segment_length();
@@ -1728,6 +1771,19 @@ size_t shtk_workspace_size(const shtk_plan *plan, int nfields, int vector) {
 int shtk_synthesis(shtk_plan *p, const void *alm, void *f, int nfields, int dtype, int real_output, int ltr,
                    void *stream) {
     return shtk_synthesis_ex(p, alm, f, nfields, dtype, real_output, ltr, SHTK_PREC_FP64, stream);
+}
+
+// shtk_synthesis with the static scratch as its workspace: the same kernels,
+// no device allocation (the plan's tables exist since shtk_plan_create).
+int shtk_synthesis_static(shtk_plan *p, const void *alm, void *f, int nfields, int dtype, int real_output, int ltr,
+                          void *stream) {
+    RC_TRY(check_common(p, nfields, dtype, {alm, f}));
+    RC_TRY(norm_ltr(p, ltr, &ltr));
+    void *ws = static_scratch(p->device);
+    if (!ws) return fail(SHTK_EHIP, "static scratch is not set up on this device");
+    Job j{0, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, 0, ltr, "leg_synth", "fft_synth"};
+    j.cplx_out = real_output != 1;
+    return run_to_grid(p, j, alm, nullptr, f, nullptr, nfields, dtype, stream, ws, kStaticScratchBytes);
 }
 
 int shtk_synthesis_ex(shtk_plan *p, const void *alm, void *f, int nfields, int dtype, int real_output, int ltr,

@@ -212,12 +212,23 @@ int shtk_adjoint_synthesis(shtk_plan *plan, const void *fbar, void *almbar, int 
 int shtk_adjoint_analysis(shtk_plan *plan, const void *almbar, void *fbar, int nfields, int dtype,
                           void *stream);
 
+/* The whole (complex) adjoint of analysis, _adjoint_analysis (src/core_transforms.jl:611-652):
+ * almbar -> complex fbar (nlat, nlon, nfields), interleaved re/im of dtype: the inverse
+ * FFT of the bins m >= 0 (the negative bins are zero).  What the analysis rrule pulls
+ * back for a complex primal (ext/SHTnsKitAdvancedADExt.jl:55-67). */
+int shtk_adjoint_analysis_cplx(shtk_plan *plan, const void *almbar, void *fbar, int nfields, int dtype,
+                               void *stream);
+
 /* Vector adjoints: _adjoint_synthesis_sphtor (src/core_transforms.jl:537-597)
  * and the real part of _adjoint_analysis_sphtor (src/sphtor_transforms.jl:385-452). */
 int shtk_adjoint_synthesis_sphtor(shtk_plan *plan, const void *Vtb, const void *Vpb, void *Sb, void *Tb,
                                   int nfields, int dtype, int real_output, void *stream);
 int shtk_adjoint_analysis_sphtor(shtk_plan *plan, const void *Sb, const void *Tb, void *Vtb, void *Vpb,
                                  int nfields, int dtype, void *stream);
+/* The whole (complex) _adjoint_analysis_sphtor (src/sphtor_transforms.jl:385-452):
+ * complex Vtb, Vpb (interleaved re/im of dtype). */
+int shtk_adjoint_analysis_sphtor_cplx(shtk_plan *plan, const void *Sb, const void *Tb, void *Vtb, void *Vpb,
+                                      int nfields, int dtype, void *stream);
 
 /* Per-kernel timing for benchmarks; no counterpart in the reference.  While
  * enabled, every kernel this plan launches is bracketed by HIP events on the

@@ -15,7 +15,8 @@ from .transforms import (CPU, GPU, on_device, synthesis, analysis, synthesis_bat
                          synthesis_l, analysis_l, synthesis_sphtor, analysis_sphtor,
                          synthesis_sphtor_batch, analysis_sphtor_batch, synthesis_sphtor_l, analysis_sphtor_l,
                          synthesis_sph_l, synthesis_tor_l, synthesis_qst, analysis_qst, synthesis_qst_batch,
-                         analysis_qst_batch, synthesis_qst_l, analysis_qst_l, synthesis_packed_ml,
+                         analysis_qst_batch, synthesis_qst_l, analysis_qst_l, synthesis_packed, analysis_packed,
+                         synthesis_packed_ml,
                          analysis_packed_ml, synthesis_sphtor_ml, analysis_sphtor_ml, synthesis_sph_ml,
                          synthesis_tor_ml, synthesis_grad_ml, synthesis_qst_ml, analysis_qst_ml, _adjoint_synthesis,
                          _adjoint_analysis, _adjoint_synthesis_sphtor, _adjoint_analysis_sphtor)

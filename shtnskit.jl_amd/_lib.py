@@ -24,7 +24,8 @@ EXPORTS = (
     "shtk_synthesis_sphtor_ex", "shtk_analysis_sphtor_ex",
     "shtk_synthesis_ml", "shtk_analysis_ml", "shtk_synthesis_sphtor_ml", "shtk_analysis_sphtor_ml",
     "shtk_adjoint_synthesis", "shtk_adjoint_analysis", "shtk_adjoint_synthesis_sphtor",
-    "shtk_adjoint_analysis_sphtor", "shtk_batch_create", "shtk_batch_destroy", "shtk_batch_plan",
+    "shtk_adjoint_analysis_sphtor", "shtk_adjoint_analysis_cplx", "shtk_adjoint_analysis_sphtor_cplx",
+    "shtk_batch_create", "shtk_batch_destroy", "shtk_batch_plan",
     "shtk_batch_set_graph", "shtk_batch_set_tiles", "shtk_batch_synthesis", "shtk_batch_analysis",
     "shtk_batch_autotune", "shtk_profile_enable", "shtk_profile_read", "shtk_stream_sync",
     "shtk_last_error", "shtk_version",
@@ -85,6 +86,8 @@ def load():
     lib.shtk_adjoint_analysis.argtypes = [p, p, p, i, i, p]
     lib.shtk_adjoint_synthesis_sphtor.argtypes = [p, p, p, p, p, i, i, i, p]
     lib.shtk_adjoint_analysis_sphtor.argtypes = [p, p, p, p, p, i, i, p]
+    lib.shtk_adjoint_analysis_cplx.argtypes = [p, p, p, i, i, p]
+    lib.shtk_adjoint_analysis_sphtor_cplx.argtypes = [p, p, p, p, p, i, i, p]
     lib.shtk_batch_create.argtypes = [ctypes.POINTER(ShtkDesc), p, i, i, ctypes.POINTER(p)]
     lib.shtk_batch_destroy.argtypes = [p]
     lib.shtk_batch_plan.argtypes = [p, i, ctypes.POINTER(p)]

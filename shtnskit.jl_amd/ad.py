@@ -34,7 +34,7 @@ class _Analysis(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, abar):
-        return None, T._adjoint_analysis(ctx.cfg, abar)
+        return None, T._adjoint_analysis(ctx.cfg, abar, project_real=True)
 
 
 class _SynthesisSphtor(torch.autograd.Function):
@@ -67,7 +67,7 @@ class _AnalysisSphtor(torch.autograd.Function):
             sb = torch.zeros_like(tb)
         if tb is None:
             tb = torch.zeros_like(sb)
-        vt, vp = T._adjoint_analysis_sphtor(ctx.cfg, sb, tb)
+        vt, vp = T._adjoint_analysis_sphtor(ctx.cfg, sb, tb, project_real=True)
         return None, vt, vp
 
 

@@ -1853,6 +1853,15 @@ int shtk_adjoint_analysis(shtk_plan *p, const void *almbar, void *fbar, int nfie
     return run_to_grid(p, j, almbar, nullptr, fbar, nullptr, nfields, dtype, stream);
 }
 
+int shtk_adjoint_analysis_cplx(shtk_plan *p, const void *almbar, void *fbar, int nfields, int dtype, void *stream) {
+    RC_TRY(check_common(p, nfields, dtype, {fbar, almbar}));
+    // the whole of src/core_transforms.jl:611-652: F = 2pi w_i sum_l P abar over the
+    // bins m >= 0 (the negative ones stay zero), ifft to a complex grid
+    Job j{0, 1, p->wscaled, 1.0, 0, 0, 0, p->d.lmax, "leg_synth", "fft_synth"};
+    j.cplx_out = 1;
+    return run_to_grid(p, j, almbar, nullptr, fbar, nullptr, nfields, dtype, stream);
+}
+
 // --- vector ------------------------------------------------------------------
 int shtk_synthesis_sphtor(shtk_plan *p, const void *S, const void *T, void *Vt, void *Vp, int nfields, int dtype,
                           int real_output, int ltr, void *stream) {
@@ -2028,6 +2037,18 @@ int shtk_adjoint_analysis_sphtor(shtk_plan *p, const void *Sb, const void *Tb, v
                                  "nodes; use a Gauss or pole-free regular grid");
     Job j{1, 1, p->winvs, 1.0, 2, 1, p->d.robert_form ? 1 : 0, p->d.lmax, "leg_synth_sphtor", "fft_synth"};
     j.pole_fix = 2;
+    return run_to_grid(p, j, Sb, Tb, Vtb, Vpb, nfields, dtype, stream);
+}
+
+int shtk_adjoint_analysis_sphtor_cplx(shtk_plan *p, const void *Sb, const void *Tb, void *Vtb, void *Vpb,
+                                      int nfields, int dtype, void *stream) {
+    RC_TRY(check_common(p, nfields, dtype, {Vtb, Vpb, Sb, Tb}));
+    if (p->pole_weights && p->d.robert_form && p->d.mres == 1 && p->d.mmax >= 1)
+        return fail(SHTK_EINVAL, "Robert-form components with m=1 cannot be unweighted at nonzero-weight pole "
+                                 "nodes; use a Gauss or pole-free regular grid");
+    Job j{1, 1, p->winvs, 1.0, 0, 1, p->d.robert_form ? 1 : 0, p->d.lmax, "leg_synth_sphtor", "fft_synth"};
+    j.pole_fix = 2;
+    j.cplx_out = 1;
     return run_to_grid(p, j, Sb, Tb, Vtb, Vpb, nfields, dtype, stream);
 }
 

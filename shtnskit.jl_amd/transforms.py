@@ -387,6 +387,41 @@ def _ml_call(fn, cfg, im, ins, n_in, n_out, ltr, op):
     return outs
 
 
+def analysis_packed(*args):
+    """``analysis_packed(cfg, Vr_flat)`` (src/transforms.jl:125-136): a flattened real
+    grid (length nlat*nlon, latitude fastest as Julia's ``vec``) -> packed Qlm in SHTns
+    LM order (length ``cfg.nlm``; ``pack_lm`` skips the orders that are not multiples
+    of mres)."""
+    from .layout import pack_lm
+
+    adapt, args = _split_device(args)
+    cfg, Vr = args
+    Vr = _adapt(Vr, adapt, "analysis_packed")
+    if Vr.dim() != 1 or Vr.shape[0] != cfg.nspat:
+        raise ValueError(f"analysis_packed: Vr must have length {cfg.nspat}")
+    if Vr.is_complex():
+        raise TypeError("analysis_packed: Vr must be real-valued")
+    f = Vr.reshape(cfg.nlon, cfg.nlat).T  # reshape(Vr, nlat, nlon) of column-major storage
+    return pack_lm(cfg, analysis(cfg, f))
+
+
+def synthesis_packed(*args):
+    """``synthesis_packed(cfg, Qlm)`` (src/transforms.jl:145-157): packed Qlm (SHTns LM
+    order, length ``cfg.nlm``) -> the flattened real grid (length nlat*nlon,
+    latitude fastest)."""
+    from .layout import unpack_lm
+
+    adapt, args = _split_device(args)
+    cfg, Qlm = args
+    Qlm = _adapt(Qlm, adapt, "synthesis_packed")
+    if Qlm.dim() != 1 or Qlm.shape[0] != cfg.nlm:
+        raise ValueError(f"synthesis_packed: Qlm must have length {cfg.nlm}")
+    if not Qlm.is_complex():
+        raise TypeError("synthesis_packed: Qlm must be complex-valued")
+    f = synthesis(cfg, unpack_lm(cfg, Qlm), real_output=True)
+    return f.T.reshape(-1)  # vec(f): latitude fastest
+
+
 def synthesis_packed_ml(cfg: SHTConfig, im: int, Ql, ltr: int):
     """``synthesis_packed_ml(cfg, mval, Ql, ltr)`` (src/transforms.jl:394-437): one stored
     order's coefficients (degrees m..ltr) -> its complex Fourier column (nlat,)."""
@@ -505,7 +540,13 @@ def _batchdim(t: torch.Tensor) -> bool:
 
 
 def _adjoint_synthesis(cfg: SHTConfig, fbar, real_output: bool = True):
-    """Adjoint of ``synthesis`` (Zygote/torch convention ``dL/dRe + i dL/dIm``)."""
+    """Adjoint of ``synthesis`` (src/core_transforms.jl:477-519; Zygote/torch convention
+    ``dL/dRe + i dL/dIm``).  A complex cotangent is taken as its real and imaginary
+    parts: the adjoint is complex-linear in ``fbar``."""
+    if isinstance(fbar, torch.Tensor) and fbar.is_complex():
+        re = _adjoint_synthesis(cfg, fbar.real.contiguous(), real_output)
+        im = _adjoint_synthesis(cfg, fbar.imag.contiguous(), real_output)
+        return re + 1j * im
     batch = _batchdim(fbar)
     fbar, rdt = _grid_in(cfg, fbar, False, "_adjoint_synthesis", batch)
     nf = _nf(fbar, batch)
@@ -517,22 +558,36 @@ def _adjoint_synthesis(cfg: SHTConfig, fbar, real_output: bool = True):
     return out
 
 
-def _adjoint_analysis(cfg: SHTConfig, almbar):
-    """Real part of the adjoint of ``analysis`` (what ``ProjectTo`` keeps for real fields)."""
+def _adjoint_analysis(cfg: SHTConfig, almbar, project_real: bool = False):
+    """Adjoint of ``analysis`` (src/core_transforms.jl:611-652): the complex grid of the
+    inverse FFT over the bins m >= 0.  ``project_real=True`` returns its real part, what
+    ``ProjectTo`` keeps for a real primal (ext/SHTnsKitAdvancedADExt.jl:55-67)."""
     batch = _batchdim(almbar)
     almbar, rdt = _spec_in(cfg, almbar, False, "_adjoint_analysis", batch)
     nf = _nf(almbar, batch)
     shape = (cfg.nlat, cfg.nlon) + ((nf,) if batch else ())
-    out = _empty_colmajor(shape, rdt, almbar.device)
-    _lib.check(_lib.load().shtk_adjoint_analysis(ctypes.c_void_p(cfg.plan), _ptr(almbar), _ptr(out), nf,
-                                                 _code(rdt), _stream(almbar.device)), "_adjoint_analysis")
+    out = _empty_colmajor(shape, rdt if project_real else _cplx(rdt), almbar.device)
+    fn = _lib.load().shtk_adjoint_analysis if project_real else _lib.load().shtk_adjoint_analysis_cplx
+    _lib.check(fn(ctypes.c_void_p(cfg.plan), _ptr(almbar), _ptr(out), nf, _code(rdt), _stream(almbar.device)),
+               "_adjoint_analysis")
     return out
 
 
 def _adjoint_synthesis_sphtor(cfg: SHTConfig, Vtb, Vpb, real_output: bool = True):
+    """Adjoint of ``synthesis_sphtor`` (src/core_transforms.jl:537-597); complex-linear
+    in the cotangents, so complex ones are taken as real and imaginary parts."""
+    if any(isinstance(v, torch.Tensor) and v.is_complex() for v in (Vtb, Vpb)):
+        Vtb, Vpb = torch.as_tensor(Vtb), torch.as_tensor(Vpb)
+        sr, tr = _adjoint_synthesis_sphtor(cfg, Vtb.real.contiguous(), Vpb.real.contiguous(), real_output)
+        zt = Vtb.imag if Vtb.is_complex() else torch.zeros_like(Vtb)
+        zp = Vpb.imag if Vpb.is_complex() else torch.zeros_like(Vpb)
+        si, ti = _adjoint_synthesis_sphtor(cfg, zt.contiguous(), zp.contiguous(), real_output)
+        return sr + 1j * si, tr + 1j * ti
     batch = _batchdim(Vtb)
     Vtb, rdt = _grid_in(cfg, Vtb, False, "_adjoint_synthesis_sphtor", batch)
     Vpb, _ = _grid_in(cfg, Vpb.to(Vtb.dtype), False, "_adjoint_synthesis_sphtor", batch)
+    if Vtb.shape != Vpb.shape:
+        raise ValueError("_adjoint_synthesis_sphtor: Vt and Vp cotangents must have equal shapes")
     nf = _nf(Vtb, batch)
     shape = (cfg.lmax + 1, cfg.mmax + 1) + ((nf,) if batch else ())
     S = _empty_colmajor(shape, _cplx(rdt), Vtb.device)
@@ -543,17 +598,23 @@ def _adjoint_synthesis_sphtor(cfg: SHTConfig, Vtb, Vpb, real_output: bool = True
     return S, T
 
 
-def _adjoint_analysis_sphtor(cfg: SHTConfig, Sb, Tb):
+def _adjoint_analysis_sphtor(cfg: SHTConfig, Sb, Tb, project_real: bool = False):
+    """Adjoint of ``analysis_sphtor`` (src/sphtor_transforms.jl:385-452): complex grids;
+    ``project_real=True`` returns their real parts (a real primal's projection)."""
     batch = _batchdim(Sb)
     Sb, rdt = _spec_in(cfg, Sb, False, "_adjoint_analysis_sphtor", batch)
     Tb, _ = _spec_in(cfg, Tb.to(Sb.dtype), False, "_adjoint_analysis_sphtor", batch)
+    if Sb.shape != Tb.shape:
+        raise ValueError("_adjoint_analysis_sphtor: S and T cotangents must have equal shapes")
     nf = _nf(Sb, batch)
     shape = (cfg.nlat, cfg.nlon) + ((nf,) if batch else ())
-    Vt = _empty_colmajor(shape, rdt, Sb.device)
-    Vp = _empty_colmajor(shape, rdt, Sb.device)
-    _lib.check(_lib.load().shtk_adjoint_analysis_sphtor(ctypes.c_void_p(cfg.plan), _ptr(Sb), _ptr(Tb), _ptr(Vt),
-                                                        _ptr(Vp), nf, _code(rdt), _stream(Sb.device)),
-               "_adjoint_analysis_sphtor")
+    odt = rdt if project_real else _cplx(rdt)
+    Vt = _empty_colmajor(shape, odt, Sb.device)
+    Vp = _empty_colmajor(shape, odt, Sb.device)
+    lib = _lib.load()
+    fn = lib.shtk_adjoint_analysis_sphtor if project_real else lib.shtk_adjoint_analysis_sphtor_cplx
+    _lib.check(fn(ctypes.c_void_p(cfg.plan), _ptr(Sb), _ptr(Tb), _ptr(Vt), _ptr(Vp), nf, _code(rdt),
+                  _stream(Sb.device)), "_adjoint_analysis_sphtor")
     return Vt, Vp
 
 

@@ -111,9 +111,15 @@ def test_adjoints_match_oracle(shtk, oracle, cuda):
     got = shtk._adjoint_synthesis(cfg, to_dev(ybar))
     assert rel_l2(to_host(got), ref) < TOL
     abar = oracle.random_alm(lmax, lmax, seed=10)
-    ref = ocfg.adjoint_analysis(abar).real
+    ref = ocfg.adjoint_analysis(abar)
     got = shtk._adjoint_analysis(cfg, to_dev(abar))
-    assert rel_l2(to_host(got), ref) < TOL
+    assert got.is_complex() and rel_l2(to_host(got), ref) < TOL
+    got_re = shtk._adjoint_analysis(cfg, to_dev(abar), project_real=True)
+    assert not got_re.is_complex() and rel_l2(to_host(got_re), ref.real) < TOL
+    # a complex cotangent of the synthesis: the adjoint is complex-linear
+    ycplx = ybar + 1j * np.asfortranarray(rng.standard_normal((nlat, nlon)))
+    ref = ocfg.adjoint_synthesis(ycplx.real) + 1j * ocfg.adjoint_synthesis(ycplx.imag)
+    assert rel_l2(to_host(shtk._adjoint_synthesis(cfg, to_dev(ycplx))), ref) < TOL
 
 
 def test_errors(shtk, cuda):

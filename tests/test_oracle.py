@@ -346,6 +346,12 @@ def test_adjoint_dot_product_identities(oracle):
     vt, vp = cfg.adjoint_analysis_sphtor(S, T)
     lhs = _inner_real(Sa, S) + _inner_real(Ta, T)
     assert abs(lhs - np.sum(yt * vt.real) - np.sum(yp * vp.real)) < 1e-11 * abs(lhs)
+    # the whole complex analysis adjoint against the analysis of complex fields:
+    # Re<analysis_cplx(g), b> = Re<g, adj(b)> for complex g
+    gc = np.asfortranarray(rng.standard_normal((22, 48)) + 1j * rng.standard_normal((22, 48)))
+    lhs = _inner_real(cfg.analysis_cplx(gc), b)
+    rhs = _inner_real(gc, cfg.adjoint_analysis(b))
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
 
 
 def test_adjoint_synthesis_finite_difference(oracle):

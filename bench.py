@@ -51,6 +51,7 @@ sys.path.insert(0, ROOT)
 METRIC = "round-trip SHT/s at lmax=1023: plain FP64 vs batched split-FP32 at ≤1e-11 rel err"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix peaks coincide)
 PEAK_INT8_TOPS = 5000.0  # MI355X dense int8 matrix
+PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0
 SPLIT_PRODUCTS = 21  # slice products of the 6-slice Ozaki split (digit orders k + t <= 5)
 LMAX, NLAT, NLON = 1023, 1024, 2048
@@ -75,6 +76,12 @@ def split_int8_ops(lmax: int, nlat: int, nfields: int) -> float:
     return SPLIT_PRODUCTS * 2.0 * n_triples(lmax, nlat) * 2 * nfields
 
 
+def fp32_mfma_flops(lmax: int, nlat: int, nfields: int) -> float:
+    """One direction of the FP32 matrix-core contraction: every folded triple
+    against 2 real columns per field, 2 flops per multiply-add."""
+    return 4.0 * n_triples(lmax, nlat) * nfields
+
+
 def fft_bytes(nlat: int, nlon: int, mmax: int, nfields: int = 1) -> float:
     """Compulsory HBM bytes of one phi-FFT launch: the real grid and the
     half-spectrum intermediate, read once and written once."""
@@ -94,30 +101,53 @@ def measured_traffic(kernel: str):
     return None if ent is None else ent.get("bytes_per_launch")
 
 
-def cpu_baseline(lmax, nlat, nlon, budget_s=15.0):
+def _cpu_round_trips(oracle, lmax, nlat, nlon, power, seed, threads, warmup, runs):
+    """Median wall time of `runs` round trips after `warmup` untimed ones (plan and
+    tables built outside the timing), on `threads` OpenMP threads."""
+    oracle.set_threads(threads)
+    cfg = oracle.Config(lmax, nlat, nlon=nlon)
+    alm = oracle.random_alm(lmax, lmax, seed=seed, power=power)
+    for _ in range(warmup):
+        cfg.analysis(cfg.synthesis(alm))
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        f = cfg.synthesis(alm)
+        cfg.analysis(f)
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times)
+
+
+def cpu_baseline(lmax, nlat, nlon, single_thread_runs=2):
+    """The oracle's restatement of the reference CPU path (threads over m in the
+    balanced order, per-step recurrence coefficients, no folding), on the host
+    cores this process may use and on one thread.  All-core figures: median of
+    10 round trips after 2 warmups; the one-thread figure at lmax 1023 is a
+    bounded sample (`single_thread_runs` after 1 warmup, about 10 s each); the
+    lmax 127 workload (128 x 256) is timed the same way on both."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # checker / baseline only
 
     # the host share the process may use: OMP_NUM_THREADS if set, else its CPU affinity
     threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
-    oracle.set_threads(threads)
-    cfg = oracle.Config(lmax, nlat, nlon=nlon)
-    alm = oracle.random_alm(lmax, lmax, seed=2, power=1.5)
-    times = []
-    t0 = time.perf_counter()
-    f = cfg.synthesis(alm)
-    cfg.analysis(f)
-    first = time.perf_counter() - t0
-    n = max(2, min(9, int(budget_s / max(first, 1e-3))))
-    for _ in range(n):
-        t0 = time.perf_counter()
-        f = cfg.synthesis(alm)
-        cfg.analysis(f)
-        times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    return {"value": 1.0 / med, "unit": "SHT/s", "cores": threads, "kind": "port",
-            "sample": f"{n} round trips at lmax=1023 (1024x2048, 1 field) after 1 warmup, median; "
-                      f"OpenMP threads over m in balanced order, FP64, per-step recurrence coefficients"}
+    t_all = _cpu_round_trips(oracle, lmax, nlat, nlon, 1.5, 2, threads, 2, 10)
+    t_one = _cpu_round_trips(oracle, lmax, nlat, nlon, 1.5, 2, 1, 1, single_thread_runs)
+    c1_all = _cpu_round_trips(oracle, 127, 128, 256, 1.0, 1, threads, 2, 10)
+    c1_one = _cpu_round_trips(oracle, 127, 128, 256, 1.0, 1, 1, 2, 10)
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        model = "unknown"
+    return {"value": 1.0 / t_all, "unit": "SHT/s", "cores": threads, "kind": "port",
+            "sample": f"median of 10 round trips after 2 warmups at lmax=1023 (1024x2048, 1 field, seed 2, "
+                      f"(l+1)^-1.5); OpenMP threads over m in balanced order, FP64, per-step recurrence "
+                      f"coefficients, project-written FFT (FFTW absent); host CPU: {model}",
+            "single_thread": {"value": 1.0 / t_one, "unit": "SHT/s", "cores": 1,
+                              "sample": f"median of {single_thread_runs} round trips after 1 warmup, lmax=1023"},
+            "lmax127": {"value": 1.0 / c1_all, "unit": "SHT/s", "cores": threads,
+                        "single_thread_value": 1.0 / c1_one,
+                        "sample": "median of 10 round trips after 2 warmups, lmax=127 (128x256, seed 1, "
+                                  "(l+1)^-1), all cores and one thread"}}
 
 
 def spectrum(np, nfields, seed=2):
@@ -181,8 +211,12 @@ def plain_roofline(kernels_us):
     us = kernels_us[dom]
     if dom.startswith("leg"):
         fl = algorithmic_flops(LMAX, NLAT)
-        return roofline_entry(dom, us, fl, PEAK_FP64_TFLOPS, "TFLOP/s", "mfma",
-                              f"{fl:.4e} flop = 8 x (nlat/2) x (lmax+1)(lmax+2)/2 (FP64 VALU)")
+        ent = roofline_entry(dom, us, fl, PEAK_FP64_TFLOPS, "TFLOP/s", "mfma",
+                             f"{fl:.4e} flop = 8 x (nlat/2) x (lmax+1)(lmax+2)/2 (FP64 VALU)")
+        # compute-bound; the instructions are FP64 vector FMAs (v_fmac_f64), whose
+        # peak equals the FP64 matrix peak on MI355X
+        ent["engine"] = "FP64 VALU (peak = FP64 vector = FP64 matrix, 78.6 TFLOP/s)"
+        return ent
     by = fft_bytes(NLAT, NLON, LMAX)
     return roofline_entry(dom, us, by * 1e3, PEAK_HBM_GBS, "GB/s", "hbm", f"{by:.4e} bytes")
 
@@ -244,12 +278,34 @@ def measure_uneven(torch, np, shtk, dev, steps):
     e1.record()
     torch.cuda.synchronize()
     fp64_ms = e0.elapsed_time(e1) / steps
+    # FP32 matrix-core contraction, captured the same way
+    cfgs4 = [shtk.create_gauss_config(L, L + 1, nlon=2 * L + 2) for L, _ in groups]
+    g32 = shtk.UnevenBatch(cfgs4, nfs, precision="fp32", graph=True)
+    with torch.cuda.stream(s):
+        for _ in range(2):
+            back32 = g32.analysis(g32.synthesis(alms))
+        torch.cuda.synchronize()
+        err32 = 0.0
+        for fa, ra in zip(back32, ref_a):
+            for k in range(fa.shape[2]):
+                err32 = max(err32, float(torch.linalg.vector_norm(fa[..., k] - ra[..., k]) /
+                                         torch.linalg.vector_norm(ra[..., k])))
+        e0.record(s)
+        for _ in range(steps):
+            g32.analysis(g32.synthesis(alms))
+        e1.record(s)
+        torch.cuda.synchronize()
+    fp32_ms = e0.elapsed_time(e1) / steps
+    g32.close()
     g.close()
     fp64.close()
     return {"value": round(1000.0 * 64 / ms, 2), "unit": "field round trips/s", "fields": 64,
             "groups": {str(L): nf for L, nf in groups}, "steps": steps,
             "ms_per_batch_round_trip": round(ms, 4), "fp64_path_ms_per_batch_round_trip": round(fp64_ms, 4),
-            "max_field_rel_l2_vs_fp64_path": err, "autotuned": tune,
+            "fp32_path_ms_per_batch_round_trip": round(fp32_ms, 4),
+            "fp32_path_value": round(1000.0 * 64 / fp32_ms, 2),
+            "max_field_rel_l2_vs_fp64_path": err, "fp32_max_field_rel_l2_vs_fp64_path": err32,
+            "autotuned": tune,
             "launches": "one hipGraph per direction; Legendre stages as persistent kernels over all groups"}
 
 
@@ -477,6 +533,52 @@ def main() -> int:
                        "lmax": LMAX, "nlat": NLAT, "nlon": NLON, "fields": nf},
             "roofline": sroof,
             "kernels_avg_us": {k: round(v, 2) for k, v in sk_us.items()},
+        }
+
+    # ---- batched FP32 matrix-core contraction, 16 fields
+    if not args.no_split:
+        nf = SPLIT_FIELDS
+        balm = to_device(torch, np, spectrum(np, nf), dev)
+        torch.cuda.synchronize()
+
+        def fstep():
+            return shtk.analysis_batch(cfg, shtk.synthesis_batch(cfg, balm, precision="fp32"), precision="fp32")
+
+        for _ in range(args.warmup):
+            fout = fstep()
+        fsyn = shtk.synthesis_batch(cfg, balm, precision="fp32")
+        rsyn = shtk.synthesis_batch(cfg, balm)  # FP64 batch path
+        torch.cuda.synchronize()
+        err_syn = max(float(torch.linalg.vector_norm(fsyn[..., k] - rsyn[..., k]) /
+                            torch.linalg.vector_norm(rsyn[..., k])) for k in range(nf))
+        err_in = max(float(torch.linalg.vector_norm(fout[..., k] - balm[..., k]) /
+                           torch.linalg.vector_norm(balm[..., k])) for k in range(nf))
+        fsteps = max(4, args.steps // 4)
+        f_no, f_ev, fkern, _ = timed_steps(torch, _lib, cfg, fstep, fsteps, stream)
+        fk_us = {k: 1000.0 * v[0] / v[1] for k, v in fkern.items()}
+        dom = max(fk_us, key=fk_us.get)
+        if dom.startswith("f32"):
+            fl = fp32_mfma_flops(LMAX, NLAT, nf)
+            froof = roofline_entry(dom, fk_us[dom], fl, PEAK_FP32_MFMA_TFLOPS, "TFLOP/s", "mfma",
+                                   f"{fl:.4e} flop = 2 x 2 real columns x (nlat/2)(lmax+1)(lmax+2)/2 x {nf} fields")
+        else:
+            by = fft_bytes(NLAT, NLON, LMAX, nf)
+            froof = roofline_entry(dom, fk_us[dom], by * 1e3, PEAK_HBM_GBS, "GB/s", "hbm", f"{by:.4e} bytes")
+        ms_batch = f_no / fsteps
+        res["fp32_batched"] = {
+            "value": round(1000.0 * nf / ms_batch, 2),
+            "unit": "SHT/s",
+            "fields": nf,
+            "steps": fsteps,
+            "ms_per_batch_round_trip": round(ms_batch, 5),
+            "speedup_vs_plain": round((1000.0 * nf / ms_batch) / res["value"], 3),
+            "dtype": "FP32 MFMA contraction (FP32 tiles of 64 products, FP64 sums across tiles), FP64 Legendre",
+            "max_field_rel_l2_synthesis_vs_fp64_path": err_syn,
+            "max_field_rel_l2_round_trip": err_in,
+            "config": {"workload": "batched FP32-MFMA scalar round trip (synthesis_batch+analysis_batch)",
+                       "lmax": LMAX, "nlat": NLAT, "nlon": NLON, "fields": nf},
+            "roofline": froof,
+            "kernels_avg_us": {k: round(v, 2) for k, v in fk_us.items()},
         }
 
     if not args.no_split and not args.no_uneven:

@@ -55,8 +55,12 @@ enum shtk_dtype { SHTK_F64 = 0, SHTK_F32 = 1 };
  * SHTK_PREC_FP64: the FP64 recurrence-and-contraction kernels (the plain path).
  * SHTK_PREC_SPLIT: FP64 Legendre values and tables, contraction as an Ozaki
  *   split on int8 matrix cores (6 slices, exact int32 accumulation, FP64
- *   cross-slice sums); FP64-grade results (relative L2 error ~1e-12). */
-enum shtk_precision { SHTK_PREC_FP64 = 0, SHTK_PREC_SPLIT = 1 };
+ *   cross-slice sums); FP64-grade results (relative L2 error ~1e-12).
+ * SHTK_PREC_FP32: FP64 Legendre values and tables, contraction on the FP32
+ *   matrix cores (v_mfma_f32_16x16x4_f32) with FP32 accumulation over K-tiles of
+ *   64 products and FP64 sums across tiles; FP32-grade results (relative L2
+ *   error ~1e-7 per field). */
+enum shtk_precision { SHTK_PREC_FP64 = 0, SHTK_PREC_SPLIT = 1, SHTK_PREC_FP32 = 2 };
 
 /* Grid and spectral descriptor.  Mirrors the keyword set of
  * create_gauss_config(lmax, nlat; mmax, mres, nlon, norm, cs_phase, real_norm,

@@ -18,7 +18,8 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 OBJ_DIR = os.path.join(PKG_DIR, "_build")
 LIB_NAME = "libshtk_mi355x.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
-SOURCES = ["shtk_plan.cpp", "shtk_legendre.hip", "shtk_fft.hip", "shtk_split.hip", "shtk_ml.hip"]
+SOURCES = ["shtk_plan.cpp", "shtk_legendre.hip", "shtk_fft.hip", "shtk_split.hip", "shtk_ml.hip",
+           "shtk_mfma32.hip"]
 ARCH = os.environ.get("SHTK_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 instructions are disabled for all device code; the host pass

@@ -14,7 +14,7 @@ from ._build import LIB_PATH
 
 SHTK_OK, SHTK_EINVAL, SHTK_EUNSUPPORTED, SHTK_ENOMEM, SHTK_EHIP = 0, 1, 2, 3, 4
 SHTK_F64, SHTK_F32 = 0, 1
-SHTK_PREC_FP64, SHTK_PREC_SPLIT = 0, 1
+SHTK_PREC_FP64, SHTK_PREC_SPLIT, SHTK_PREC_FP32 = 0, 1, 2
 
 # Every symbol include/shtk.h declares.
 EXPORTS = (

@@ -193,8 +193,10 @@ __device__ void stockham_gen(double2 *z, int N, int Ns, int R, const double2 *tw
     __syncthreads();
 }
 
-// All passes of a length-N transform of the LDS row z (unnormalised).
-template <int SIGN>
+// All passes of a length-N transform of the LDS row z (unnormalised).  GEN =
+// false compiles the radix 2/3/4/5/8 butterflies alone (the common lengths keep
+// their registers free of the generic pass's state).
+template <int SIGN, bool GEN>
 __device__ void fft_passes(double2 *z, int N, int nrad, unsigned long long radix, const double2 *tw, int tid,
                            int nt) {
     int Ns = 1;
@@ -205,21 +207,29 @@ __device__ void fft_passes(double2 *z, int N, int nrad, unsigned long long radix
             case 4: stockham_pass<4, SIGN>(z, N, Ns, tw, tid, nt); break;
             case 2: stockham_pass<2, SIGN>(z, N, Ns, tw, tid, nt); break;
             case 3: stockham_pass<3, SIGN>(z, N, Ns, tw, tid, nt); break;
-            case 5: stockham_pass<5, SIGN>(z, N, Ns, tw, tid, nt); break;
-            default: stockham_gen<SIGN>(z, N, Ns, R, tw, tid, nt); break;
+            default:
+                if constexpr (GEN) {
+                    if (R != 5) {
+                        stockham_gen<SIGN>(z, N, Ns, R, tw, tid, nt);
+                        break;
+                    }
+                }
+                stockham_pass<5, SIGN>(z, N, Ns, tw, tid, nt);
+                break;
         }
         Ns *= R;
     }
 }
 
 // Length-nlon DFT of the LDS row z (entries at zp(j), j < nlon; the row holds
-// nfft + nfft/16 + 1 slots).  Direct passes, or Bluestein: chirp, zero-pad to
-// nfft, cyclic convolution with the kernel spectrum (forward FFT, pointwise
-// product, inverse FFT), chirp.  The caller's barrier precedes; one follows.
-template <int SIGN>
+// nfft + nfft/16 + 1 slots).  Direct passes, or (GEN only) Bluestein: chirp,
+// zero-pad to nfft, cyclic convolution with the kernel spectrum (forward FFT,
+// pointwise product, inverse FFT), chirp.  The caller's barrier precedes; one
+// follows.
+template <int SIGN, bool GEN>
 __device__ void fft_lds(double2 *z, const FftArgs &a, int tid, int nt) {
-    if (!a.blue) {
-        fft_passes<SIGN>(z, a.nlon, a.nrad, a.radix, a.tw, tid, nt);
+    if (!GEN || !a.blue) {
+        fft_passes<SIGN, GEN>(z, a.nlon, a.nrad, a.radix, a.tw, tid, nt);
         return;
     }
     const int N = a.nlon, M = a.nfft;
@@ -233,11 +243,11 @@ __device__ void fft_lds(double2 *z, const FftArgs &a, int tid, int nt) {
         }
     }
     __syncthreads();
-    fft_passes<-1>(z, M, a.nrad, a.radix, a.tw, tid, nt);
+    fft_passes<-1, GEN>(z, M, a.nrad, a.radix, a.tw, tid, nt);
     const double2 *bh = a.bhat + (SIGN > 0 ? M : 0);
     for (int j = tid; j < M; j += nt) z[zp(j)] = cmul(z[zp(j)], bh[j]);
     __syncthreads();
-    fft_passes<+1>(z, M, a.nrad, a.radix, a.tw, tid, nt);
+    fft_passes<+1, GEN>(z, M, a.nrad, a.radix, a.tw, tid, nt);
     for (int j = tid; j < N; j += nt) {
         double2 c = a.chirp[j];
         if (SIGN > 0) c.y = -c.y;
@@ -290,7 +300,7 @@ __device__ __forceinline__ double fft_row_scale(const FftArgs &a, int lat) {
 // rows z = f_a + i f_b of latitudes L0 + g and L0 + 2G - 1 - g at positions zp(j)
 // (row stride RS = N + N/16 + 1 complex: the rows of one longitude fall in
 // different LDS banks).
-template <int G, typename GT>
+template <int G, typename GT, bool GEN>
 __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfields) {
     extern __shared__ double2 zs[];
     constexpr int L = 2 * G;
@@ -338,7 +348,7 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
             }
         }
         __syncthreads();
-        fft_lds<+1>(zs + row * RS, a, lt, tr);
+        fft_lds<+1, GEN>(zs + row * RS, a, lt, tr);
         const int nrow = a.nrow;
         GT *gf = (GT *)a.grid + (size_t)f * nrow * N + lat;
         if (live && lat < nrow) {
@@ -357,7 +367,7 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
 // each latitude is its own complex row with the bins m = 0 .. mmax of the
 // Fourier data (no Hermitian mirror) and the grid element is the complex value.
 // Tiles of G adjacent latitudes, one LDS row each.
-template <int G, typename GT>
+template <int G, typename GT, bool GEN>
 __global__ __launch_bounds__(1024) void k_fft_synth_c(const FftArgs a, int nfields) {
     extern __shared__ double2 zs[];
     const int N = a.nlon, nlat = a.nlat, RS = a.nfft + (a.nfft >> 4) + 1;
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(1024) void k_fft_synth_c(const FftArgs a, int nfiel
             if (live) z[zp(im * a.mres)] = Ff[(size_t)im * nlat];
         }
         __syncthreads();
-        fft_lds<+1>(zs + row * RS, a, lt, tr);
+        fft_lds<+1, GEN>(zs + row * RS, a, lt, tr);
         const int nrow = a.nrow;
         GT *gf = (GT *)a.grid + 2 * ((size_t)f * nrow * N + lat);
         if (live && lat < nrow) {
@@ -397,7 +407,7 @@ __global__ __launch_bounds__(1024) void k_fft_synth_c(const FftArgs a, int nfiel
 // 169-207 with a complex eltype): each latitude's complex row is transformed
 // and the bins m = 0 .. mmax are kept, the columns the m-loop reads
 // (Fph[:, m+1]).  Tiles of G adjacent latitudes, one LDS row each.
-template <int G, typename GT>
+template <int G, typename GT, bool GEN>
 __global__ __launch_bounds__(1024) void k_fft_anal_c(const FftArgs a, int nfields) {
     extern __shared__ double2 zs[];
     const int N = a.nlon, nlat = a.nlat, RS = a.nfft + (a.nfft >> 4) + 1;
@@ -420,7 +430,7 @@ __global__ __launch_bounds__(1024) void k_fft_anal_c(const FftArgs a, int nfield
                                : make_double2(0.0, 0.0);
         }
         __syncthreads();
-        fft_lds<-1>(zs + row * RS, a, lt, tr);
+        fft_lds<-1, GEN>(zs + row * RS, a, lt, tr);
         if (live) {
             double2 *Ff = a.F + (size_t)f * a.nm * nlat + lat;
             for (int e = tid; e < a.nm * G; e += nt) {
@@ -434,7 +444,7 @@ __global__ __launch_bounds__(1024) void k_fft_anal_c(const FftArgs a, int nfield
 
 // Analysis: grid rows -> Fourier data, F_a = (Z_m + conj Z_{N-m}) / 2,
 // F_b = (Z_m - conj Z_{N-m}) / (2i) of the packed row Z = f_a + i f_b.
-template <int G, typename GT>
+template <int G, typename GT, bool GEN>
 __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfields) {
     extern __shared__ double2 zs[];
     constexpr int L = 2 * G;
@@ -472,7 +482,7 @@ __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfield
             }
         }
         __syncthreads();
-        fft_lds<-1>(zs + row * RS, a, lt, tr);
+        fft_lds<-1, GEN>(zs + row * RS, a, lt, tr);
         if (live) {
             double2 *Ff = a.F + (size_t)f * a.nm * nlat + lat;
             for (int e = tid; e < a.nm * L; e += nt) {
@@ -491,38 +501,6 @@ __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfield
 }  // namespace
 
 namespace {
-template <int G>
-hipError_t launch_fft_g(const FftArgs &a, int nfields, int inverse, int tr, hipStream_t st) {
-    const int nt = G * tr;
-    const size_t lds = sizeof(double2) * (size_t)G * (a.nfft + (a.nfft >> 4) + 1);
-    const void *fn = inverse ? (a.io_f32 ? (const void *)k_fft_synth_t<G, float> : (const void *)k_fft_synth_t<G, double>)
-                             : (a.io_f32 ? (const void *)k_fft_anal_t<G, float> : (const void *)k_fft_anal_t<G, double>);
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    const int ntile = (a.nlat + 2 * G - 1) / (2 * G);
-    const int total = ntile * nfields;
-    // persistent grid: as many workgroups as fit the chip (LDS-limited), a multiple of 8
-    const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
-    int grid = std::min(256 * per_cu, (total + 7) / 8 * 8);
-    grid = std::max(8, grid / 8 * 8);
-    if (inverse) {
-        if (a.io_f32)
-            hipLaunchKernelGGL((k_fft_synth_t<G, float>), dim3(grid), dim3(nt), lds, st, a, nfields);
-        else
-            hipLaunchKernelGGL((k_fft_synth_t<G, double>), dim3(grid), dim3(nt), lds, st, a, nfields);
-    } else {
-        if (a.io_f32)
-            hipLaunchKernelGGL((k_fft_anal_t<G, float>), dim3(grid), dim3(nt), lds, st, a, nfields);
-        else
-            hipLaunchKernelGGL((k_fft_anal_t<G, double>), dim3(grid), dim3(nt), lds, st, a, nfields);
-    }
-    return hipGetLastError();
-}
-}  // namespace
-
-namespace {
 // Threads per LDS row: nfft / FFT_VPT rounded up to whole wavefronts (every
 // pass has at most FFT_VPT values per thread); 0 if the row does not fit.
 int fft_row_threads(const FftArgs &a, size_t *row_bytes) {
@@ -533,34 +511,77 @@ int fft_row_threads(const FftArgs &a, size_t *row_bytes) {
     return tr;
 }
 
-template <int G, typename GT>
+// The general kernels (generic prime passes, Bluestein) only where the length needs them.
+bool fft_general(const FftArgs &a) {
+    if (a.blue) return true;
+    for (int q = 0; q < a.nrad; ++q) {
+        const int R = (int)((a.radix >> (FFT_RADIX_BITS * q)) & ((1ull << FFT_RADIX_BITS) - 1));
+        if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) return true;
+    }
+    return false;
+}
+
+template <int G, typename GT, bool GEN>
+hipError_t launch_tile_g(const FftArgs &a, int nfields, int inverse, int tr, size_t row, hipStream_t st) {
+    const int nt = G * tr;
+    const size_t lds = row * G;
+    const void *fn = inverse ? (const void *)k_fft_synth_t<G, GT, GEN> : (const void *)k_fft_anal_t<G, GT, GEN>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const int ntile = (a.nlat + 2 * G - 1) / (2 * G);
+    const int total = ntile * nfields;
+    // persistent grid: as many workgroups as fit the chip (LDS-limited), a multiple of 8
+    const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
+    int grid = std::min(256 * per_cu, (total + 7) / 8 * 8);
+    grid = std::max(8, grid / 8 * 8);
+    if (inverse) hipLaunchKernelGGL((k_fft_synth_t<G, GT, GEN>), dim3(grid), dim3(nt), lds, st, a, nfields);
+    else hipLaunchKernelGGL((k_fft_anal_t<G, GT, GEN>), dim3(grid), dim3(nt), lds, st, a, nfields);
+    return hipGetLastError();
+}
+
+template <int G>
+hipError_t launch_tile(const FftArgs &a, int nfields, int inverse, int tr, size_t row, hipStream_t st) {
+    const bool gen = fft_general(a);
+    if (a.io_f32)
+        return gen ? launch_tile_g<G, float, true>(a, nfields, inverse, tr, row, st)
+                   : launch_tile_g<G, float, false>(a, nfields, inverse, tr, row, st);
+    return gen ? launch_tile_g<G, double, true>(a, nfields, inverse, tr, row, st)
+               : launch_tile_g<G, double, false>(a, nfields, inverse, tr, row, st);
+}
+
+template <int G, typename GT, bool GEN>
 hipError_t launch_cplx_g(const FftArgs &a, int nfields, int inverse, int tr, size_t row, hipStream_t st) {
     const size_t lds = row * G;
-    const void *fn = inverse ? (const void *)k_fft_synth_c<G, GT> : (const void *)k_fft_anal_c<G, GT>;
+    const void *fn = inverse ? (const void *)k_fft_synth_c<G, GT, GEN> : (const void *)k_fft_anal_c<G, GT, GEN>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const int total = (a.nlat + G - 1) / G * nfields;
     const int grid = std::max(1, std::min(256 * (int)std::max<size_t>(1, (160 * 1024) / lds), total));
-    if (inverse) hipLaunchKernelGGL((k_fft_synth_c<G, GT>), dim3(grid), dim3(G * tr), lds, st, a, nfields);
-    else hipLaunchKernelGGL((k_fft_anal_c<G, GT>), dim3(grid), dim3(G * tr), lds, st, a, nfields);
+    if (inverse) hipLaunchKernelGGL((k_fft_synth_c<G, GT, GEN>), dim3(grid), dim3(G * tr), lds, st, a, nfields);
+    else hipLaunchKernelGGL((k_fft_anal_c<G, GT, GEN>), dim3(grid), dim3(G * tr), lds, st, a, nfields);
     return hipGetLastError();
 }
 
-template <typename GT>
+template <typename GT, bool GEN>
 hipError_t launch_cplx_t(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
     size_t row = 0;
     const int tr = fft_row_threads(a, &row);
     if (!tr) return hipErrorNotSupported;
-    if (4 * row <= 160 * 1024 && 4 * tr <= 1024) return launch_cplx_g<4, GT>(a, nfields, inverse, tr, row, st);
-    if (2 * row <= 160 * 1024 && 2 * tr <= 1024) return launch_cplx_g<2, GT>(a, nfields, inverse, tr, row, st);
-    return launch_cplx_g<1, GT>(a, nfields, inverse, tr, row, st);
+    if (4 * row <= 160 * 1024 && 4 * tr <= 1024) return launch_cplx_g<4, GT, GEN>(a, nfields, inverse, tr, row, st);
+    if (2 * row <= 160 * 1024 && 2 * tr <= 1024) return launch_cplx_g<2, GT, GEN>(a, nfields, inverse, tr, row, st);
+    return launch_cplx_g<1, GT, GEN>(a, nfields, inverse, tr, row, st);
 }
 }  // namespace
 
 hipError_t launch_fft_cplx(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
-    return a.io_f32 ? launch_cplx_t<float>(a, nfields, inverse, st) : launch_cplx_t<double>(a, nfields, inverse, st);
+    const bool gen = fft_general(a);
+    if (a.io_f32)
+        return gen ? launch_cplx_t<float, true>(a, nfields, inverse, st) : launch_cplx_t<float, false>(a, nfields, inverse, st);
+    return gen ? launch_cplx_t<double, true>(a, nfields, inverse, st) : launch_cplx_t<double, false>(a, nfields, inverse, st);
 }
 
 hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st) {
@@ -571,9 +592,9 @@ hipError_t launch_fft(const FftArgs &a, int nfields, int inverse, hipStream_t st
     const long long units4 = (long long)((a.nlat + 7) / 8) * nfields, units2 = (long long)((a.nlat + 3) / 4) * nfields;
     int G = a.tile_pairs;
     if (G <= 0) G = units4 >= 512 ? 4 : units2 >= 256 ? 2 : 1;
-    if (G >= 4 && 4 * row <= 160 * 1024 && 4 * tr <= 1024) return launch_fft_g<4>(a, nfields, inverse, tr, st);
-    if (G >= 2 && 2 * row <= 160 * 1024 && 2 * tr <= 1024) return launch_fft_g<2>(a, nfields, inverse, tr, st);
-    return launch_fft_g<1>(a, nfields, inverse, tr, st);
+    if (G >= 4 && 4 * row <= 160 * 1024 && 4 * tr <= 1024) return launch_tile<4>(a, nfields, inverse, tr, row, st);
+    if (G >= 2 && 2 * row <= 160 * 1024 && 2 * tr <= 1024) return launch_tile<2>(a, nfields, inverse, tr, row, st);
+    return launch_tile<1>(a, nfields, inverse, tr, row, st);
 }
 
 }  // namespace shtk

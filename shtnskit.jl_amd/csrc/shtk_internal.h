@@ -275,6 +275,7 @@ struct OzArgs {
     int4 *gdig;             // [nfg][nm][nks][2][6][2][64] B operands (pairs as k)
     double *gcolsum;        // [nfg][nm][nks][2][32]
     int *eg;                // [nfg][nm][npb][2][32] column exponents per pair block
+    float *gf32;            // FP32 contraction: analysis field operand (k_f32_gprep layout)
 };
 
 struct OzSeedArgs {
@@ -316,6 +317,11 @@ hipError_t launch_oz_anal_grp(const OzGroups *gprep, const OzGroups *units, int 
 hipError_t launch_oz_synth(const OzArgs &a, hipStream_t st);
 hipError_t launch_oz_asc(const OzAscArgs &a, hipStream_t st);
 hipError_t launch_oz_anal(const OzArgs &a, hipStream_t st);
+// FP32 matrix-core contraction (shtk_mfma32.hip): same arguments and outputs as
+// the split kernels (synthesis reads the canonical coefficients directly; the
+// analysis builds its FP32 field operand in a.gf32 and writes a.spec_out).
+hipError_t launch_f32_synth(const OzArgs &a, hipStream_t st);
+hipError_t launch_f32_anal(const OzArgs &a, hipStream_t st);
 
 // Kernel launchers (shtk_legendre.hip, shtk_fft.hip).
 hipError_t launch_seed(const SeedArgs &a, hipStream_t st);

@@ -443,6 +443,18 @@ int check_common(shtk_plan *p, int nfields, int dtype, std::initializer_list<con
     return SHTK_OK;
 }
 
+// Contraction precision of the *_ex entry points; the matrix-core paths share the
+// split tables, which cover lmax <= 4095 (4094 for the vector forms, which run
+// the scalar contraction to degree lmax + 1).
+int check_precision(const shtk_plan *p, int precision, int lmax_limit) {
+    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT && precision != SHTK_PREC_FP32)
+        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64, SHTK_PREC_SPLIT or SHTK_PREC_FP32");
+    if (precision != SHTK_PREC_FP64 && p && p->d.lmax > lmax_limit)
+        return fail(SHTK_EUNSUPPORTED, "matrix-core (split or FP32) transforms support lmax <= " +
+                                           std::to_string(lmax_limit));
+    return SHTK_OK;
+}
+
 int norm_ltr(const shtk_plan *p, int ltr, int *out) {
     if (ltr < 0) ltr = p->d.lmax;
     if (ltr > p->d.lmax) return fail(SHTK_EINVAL, "ltr must be <= lmax");
@@ -646,6 +658,59 @@ size_t oz_ws_bytes(const shtk_plan *p, int nf, bool with_acc) {
     return std::max(oz_ws_layout(p, nfg, false).total, oz_ws_layout(p, nfg, true).total + acc);
 }
 
+// FP32 contraction workspace: the analysis field operand (FP32, k_f32_gprep
+// layout: per field group, order, pair block and K half, 4 x 4096 floats) and
+// an FP64 canonical accumulator when the output is not canonical FP64.
+size_t f32_gbytes(const shtk_plan *p, int nf) {
+    const size_t nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
+    return (nfg * p->nm * oz_npb(p) * oz_nh(p) * 4 * 4096 * sizeof(float) + 255) / 256 * 256;
+}
+
+size_t f32_ws_bytes(const shtk_plan *p, int nf, bool with_acc) {
+    const size_t acc = with_acc ? (size_t)nf * (p->d.lmax + 1) * (p->d.mmax + 1) * 16 : 0;
+    return f32_gbytes(p, nf) + acc;
+}
+
+shtk::OzArgs f32_synth_args(const shtk_plan *p, const void *spec_in, int in_f32, int nf, int ltr, double g,
+                            double2 *F0) {
+    shtk::OzArgs o = oz_common_args(p, nf, ltr, g);
+    o.q0 = p->oz_q0;
+    o.q1 = p->oz_q1;
+    o.qk = p->oz_qk;
+    o.ep = p->oz_ep;
+    o.spec_in = spec_in;
+    o.in_f32 = in_f32;
+    o.F0 = F0;
+    return o;
+}
+
+shtk::OzArgs f32_anal_args(const shtk_plan *p, double2 *F0, double *acc, int nf, int ltr, const double *wlat,
+                           double g) {
+    shtk::OzArgs o = oz_common_args(p, nf, ltr, g);
+    o.F0 = F0;
+    o.qc = p->qc;
+    o.asc = p->oz_asc;
+    o.epl = p->oz_epl;
+    o.qb = p->qb;
+    o.qcc = p->qcc;
+    o.seedq0 = p->seedq0;
+    o.seedq1 = p->seedq1;
+    o.seedk = p->seedk;
+    o.segbase = p->segbase;
+    o.segm = p->segm;
+    o.pfirst = p->pfirst;
+    o.seg = p->seg;
+    o.nseg_total = p->nseg_total;
+    o.nh = oz_nh(p);
+    o.npb = oz_npb(p);
+    o.run = p->oz_run;
+    o.nrun = p->oz_nrun;
+    o.wlat = wlat;
+    o.spec_out = acc;
+    o.gf32 = (float *)p->oz_ws;
+    return o;
+}
+
 // One transform direction with its scalings (see the entry points below).
 struct Job {
     int vector;     // 0 scalar, 1 spheroidal/toroidal pair
@@ -716,7 +781,7 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
     a.in0 = s0;
     a.in1 = s1;
     a.io_f32 = dtype == SHTK_F32;
-    if (!j.vector && j.prec != SHTK_PREC_SPLIT) {
+    if (!j.vector && j.prec == SHTK_PREC_FP64) {
         // scalar FP64 synthesis: coefficients times c_l (Q form), converted to the
         // canonical convention on the way
         double *dst = (double *)spec;
@@ -725,7 +790,7 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
                 "coefficient scaling");
         a.in0 = dst;
         a.io_f32 = 0;
-    } else if (j.vector && j.prec != SHTK_PREC_SPLIT) {
+    } else if (j.vector && j.prec == SHTK_PREC_FP64) {
         // vector FP64 synthesis: records (S', T', c_l S', c_l T') with the convention
         // map and the l(l+1) factor applied
         double *dst = (double *)spec;
@@ -759,6 +824,11 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
         const shtk::OzArgs o = oz_synth_args(p, a.in0, a.io_f32, nf, j.ltr, j.g, F0);
         HIP_TRY(timed(p, "oz_synth", st, [&] { return shtk::launch_oz_synth(o, st); }),
                 "split synthesis launch");
+    } else if (j.prec == SHTK_PREC_FP32) {
+        RC_TRY(ensure_oz_tables(p, st));
+        const shtk::OzArgs o = f32_synth_args(p, a.in0, a.io_f32, nf, j.ltr, j.g, F0);
+        HIP_TRY(timed(p, "f32_synth", st, [&] { return shtk::launch_f32_synth(o, st); }),
+                "FP32 synthesis launch");
     } else {
         HIP_TRY(timed(p, j.leg_name, st,
                       [&] { return shtk::launch_leg_synth(a, nf, j.vector, p->synth_chains, st); }),
@@ -837,6 +907,20 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
         a.out0 = spec;
         a.out1 = spec + L.spec_bytes;
         a.io_f32 = 0;
+    }
+    if (j.prec == SHTK_PREC_FP32) {
+        RC_TRY(ensure_oz_tables(p, st));
+        const bool direct = p->canonical && dtype == SHTK_F64;
+        RC_TRY(ensure_oz_ws(p, f32_ws_bytes(p, nf, !direct)));
+        double *acc = direct ? (double *)s0 : (double *)((char *)p->oz_ws + f32_gbytes(p, nf));
+        HIP_TRY(shtk::launch_zero(acc, (size_t)nf * a.spec_fstride * 16, st), "output clear");
+        const shtk::OzArgs o = f32_anal_args(p, F0, acc, nf, j.ltr, j.wlat, j.g);
+        HIP_TRY(timed(p, "f32_anal", st, [&] { return shtk::launch_f32_anal(o, st); }), "FP32 analysis launch");
+        if (!direct)
+            HIP_TRY(shtk::launch_spec_scale(acc, s0, 0, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale,
+                                            j.conv_div, p->d.lmax, p->d.mmax, p->d.mres, a.spec_fstride, nf, st),
+                    "coefficient conversion");
+        return SHTK_OK;
     }
     if (j.prec == SHTK_PREC_SPLIT) {
         RC_TRY(ensure_oz_tables(p, st));
@@ -944,7 +1028,8 @@ int run_vsplit_to_grid(shtk_plan *p, const Job &j, const void *S, const void *T,
     RC_TRY(ensure_vshadow(p));
     shtk_plan *s = p->vshadow;
     RC_TRY(ensure_oz_tables(s, st));
-    RC_TRY(ensure_oz_ws(s, oz_ws_bytes(s, 2 * nf, false)));
+    RC_TRY(ensure_oz_ws(s, j.prec == SHTK_PREC_FP32 ? f32_ws_bytes(s, 2 * nf, false)
+                                                   : oz_ws_bytes(s, 2 * nf, false)));
     char *ws = (char *)p->ws;
     double2 *F0 = (double2 *)ws, *F1 = F0 + (size_t)nf * p->nm * p->d.nlat;
     double *C = (double *)(ws + 2 * L.f_bytes);  // 2 nf (lmax+2)(mmax+1) complex <= the 4 spectral regions
@@ -953,8 +1038,13 @@ int run_vsplit_to_grid(shtk_plan *p, const Job &j, const void *S, const void *T,
     HIP_TRY(shtk::launch_vsplit_coef(S, T, C, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale, j.conv_div,
                                      p->d.lmax, p->d.mres, p->nm, j.ltr, in_fs, out_fs, nf, st),
             "vector coefficient combination");
-    const shtk::OzArgs o = oz_synth_args(s, C, 0, 2 * nf, j.ltr + 1, j.g, F0);
-    HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_oz_synth(o, st); }), "split synthesis launch");
+    if (j.prec == SHTK_PREC_FP32) {
+        const shtk::OzArgs o = f32_synth_args(s, C, 0, 2 * nf, j.ltr + 1, j.g, F0);
+        HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_f32_synth(o, st); }), "FP32 synthesis launch");
+    } else {
+        const shtk::OzArgs o = oz_synth_args(s, C, 0, 2 * nf, j.ltr + 1, j.g, F0);
+        HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_oz_synth(o, st); }), "split synthesis launch");
+    }
     // Low orders: near the poles sin(theta) V is far smaller than its terms there
     // (for m = 0 it vanishes like sin^2), so the division by sin(theta) would lift
     // the split rounding of those rows above 1e-12; the FP64 kernel recomputes the
@@ -1006,7 +1096,8 @@ int run_vsplit_to_spec(shtk_plan *p, const Job &j, const void *Vt, const void *V
     RC_TRY(ensure_vshadow(p));
     shtk_plan *s = p->vshadow;
     RC_TRY(ensure_oz_tables(s, st));
-    RC_TRY(ensure_oz_ws(s, oz_ws_bytes(s, 2 * nf, false)));
+    RC_TRY(ensure_oz_ws(s, j.prec == SHTK_PREC_FP32 ? f32_ws_bytes(s, 2 * nf, false)
+                                                   : oz_ws_bytes(s, 2 * nf, false)));
     char *ws = (char *)p->ws;
     double2 *F0 = (double2 *)ws, *F1 = F0 + (size_t)nf * p->nm * p->d.nlat;
     const void *grids[2] = {Vt, Vp};
@@ -1022,8 +1113,13 @@ int run_vsplit_to_spec(shtk_plan *p, const Job &j, const void *Vt, const void *V
     const long long out_fs = (long long)(p->d.lmax + 1) * (p->d.mmax + 1);
     const long long g_fs = (long long)(p->d.lmax + 2) * (p->d.mmax + 1);
     HIP_TRY(shtk::launch_zero(G, (size_t)2 * nf * g_fs * 16, st), "analysis sums clear");
-    const shtk::OzArgs o = oz_anal_args(s, F0, G, 2 * nf, j.ltr + 1, j.wlat, j.g);
-    HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_oz_anal(o, st); }), "split analysis launch");
+    if (j.prec == SHTK_PREC_FP32) {
+        const shtk::OzArgs o = f32_anal_args(s, F0, G, 2 * nf, j.ltr + 1, j.wlat, j.g);
+        HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_f32_anal(o, st); }), "FP32 analysis launch");
+    } else {
+        const shtk::OzArgs o = oz_anal_args(s, F0, G, 2 * nf, j.ltr + 1, j.wlat, j.g);
+        HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_oz_anal(o, st); }), "split analysis launch");
+    }
     HIP_TRY(shtk::launch_vsplit_out(G, S, T, dtype == SHTK_F32, p->canonical ? nullptr : p->convscale, j.conv_div,
                                     p->d.lmax, p->d.mmax, p->d.mres, j.ltr, g_fs, out_fs, nf, st),
             "vector coefficient combination");
@@ -1189,9 +1285,14 @@ int batch_split(shtk_batch *b, int dir, const void *const *in, void *const *out,
 // One direction of a batch, eagerly (no graph).
 int batch_issue(shtk_batch *b, int dir, const void *const *in, void *const *out, int dtype, hipStream_t st) {
     if (b->precision == SHTK_PREC_SPLIT) return batch_split(b, dir, in, out, dtype, st);
-    for (int gi = 0; gi < b->ngroups; ++gi) {
-        const int rc = dir == 0 ? shtk_synthesis(b->plans[gi], in[gi], out[gi], b->nf[gi], dtype, 1, -1, st)
-                                : shtk_analysis(b->plans[gi], in[gi], out[gi], b->nf[gi], dtype, -1, st);
+    // FP64 and FP32: one configuration at a time, largest first (a captured graph
+    // replays the whole sequence with one launch)
+    for (int q = 0; q < b->ngroups; ++q) {
+        const int gi = b->order[q];
+        const int rc = dir == 0 ? shtk_synthesis_ex(b->plans[gi], in[gi], out[gi], b->nf[gi], dtype, 1, -1,
+                                                    b->precision, st)
+                                : shtk_analysis_ex(b->plans[gi], in[gi], out[gi], b->nf[gi], dtype, -1, b->precision,
+                                                   st);
         if (rc) return rc;
     }
     return SHTK_OK;
@@ -1206,6 +1307,9 @@ int batch_prepare(shtk_batch *b, int dtype, hipStream_t st) {
         if (b->precision == SHTK_PREC_SPLIT) {
             RC_TRY(ensure_oz_tables(p, st));
             RC_TRY(ensure_oz_ws(p, oz_ws_bytes(p, b->nf[gi], !p->canonical || dtype != SHTK_F64)));
+        } else if (b->precision == SHTK_PREC_FP32) {
+            RC_TRY(ensure_oz_tables(p, st));
+            RC_TRY(ensure_oz_ws(p, f32_ws_bytes(p, b->nf[gi], !p->canonical || dtype != SHTK_F64)));
         }
     }
     return SHTK_OK;
@@ -1281,12 +1385,12 @@ int shtk_batch_create(const shtk_desc *descs, const int *nfields, int ngroups, i
     if (!descs || !nfields) return fail(SHTK_EINVAL, "descs and nfields must not be NULL");
     if (ngroups < 1 || ngroups > shtk::OZ_MAX_GROUPS)
         return fail(SHTK_EINVAL, "ngroups must be between 1 and " + std::to_string(shtk::OZ_MAX_GROUPS));
-    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT)
-        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
+    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT && precision != SHTK_PREC_FP32)
+        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64, SHTK_PREC_SPLIT or SHTK_PREC_FP32");
     for (int g = 0; g < ngroups; ++g) {
         if (nfields[g] < 1) return fail(SHTK_EINVAL, "every group needs at least one field");
-        if (precision == SHTK_PREC_SPLIT && descs[g].lmax > 4095)
-            return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
+        if (precision != SHTK_PREC_FP64 && descs[g].lmax > 4095)
+            return fail(SHTK_EUNSUPPORTED, "matrix-core (split or FP32) transforms support lmax <= 4095");
     }
     shtk_batch *b = new (std::nothrow) shtk_batch();
     if (!b) return fail(SHTK_ENOMEM, "host allocation failed");
@@ -1789,10 +1893,7 @@ int shtk_synthesis_static(shtk_plan *p, const void *alm, void *f, int nfields, i
 int shtk_synthesis_ex(shtk_plan *p, const void *alm, void *f, int nfields, int dtype, int real_output, int ltr,
                       int precision, void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {alm, f}));
-    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT)
-        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
-    if (precision == SHTK_PREC_SPLIT && p->d.lmax > 4095)
-        return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
+    RC_TRY(check_precision(p, precision, 4095));
     RC_TRY(norm_ltr(p, ltr, &ltr));
     // real output: f = Re ifft(phi_inv_scale * F) with the Hermitian mirror, the 1/nlon
     // of the inverse FFT folded in g; complex output (src/core_transforms.jl:294-344):
@@ -1806,10 +1907,7 @@ int shtk_synthesis_ex(shtk_plan *p, const void *alm, void *f, int nfields, int d
 int shtk_analysis_ex(shtk_plan *p, const void *f, void *alm, int nfields, int dtype, int ltr, int precision,
                      void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {alm, f}));
-    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT)
-        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
-    if (precision == SHTK_PREC_SPLIT && p->d.lmax > 4095)
-        return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
+    RC_TRY(check_precision(p, precision, 4095));
     RC_TRY(norm_ltr(p, ltr, &ltr));
     Job j{0, 1, p->wscaled, 1.0, 0, 0, 0, ltr, "leg_anal", "fft_anal"};
     j.prec = precision;
@@ -1819,10 +1917,7 @@ int shtk_analysis_ex(shtk_plan *p, const void *f, void *alm, int nfields, int dt
 int shtk_analysis_cplx(shtk_plan *p, const void *f, void *alm, int nfields, int dtype, int ltr, int precision,
                        void *stream) {
     RC_TRY(check_common(p, nfields, dtype, {alm, f}));
-    if (precision != SHTK_PREC_FP64 && precision != SHTK_PREC_SPLIT)
-        return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
-    if (precision == SHTK_PREC_SPLIT && p->d.lmax > 4095)
-        return fail(SHTK_EUNSUPPORTED, "split-precision transforms support lmax <= 4095");
+    RC_TRY(check_precision(p, precision, 4095));
     RC_TRY(norm_ltr(p, ltr, &ltr));
     Job j{0, 1, p->wscaled, 1.0, 0, 0, 0, ltr, "leg_anal", "fft_anal"};
     j.prec = precision;
@@ -1894,9 +1989,8 @@ int shtk_synthesis_sphtor_ex(shtk_plan *p, const void *S, const void *T, void *V
                              int dtype, int real_output, int ltr, int precision, void *stream) {
     if (precision == SHTK_PREC_FP64)
         return shtk_synthesis_sphtor(p, S, T, Vt, Vp, nfields, dtype, real_output, ltr, stream);
-    if (precision != SHTK_PREC_SPLIT) return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
+    RC_TRY(check_precision(p, precision, 4094));
     RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
-    if (p->d.lmax > 4094) return fail(SHTK_EUNSUPPORTED, "split-precision vector transforms support lmax <= 4094");
     RC_TRY(norm_ltr(p, ltr, &ltr));
     // the contraction gives sin(theta) V; the phi FFT divides by sin(theta) unless Robert form
     Job j{1, 0, nullptr, p->inv_scale / p->d.nlon, 0, 0, p->d.robert_form ? 0 : 1, ltr, "oz_synth_sphtor",
@@ -1909,9 +2003,8 @@ int shtk_synthesis_sphtor_ex(shtk_plan *p, const void *S, const void *T, void *V
 int shtk_analysis_sphtor_ex(shtk_plan *p, const void *Vt, const void *Vp, void *S, void *T, int nfields, int dtype,
                             int ltr, int precision, void *stream) {
     if (precision == SHTK_PREC_FP64) return shtk_analysis_sphtor(p, Vt, Vp, S, T, nfields, dtype, ltr, stream);
-    if (precision != SHTK_PREC_SPLIT) return fail(SHTK_EINVAL, "precision must be SHTK_PREC_FP64 or SHTK_PREC_SPLIT");
+    RC_TRY(check_precision(p, precision, 4094));
     RC_TRY(check_common(p, nfields, dtype, {S, T, Vt, Vp}));
-    if (p->d.lmax > 4094) return fail(SHTK_EUNSUPPORTED, "split-precision vector transforms support lmax <= 4094");
     RC_TRY(norm_ltr(p, ltr, &ltr));
     if (p->pole_weights && p->d.robert_form && p->d.mres == 1 && p->d.mmax >= 1 && ltr >= 1)
         return fail(SHTK_EINVAL, "Robert-form components with m=1 cannot be unweighted at nonzero-weight pole "

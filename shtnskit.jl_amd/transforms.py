@@ -144,12 +144,13 @@ def _nf(t: torch.Tensor, batch: bool) -> int:
     return int(t.shape[2]) if batch else 1
 
 
-_PRECISIONS = {"fp64": _lib.SHTK_PREC_FP64, "split": _lib.SHTK_PREC_SPLIT}
+_PRECISIONS = {"fp64": _lib.SHTK_PREC_FP64, "split": _lib.SHTK_PREC_SPLIT, "fp32": _lib.SHTK_PREC_FP32}
 
 
 def _precision(op, prec) -> int:
     """Contraction precision keyword of the batch transforms: "fp64" (the plain
-    FP64 kernels) or "split" (Ozaki split on int8 matrix cores, FP64-grade)."""
+    FP64 kernels), "split" (Ozaki split on int8 matrix cores, FP64-grade) or
+    "fp32" (FP32 matrix cores, FP32-grade; FP64 Legendre values)."""
     try:
         return _PRECISIONS[str(prec).lstrip(":").lower()]
     except KeyError:

@@ -151,6 +151,7 @@ struct shtk_plan {
     shtk_desc d{};
     int np = 0, nm = 0, seg = 128, nseg_total = 0;
     int xcd_map = 0, synth_chains = 4, fft_tile = 0;  // launch-shape knobs
+    int vsynth_chains = 1;  // latitude pairs per lane of the vector synthesis (1 or 2)
     int oz_nh_force = 0;  // split analysis pair halves per block (0: by grid size)
     bool canonical = true;
     bool poles = false;  // nodes at theta = 0 and pi (the first pair)

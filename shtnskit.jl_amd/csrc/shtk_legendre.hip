@@ -28,6 +28,8 @@
 //     segment start take a branch-free loop.
 //   * Analysis: each lane sums its K chains in registers; per-degree sums over
 //     lanes go through a wave-private LDS transpose every few degrees.
+#include <algorithm>
+
 #include "shtk_internal.h"
 
 namespace shtk {
@@ -646,6 +648,223 @@ __global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Vector synthesis with the per-degree operands in VGPRs (64-degree segments),
+// the treatment of k_synth_d: lane 16 r + n holds the record of degree
+// l0 + 16 g + n of record group g -- beta_{l+1}, l c_l, g_l c_{l-1} and the
+// prologue record (S', T', c_l S', c_l T') -- and every use broadcasts lane n of
+// the row (DPP row_newbcast:n) into v_fmac_f64, so the degree loop has no scalar
+// loads and no waits.  The next group's records are loaded (vector loads)
+// while the current group runs.  Per degree and chain: x Q_l, D_l = l c_l x Q_l
+// - g_l c_{l-1} Q_{l-1} (two FMAs), eight contraction FMAs and the recurrence FMA.
+// ---------------------------------------------------------------------------
+struct VRec {
+    double b, dl, dg;  // beta_{l+1}, l c_l, g_l c_{l-1}
+    double R[8];       // S'r, S'i, T'r, T'i, c S'r, c S'i, c T'r, c T'i
+};
+
+#define SHTK_FMAC_BC(dst, bc, v, N, NEG)                                                             \
+    asm volatile("v_fmac_f64_dpp %0, " NEG "%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" \
+                 : "+v"(dst)                                                                    \
+                 : "v"(bc), "v"(v), "i"(N))
+
+template <int K, int PAR, int N, bool NOP, bool FAST>
+__device__ __forceinline__ void vdstep(const VRec &r, const double *x, double *prev, double *cur, const double *cm,
+                                       double (*acc)[2][8]) {
+    double xc[K], D[K], Q[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        xc[j] = x[j] * cur[j];
+        D[j] = 0.0;
+    }
+    // five wait states between a VALU write of the records (or of EXEC in a
+    // rescale) and their first DPP read
+    if (NOP) asm volatile("s_nop 4");
+#pragma unroll
+    for (int j = 0; j < K; ++j) SHTK_FMAC_BC(D[j], r.dl, xc[j], N, "");
+#pragma unroll
+    for (int j = 0; j < K; ++j) SHTK_FMAC_BC(D[j], r.dg, prev[j], N, "-");
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        Q[j] = cur[j];
+        if (!FAST) {
+            D[j] *= cm[j];
+            Q[j] *= cm[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        double *A = acc[j][PAR];
+        SHTK_FMAC_BC(A[0], r.R[0], D[j], N, "");
+        SHTK_FMAC_BC(A[1], r.R[1], D[j], N, "");
+        SHTK_FMAC_BC(A[2], r.R[6], Q[j], N, "");
+        SHTK_FMAC_BC(A[3], r.R[7], Q[j], N, "");
+        SHTK_FMAC_BC(A[4], r.R[4], Q[j], N, "");
+        SHTK_FMAC_BC(A[5], r.R[5], Q[j], N, "");
+        SHTK_FMAC_BC(A[6], r.R[2], D[j], N, "");
+        SHTK_FMAC_BC(A[7], r.R[3], D[j], N, "");
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) SHTK_FMAC_BC(xc[j], r.b, prev[j], N, "-");
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        prev[j] = cur[j];
+        cur[j] = xc[j];
+    }
+}
+
+// Steps N = 0 .. 15 of one record group while N < nrem; (l - m) has the parity
+// of N.  Scaled chains are checked every 8 degrees with c_l of the last degree.
+template <int K, bool FAST, int N>
+__device__ __forceinline__ bool vdgroup_from(const VRec &r, cptr<double> qcm, int l0g, int nrem, const double *x,
+                                             double *prev, double *cur, int *kk, double *cm, double (*acc)[2][8]) {
+    if constexpr (N == 16) {
+        return true;
+    } else {
+        if (N >= nrem) return false;
+        vdstep<K, N & 1, N, N == 0 || (!FAST && N == 8), FAST>(r, x, prev, cur, cm, acc);
+        if (!FAST && (N & 7) == 7) rescale<K>(prev, cur, kk, cm, qcm[l0g + N]);
+        return vdgroup_from<K, FAST, N + 1>(r, qcm, l0g, nrem, x, prev, cur, kk, cm, acc);
+    }
+}
+
+// The accumulated sums of the pair block -> Fourier columns of both components
+// (north and south rows by parity), as k_synth_v.
+template <int K>
+__device__ __forceinline__ void vsynth_out(const LegArgs &a, double (*acc)[2][8], double (*red)[16][64], int w,
+                                           int lane, int pb0, int im, int f) {
+    const int m = im * a.mres, np = a.np;
+    const double mm = (double)m;
+    const double gm = a.g * mfac_of(a.mfac_mode, m);
+    const size_t fo = ((size_t)f * a.nm + im) * a.nlat;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (j > 0) __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 8; ++r) red[w][8 * q + r][lane] = acc[j][q][r];
+        __syncthreads();
+        const int p = pb0 + lane + 64 * j;
+        if (w == 0 && p < np) {
+            double e[8], o[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                e[r] = ((red[0][r][lane] + red[1][r][lane]) + red[2][r][lane]) + red[3][r][lane];
+                o[r] = ((red[0][8 + r][lane] + red[1][8 + r][lane]) + red[2][8 + r][lane]) + red[3][8 + r][lane];
+            }
+#pragma unroll
+            for (int r = 2; r < 6; ++r) {
+                e[r] *= mm;
+                o[r] *= mm;
+            }
+            const double sc = gm * (a.wlat ? a.wlat[p] : 1.0);
+            // [0..1] = A, [2..3] = m B, [4..5] = m C, [6..7] = E
+            const double tNr = e[0] + o[0] + (e[3] + o[3]), tNi = e[1] + o[1] - (e[2] + o[2]);
+            const double tSr = -e[0] + o[0] + (e[3] - o[3]), tSi = -e[1] + o[1] - (e[2] - o[2]);
+            const double pNr = -(e[5] + o[5]) + e[6] + o[6], pNi = e[4] + o[4] + e[7] + o[7];
+            const double pSr = -(e[5] - o[5]) - e[6] + o[6], pSi = (e[4] - o[4]) - e[7] + o[7];
+            a.F0[fo + p] = make_double2(tNr * sc, tNi * sc);
+            a.F1[fo + p] = make_double2(pNr * sc, pNi * sc);
+            const int ps = a.nlat - 1 - p;
+            if (ps != p) {
+                a.F0[fo + ps] = make_double2(tSr * sc, tSi * sc);
+                a.F1[fo + ps] = make_double2(pSr * sc, pSi * sc);
+            }
+        }
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_synth_vd(const LegArgs a) {
+    __shared__ double red[4][16][64];
+    int b, unit;
+    if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
+    const int im = unit % a.gy, f = unit / a.gy;
+    const int m = im * a.mres;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int np = a.np, lmax = a.lmax, ltr = a.ltr;
+    const int pb0 = b * 64 * K, pb1 = min(pb0 + 64 * K, np);
+    double x[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int p = pb0 + lane + 64 * j;
+        x[j] = p < np ? a.x[p] : 0.0;
+    }
+    const long long co = as_const(a.coff)[im] - m;
+    const cptr<double> qcm = as_const(a.qcc) + co;
+    const double *qbm = a.qb + co;                              // padded past lmax + 1
+    const double2 *qdm = a.qd + co;                             // entries l = m .. lmax + 1
+    const double4 *rec = (const double4 *)a.in0 + 2 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m);
+    double acc[K][2][8];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[j][q][r] = 0.0;
+    const int nseg = nseg_of(lmax, m, DSEG);
+    const int segb = as_const(a.segbase)[im];
+    // record group (segment s, group g) of this lane's degree
+    auto load_rec = [&](VRec &d, int s, int g) {
+        const int l = m + min(s, nseg - 1) * DSEG + 16 * g + (lane & 15);
+        d.b = qbm[l + 1];
+        const double2 dd = qdm[min(l, lmax + 1)];
+        d.dl = dd.x;
+        d.dg = dd.y;
+        const int lc = min(l, lmax);
+        const double4 r0 = rec[2 * (size_t)lc], r1 = rec[2 * (size_t)lc + 1];
+        d.R[0] = r0.x;
+        d.R[1] = r0.y;
+        d.R[2] = r0.z;
+        d.R[3] = r0.w;
+        d.R[4] = r1.x;
+        d.R[5] = r1.y;
+        d.R[6] = r1.z;
+        d.R[7] = r1.w;
+    };
+    VRec nr;
+    if (w < nseg) load_rec(nr, w, 0);
+    for (int s = w; s < nseg; s += 4) {
+        const int l0 = m + s * DSEG;
+        if (l0 > ltr) break;
+        const int gs = segb + s;
+        const int pf = as_const(a.pfirst)[gs], pfa = as_const(a.pfast)[gs];
+        const int lstop = min(l0 + DSEG, ltr + 1);
+        if (pb1 <= pf) {
+            if (s + 4 < nseg) load_rec(nr, s + 4, 0);
+            continue;
+        }
+        double prev[K], cur[K];
+        int kk[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int p = pb0 + lane + 64 * j;
+            const bool act = p < np;
+            const size_t q = (size_t)gs * np + (act ? p : 0);
+            prev[j] = act ? a.seedq0[q] : 0.0;
+            cur[j] = act ? a.seedq1[q] : 0.0;
+            kk[j] = act ? a.seedk[q] : 0;
+        }
+        const bool fast = pb0 >= pfa;
+        double cm[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) cm[j] = kk[j] == 0 ? 1.0 : 0.0;
+        for (int g = 0; g < 4; ++g) {
+            const int l0g = l0 + 16 * g;
+            const VRec r = nr;
+            // the next group: this segment's next, or the first of the wave's next segment
+            if (g < 3 && l0g + 16 < lstop) load_rec(nr, s, g + 1);
+            else if (s + 4 < nseg) load_rec(nr, s + 4, 0);
+            const bool full = fast ? vdgroup_from<K, true, 0>(r, qcm, l0g, lstop - l0g, x, prev, cur, kk, cm, acc)
+                                   : vdgroup_from<K, false, 0>(r, qcm, l0g, lstop - l0g, x, prev, cur, kk, cm, acc);
+            if (!full || l0g + 16 >= lstop) break;
+        }
+    }
+    vsynth_out<K>(a, acc, red, w, lane, pb0, im, f);
+}
+
 // Prologue of the vector synthesis: per (order, field) and l >= m the record
 // (S', T', c_l S', c_l T') with S' = conv(l, m) S [/ l(l+1)], T' likewise.
 template <typename IT>
@@ -952,12 +1171,129 @@ __global__ __launch_bounds__(64 * ANW_WAVES) void k_anal_w(const LegArgs a) {
 
 // Vector analysis, one wavefront per item (segment, pair block of 512,
 // field) with up to 8 chains per lane, so no coefficient has two writers.
-// Every 4 degrees the lanes' 32 partial sums (4 degrees x SD, SQ, TQ, TD x
-// re/im) go through a wave-private LDS tile; lane (o = lane & 31, h = lane >> 5)
-// adds 32 of them and one shuffle completes output o = 8 step + quantity; a
-// second shuffle pairs each quantity with the one S or T needs beside it.
+// Every 2 degrees the lanes' 16 partial sums (2 degrees x SD, SQ, TQ, TD x
+// re/im) go through a wave-private LDS tile; lane (o = lane & 15, q = lane >> 4)
+// adds 16 of them and two shuffles complete output o = 8 step + quantity; a
+// third shuffle pairs each quantity with the one S or T needs beside it.
 constexpr int AVW_WAVES = 4;
-constexpr int AVW_RS = 65;  // LDS row stride (doubles) of the reduction tile
+constexpr int AVW_RS = 65;     // LDS row stride (doubles) of the reduction tile
+constexpr int AVW_STEPS = 2;   // degrees per reduction (16 partial sums per lane)
+
+struct AVRec {
+    double b, dl, dg;  // beta_{l+1}, l c_l, g_l c_{l-1}
+};
+
+// One degree of the vector analysis for K chains; N: the degree's lane in its
+// record group (DPP broadcast source), PAR: its parity (l - m).
+template <int K, int N, int PAR, bool NOP, bool FAST>
+__device__ __forceinline__ void avw_step(const AVRec &r, const double *x, double *prev, double *cur, const int *kk,
+                                         const double (*gt)[4], const double (*gp)[4], double *q8) {
+    // even l - m: SD <- Gt-, SQ <- Gp+, TQ <- Gt+, TD <- Gp-; odd: the other signs
+    constexpr int e = PAR ? 0 : 2, od = PAR ? 2 : 0;
+    double xc[K], D[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        xc[j] = x[j] * cur[j];
+        D[j] = 0.0;
+    }
+    if (NOP) asm volatile("s_nop 4");
+#pragma unroll
+    for (int j = 0; j < K; ++j) SHTK_FMAC_BC(D[j], r.dl, xc[j], N, "");
+#pragma unroll
+    for (int j = 0; j < K; ++j) SHTK_FMAC_BC(D[j], r.dg, prev[j], N, "-");
+    double sdr = 0, sdi = 0, sqr = 0, sqi = 0, tqr = 0, tqi = 0, tdr = 0, tdi = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        double Dj = D[j], Q = cur[j];
+        if (!FAST) {
+            const double cmj = kk[j] == 0 ? 1.0 : 0.0;
+            Dj *= cmj;
+            Q *= cmj;
+        }
+        sdr = fma(Dj, gt[j][e], sdr);
+        sdi = fma(Dj, gt[j][e + 1], sdi);
+        sqr = fma(Q, gp[j][od], sqr);
+        sqi = fma(Q, gp[j][od + 1], sqi);
+        tqr = fma(Q, gt[j][od], tqr);
+        tqi = fma(Q, gt[j][od + 1], tqi);
+        tdr = fma(Dj, gp[j][e], tdr);
+        tdi = fma(Dj, gp[j][e + 1], tdi);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) SHTK_FMAC_BC(xc[j], r.b, prev[j], N, "-");
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        prev[j] = cur[j];
+        cur[j] = xc[j];
+    }
+    q8[0] = sdr;
+    q8[1] = sdi;
+    q8[2] = sqr;
+    q8[3] = sqi;
+    q8[4] = tqr;
+    q8[5] = tqi;
+    q8[6] = tdr;
+    q8[7] = tdi;
+}
+
+// Degrees c16 + N, c16 + N + 1 (N even) of a record group, their reduction and
+// stores, then the next pair; stops at nst.  Scaled chains are checked every 8
+// degrees with c_l of the last degree.
+template <int K, bool FAST, int N, typename OT>
+__device__ __forceinline__ void avw_pairs(const LegArgs &a, const Item &it, const AVRec &r, cptr<double> qcm,
+                                          double *red, OT *cs, OT *ct, int lane, int c16, int nst, const double *x,
+                                          double *prev, double *cur, int *kk, const double (*gt)[4],
+                                          const double (*gp)[4], double gm, double mm) {
+    if constexpr (N < 16) {
+        const int c0 = c16 + N;
+        if (c0 >= nst) return;
+        double q8[8];
+        avw_step<K, N, 0, N == 0 || (!FAST && N == 8), FAST>(r, x, prev, cur, kk, gt, gp, q8);
+        double *rw = red + lane;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) rw[q * AVW_RS] = q8[q];
+        avw_step<K, N + 1, 1, false, FAST>(r, x, prev, cur, kk, gt, gp, q8);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) rw[(8 + q) * AVW_RS] = q8[q];
+        if (!FAST && ((N + 1) & 7) == 7) {
+            const double cl = qcm[it.l0 + c0 + 1];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
+                    cur[j] *= kDown;
+                    prev[j] *= kDown;
+                    kk[j] -= 1;
+                }
+            }
+        }
+        // lane (o, qg) adds 16 of the 64 partials of output o = 8 step + quantity
+        const int o = lane & 15, qg = lane >> 4, kq = lane & 7, so = o >> 3;
+        __builtin_amdgcn_wave_barrier();
+        double sum = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sum += red[o * AVW_RS + 16 * qg + i];
+        __builtin_amdgcn_wave_barrier();
+        sum += __shfl_xor(sum, 16);
+        sum += __shfl_xor(sum, 32);
+        // S = SD - i m c SQ, T = i m c TQ + TD: quantity k pairs with k ^ 3
+        const double other = __shfl_xor(sum, 3);
+        const int t = c0 + so;
+        if (qg == 0 && t < nst && (kq < 2 || kq >= 6)) {
+            const int l = it.l0 + t;
+            double sc = gm;
+            if (a.ll1_mode) sc *= l > 0 ? 1.0 / ((double)l * (l + 1)) : 0.0;
+            const double mc = mm * qcm[l];
+            const double sg = (kq == 0 || kq == 7) ? 1.0 : -1.0;
+            const double v = sc * fma(sg * mc, other, sum);
+            OT *dst = (kq < 2 ? cs : ct) + 2 * (size_t)l + (kq & 1);
+            if (a.atomic)
+                atomicAdd(dst, (OT)v);
+            else
+                *dst = (OT)v;
+        }
+        avw_pairs<K, FAST, N + 2>(a, it, r, qcm, red, cs, ct, lane, c16, nst, x, prev, cur, kk, gt, gp, gm, mm);
+    }
+}
 
 template <int K, bool FAST, typename OT>
 __device__ __forceinline__ void anal_vw_run(const LegArgs &a, const Item &it, double *red, OT *cs, OT *ct,
@@ -1002,85 +1338,24 @@ __device__ __forceinline__ void anal_vw_run(const LegArgs &a, const Item &it, do
     const double mm = (double)it.m;
     const int nst = it.lstop - it.l0;
     const long long co = as_const(a.coff)[it.im] - it.m;
-    const cptr<double> qbm = as_const(a.qb) + co, qcm = as_const(a.qcc) + co;
-    const cptr<double> qdm = as_const((const double *)a.qd) + 2 * co;
-    const int o = lane & 31, hh = lane >> 5, kq = lane & 7, so = o >> 3;
-    for (int c0 = 0; c0 < nst; c0 += 4) {
-        const int lc = __builtin_amdgcn_readfirstlane(it.l0 + c0);
-#pragma unroll
-        for (int st = 0; st < 4; ++st) {
-            // beta_{l+1}, (l c_l, g_l c_{l-1}); rows past the segment are never stored
-            const double beta = qbm[lc + st + 1];
-            const int lq = 2 * min(lc + st, a.lmax + 1);
-            const double dl = qdm[lq], dg = qdm[lq + 1];
-            // even l - m: SD <- Gt-, SQ <- Gp+, TQ <- Gt+, TD <- Gp-; odd: the other signs
-            const int e = (st & 1) ? 0 : 2, od = (st & 1) ? 2 : 0;
-            double sdr = 0, sdi = 0, sqr = 0, sqi = 0, tqr = 0, tqi = 0, tdr = 0, tdi = 0;
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                const double xc = x[j] * cur[j];
-                double D = fma(dl, xc, -(dg * prev[j]));
-                double Q = cur[j];
-                if (!FAST) {
-                    const double cmj = kk[j] == 0 ? 1.0 : 0.0;
-                    D *= cmj;
-                    Q *= cmj;
-                }
-                sdr = fma(D, gt[j][e], sdr);
-                sdi = fma(D, gt[j][e + 1], sdi);
-                sqr = fma(Q, gp[j][od], sqr);
-                sqi = fma(Q, gp[j][od + 1], sqi);
-                tqr = fma(Q, gt[j][od], tqr);
-                tqi = fma(Q, gt[j][od + 1], tqi);
-                tdr = fma(D, gp[j][e], tdr);
-                tdi = fma(D, gp[j][e + 1], tdi);
-                const double nx = fma(-beta, prev[j], xc);
-                prev[j] = cur[j];
-                cur[j] = nx;
-            }
-            double *rw = red + (8 * st) * AVW_RS + lane;
-            rw[0 * AVW_RS] = sdr;
-            rw[1 * AVW_RS] = sdi;
-            rw[2 * AVW_RS] = sqr;
-            rw[3 * AVW_RS] = sqi;
-            rw[4 * AVW_RS] = tqr;
-            rw[5 * AVW_RS] = tqi;
-            rw[6 * AVW_RS] = tdr;
-            rw[7 * AVW_RS] = tdi;
-        }
-        if (!FAST && (c0 & 7) == 4) {
-            const double cl = qcm[lc + 3];
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
-                    cur[j] *= kDown;
-                    prev[j] *= kDown;
-                    kk[j] -= 1;
-                }
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        double sum = 0.0;
-#pragma unroll
-        for (int i = 0; i < 32; ++i) sum += red[o * AVW_RS + 32 * hh + i];
-        __builtin_amdgcn_wave_barrier();
-        sum += __shfl_xor(sum, 32);
-        // S = SD - i m c SQ, T = i m c TQ + TD: quantity k pairs with k ^ 3
-        const double other = __shfl_xor(sum, 3);
-        const int t = c0 + so;
-        if (hh == 0 && t < nst && (kq < 2 || kq >= 6)) {
-            const int l = it.l0 + t;
-            double sc = gm;
-            if (a.ll1_mode) sc *= l > 0 ? 1.0 / ((double)l * (l + 1)) : 0.0;
-            const double mc = mm * qcm[l];
-            const double sg = (kq == 0 || kq == 7) ? 1.0 : -1.0;
-            const double v = sc * fma(sg * mc, other, sum);
-            OT *dst = (kq < 2 ? cs : ct) + 2 * (size_t)l + (kq & 1);
-            if (a.atomic)
-                atomicAdd(dst, (OT)v);
-            else
-                *dst = (OT)v;
-        }
+    const cptr<double> qcm = as_const(a.qcc) + co;
+    // per-degree records in VGPRs, broadcast by DPP (no scalar loads in the loop):
+    // lane 16 r + n holds (beta_{l+1}, l c_l, g_l c_{l-1}) of degree l0 + c16 + n
+    const double *qbv = a.qb + co;                       // padded past lmax + 1
+    const double2 *qdv = a.qd + co;                      // entries l = m .. lmax + 1
+    AVRec nr;
+    auto load_rec = [&](AVRec &d, int c16) {
+        const int l = min(it.l0 + c16 + (lane & 15), a.lmax + 1);
+        d.b = qbv[l + 1];
+        const double2 dd = qdv[l];
+        d.dl = dd.x;
+        d.dg = dd.y;
+    };
+    load_rec(nr, 0);
+    for (int c16 = 0; c16 < nst; c16 += 16) {
+        const AVRec r = nr;
+        if (c16 + 16 < nst) load_rec(nr, c16 + 16);
+        avw_pairs<K, FAST, 0>(a, it, r, qcm, red, cs, ct, lane, c16, nst, x, prev, cur, kk, gt, gp, gm, mm);
     }
 }
 
@@ -1093,9 +1368,11 @@ __device__ __forceinline__ void anal_vw_dispatch(const LegArgs &a, const Item &i
         anal_vw_run<K, false>(a, it, red, cs, ct, lane);
 }
 
-template <typename OT>
-__global__ __launch_bounds__(64 * AVW_WAVES) void k_anal_vw(const LegArgs a) {
-    __shared__ double red_all[AVW_WAVES][32 * AVW_RS];
+// KMAX: the most chains any item of the launch needs (np / 64 up to 8); a
+// kernel compiled for fewer chains allocates fewer VGPRs and runs more waves.
+template <typename OT, int KMAX>
+__global__ __launch_bounds__(64 * AVW_WAVES, KMAX <= 4 ? 3 : 1) void k_anal_vw(const LegArgs a) {
+    __shared__ double red_all[AVW_WAVES][8 * AVW_STEPS * AVW_RS];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int item = blockIdx.x * AVW_WAVES + w;
     if (item >= a.gx * a.gy * a.gz) return;
@@ -1109,8 +1386,8 @@ __global__ __launch_bounds__(64 * AVW_WAVES) void k_anal_vw(const LegArgs a) {
     if (dead) return;
     double *red = red_all[w];
     const int keff = (it.pb1 - it.pstart + 63) / 64;
-    if (keff <= 2) anal_vw_dispatch<2>(a, it, red, cs, ct, lane);
-    else if (keff <= 4) anal_vw_dispatch<4>(a, it, red, cs, ct, lane);
+    if (KMAX <= 2 || keff <= 2) anal_vw_dispatch<2>(a, it, red, cs, ct, lane);
+    else if (KMAX <= 4 || keff <= 4) anal_vw_dispatch<4>(a, it, red, cs, ct, lane);
     else anal_vw_dispatch<8>(a, it, red, cs, ct, lane);
 }
 
@@ -1210,7 +1487,10 @@ static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
     const dim3 grid(grid_for_units(a.xcd, a.gx, a.gy * a.gz));
     if (vector) {
         // inputs are the FP64 prologue records of k_vspec_prep
-        hipLaunchKernelGGL((k_synth_v<K>), grid, dim3(256), 0, st, a);
+        if (a.seg == DSEG)
+            hipLaunchKernelGGL((k_synth_vd<K>), grid, dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_synth_v<K>), grid, dim3(256), 0, st, a);
     } else {
         if (a.seg == DSEG)
             hipLaunchKernelGGL((k_synth_d<K>), grid, dim3(64 * DSW), 0, st, a);
@@ -1223,7 +1503,10 @@ static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
 
 hipError_t launch_leg_synth(const LegArgs &a, int nfields, int vector, int chains, hipStream_t st) {
     if (a.seg > SEG_MAX || (a.seg & 7)) return hipErrorInvalidValue;
-    if (vector) launch_synth_k<VS_CHAINS>(a, nfields, 1, st);
+    if (vector) {
+        if (chains == 1) launch_synth_k<1>(a, nfields, 1, st);
+        else launch_synth_k<VS_CHAINS>(a, nfields, 1, st);
+    }
     else if (chains <= 1) launch_synth_k<1>(a, nfields, 0, st);
     else if (chains == 2) launch_synth_k<2>(a, nfields, 0, st);
     else launch_synth_k<4>(a, nfields, 0, st);
@@ -1240,10 +1523,17 @@ hipError_t launch_leg_anal(const LegArgs &a0, int nfields, int vector, hipStream
     const dim3 grid((unsigned)a.gx * a.gy * nfields);
     if (vector) {
         const dim3 gw((unsigned)(((long long)a.gx * a.gy * nfields + AVW_WAVES - 1) / AVW_WAVES));
-        if (a.io_f32)
-            hipLaunchKernelGGL(k_anal_vw<float>, gw, dim3(64 * AVW_WAVES), 0, st, a);
-        else
-            hipLaunchKernelGGL(k_anal_vw<double>, gw, dim3(64 * AVW_WAVES), 0, st, a);
+        const int kmax = (std::min(a.np, pblk) + 63) / 64;
+        if (kmax <= 2) {
+            if (a.io_f32) hipLaunchKernelGGL((k_anal_vw<float, 2>), gw, dim3(64 * AVW_WAVES), 0, st, a);
+            else hipLaunchKernelGGL((k_anal_vw<double, 2>), gw, dim3(64 * AVW_WAVES), 0, st, a);
+        } else if (kmax <= 4) {
+            if (a.io_f32) hipLaunchKernelGGL((k_anal_vw<float, 4>), gw, dim3(64 * AVW_WAVES), 0, st, a);
+            else hipLaunchKernelGGL((k_anal_vw<double, 4>), gw, dim3(64 * AVW_WAVES), 0, st, a);
+        } else {
+            if (a.io_f32) hipLaunchKernelGGL((k_anal_vw<float, 8>), gw, dim3(64 * AVW_WAVES), 0, st, a);
+            else hipLaunchKernelGGL((k_anal_vw<double, 8>), gw, dim3(64 * AVW_WAVES), 0, st, a);
+        }
     } else {
         const dim3 gw((unsigned)(((long long)a.gx * a.gy * nfields + ANW_WAVES - 1) / ANW_WAVES));
         if (a.io_f32)

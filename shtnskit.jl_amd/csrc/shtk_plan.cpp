@@ -831,7 +831,7 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
                 "FP32 synthesis launch");
     } else {
         HIP_TRY(timed(p, j.leg_name, st,
-                      [&] { return shtk::launch_leg_synth(a, nf, j.vector, p->synth_chains, st); }),
+                      [&] { return shtk::launch_leg_synth(a, nf, j.vector, j.vector ? p->vsynth_chains : p->synth_chains, st); }),
                 "Legendre synthesis launch");
         if (j.vector && j.pole_fix == 1 && j.ltr >= 1) {
             shtk::MlArgs z = zonal_args(p, j.ltr, a.spec_fstride);
@@ -991,6 +991,7 @@ int ensure_vshadow(shtk_plan *p) {
     s->seg = p->seg;
     s->xcd_map = p->xcd_map;
     s->synth_chains = p->synth_chains;
+    s->vsynth_chains = p->vsynth_chains;
     s->fft_tile = p->fft_tile;
     s->oz_run_len = p->oz_run_len;
     s->oz_nh_force = p->oz_nh_force;
@@ -1062,7 +1063,7 @@ int run_vsplit_to_grid(shtk_plan *p, const Job &j, const void *S, const void *T,
     a.wlat = nullptr;  // sin(theta) V, as the split result
     a.g = j.g;
     a.nm_run = nm_lo;
-    HIP_TRY(timed(p, "leg_synth_sphtor_low", st, [&] { return shtk::launch_leg_synth(a, nf, 1, p->synth_chains, st); }),
+    HIP_TRY(timed(p, "leg_synth_sphtor_low", st, [&] { return shtk::launch_leg_synth(a, nf, 1, p->vsynth_chains, st); }),
             "Legendre synthesis launch");
     // grids with poles: order-1 pole rows from the limits (the phi transform divides
     // those rows by 1), Robert form: every pole row zero
@@ -1772,6 +1773,7 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     // latitude pairs per lane of the scalar synthesis: 2 for the 64-degree VGPR-record
     // kernel (4 waves per SIMD), 4 for the scalar-load kernel of other segment lengths
     p->synth_chains = env_int("SHTK_SYNTH_CHAINS", p->seg == 64 ? 2 : 4);
+    p->vsynth_chains = env_int("SHTK_VSYNTH_CHAINS", 1) == 2 ? 2 : 1;
     p->fft_tile = env_int("SHTK_FFT_PAIRS", 0);
     p->oz_run_len = std::max(1, env_int("SHTK_OZ_RUN", 8));
     {

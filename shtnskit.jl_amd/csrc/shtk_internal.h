@@ -69,6 +69,7 @@ struct LegArgs {
     int gx, gy, gz;         // work items form a (gx, gy, gz = nfields) box, x fastest
     int xcd;                // synthesis: 1 keeps the pair blocks of an order on one XCD
     int nm_run;             // synthesis: only orders im < nm_run (0: all)
+    int skip_m0;            // vector: order 0 is computed by the zonal kernel (shtk_ml.hip)
 };
 
 // Work distribution over the 8 XCDs (each with its own 4 MB L2).  The
@@ -222,6 +223,11 @@ struct shtk_plan {
     hipEvent_t order_ev = nullptr;
     hipStream_t order_st = nullptr;
     bool order_used = false;
+    // Second stream of the plan (made on first use, outside graph capture) and
+    // its fork/join events: the order-0 rows of the vector transforms run on it
+    // beside the main Legendre kernel.
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
     // Incremented whenever ws or oz_ws is reallocated: captured graphs that hold
     // the old addresses are stale once it changes.
     unsigned long long ws_gen = 0;
@@ -270,7 +276,7 @@ struct OzArgs {
     int seg, nseg_total, nks;
     int nh, npb;            // analysis pair blocks: nh halves of 256 pairs each, npb blocks
     const double *wlat;
-    double *spec_out;       // FP64 canonical coefficients, accumulated (pre-zeroed)
+    double *spec_out;       // FP64 canonical coefficients (pre-zeroed when npb > 1: accumulated)
     const int4 *run;        // [nrun] (order, first segment, segment count, 0)
     int nrun;
     int4 *gdig;             // [nfg][nm][nks][2][6][2][64] B operands (pairs as k)
@@ -278,6 +284,35 @@ struct OzArgs {
     int *eg;                // [nfg][nm][npb][2][32] column exponents per pair block
     float *gf32;            // FP32 contraction: analysis field operand (k_f32_gprep layout)
 };
+
+// Analysis with one pair block (npb == 1) stores every coefficient once and
+// the output is not cleared beforehand: the work item of (run, field group)
+// also stores the zeros of the coefficients no contraction of it writes -- the
+// run's leading segments in which no pair is live, degrees past ltr, and, on
+// the run that starts its order, l < m and the orders up to the next multiple
+// of mres.  lr_live: first degree of the first live segment (the run's end if
+// none).  Every thread of the block takes part.
+__device__ inline void oz_zero_holes(const OzArgs &a, int im, int s0, int nsr, int lr_live, int fg) {
+    const int m = im * a.mres;
+    const int lr0 = m + s0 * a.seg, lr1 = min(m + (s0 + nsr) * a.seg, a.lmax + 1);
+    const int ll = min(lr_live, lr1), le = max(ll, min(lr1, a.ltr + 1));
+    const int nfl = min(OZ_FIELDS, a.nf - fg * OZ_FIELDS);
+    double2 *base = (double2 *)a.spec_out + (size_t)fg * OZ_FIELDS * a.spec_fstride;
+    auto zr = [&](int mm, int la, int lb) {  // degrees [la, lb) of order mm, every field of the group
+        const int n = lb - la;
+        if (n <= 0) return;
+        for (int i = threadIdx.x; i < nfl * n; i += blockDim.x) {
+            const int fi = i / n, l = la + (i - fi * n);
+            base[(size_t)fi * a.spec_fstride + (size_t)(a.lmax + 1) * mm + l] = make_double2(0.0, 0.0);
+        }
+    };
+    zr(m, lr0, ll);
+    zr(m, le, lr1);
+    if (s0 == 0) {
+        zr(m, 0, m);
+        for (int c = 1; c < a.mres && m + c <= a.mmax; ++c) zr(m + c, 0, a.lmax + 1);
+    }
+}
 
 struct OzSeedArgs {
     int lmax, mres, np, nm;

@@ -561,6 +561,7 @@ __global__ __launch_bounds__(256) void k_synth_v(const LegArgs a) {
     int b, unit;
     if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
     const int im = unit % a.gy, f = unit / a.gy;
+    if (a.skip_m0 && im == 0) return;  // written by the zonal kernel
     const int m = im * a.mres;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -781,6 +782,7 @@ __global__ __launch_bounds__(256) void k_synth_vd(const LegArgs a) {
     int b, unit;
     if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
     const int im = unit % a.gy, f = unit / a.gy;
+    if (a.skip_m0 && im == 0) return;  // written by the zonal kernel
     const int m = im * a.mres;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1383,7 +1385,7 @@ __global__ __launch_bounds__(64 * AVW_WAVES, KMAX <= 4 ? 3 : 1) void k_anal_vw(c
     const bool dead = it.lstop <= it.l0 || it.pstart >= it.pb1;
     zero_fill(a, it, cs, dead, lane);
     zero_fill(a, it, ct, dead, lane);
-    if (dead) return;
+    if (dead || (a.skip_m0 && it.im == 0)) return;  // order 0: written by the zonal kernel
     double *red = red_all[w];
     const int keff = (it.pb1 - it.pstart + 63) / 64;
     if (KMAX <= 2 || keff <= 2) anal_vw_dispatch<2>(a, it, red, cs, ct, lane);

@@ -260,6 +260,7 @@ __device__ __forceinline__ void anal_f32_body(const OzArgs &a, const double *__r
 
     int sr0 = 0;
     while (sr0 < nsr && pbase + 256 * NH - 1 < pfc[gs0 + sr0]) ++sr0;
+    if (!atom) oz_zero_holes(a, im, s0, nsr, m + (s0 + sr0) * a.seg, fg);
     if (sr0 == nsr) return;
     double prev, cur;
     int kk;

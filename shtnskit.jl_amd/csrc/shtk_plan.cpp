@@ -250,6 +250,9 @@ void free_plan(shtk_plan *p) {
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (p->order_ev) (void)hipEventDestroy(p->order_ev);
+    if (p->side_fork) (void)hipEventDestroy(p->side_fork);
+    if (p->side_join) (void)hipEventDestroy(p->side_join);
+    if (p->side) (void)hipStreamDestroy(p->side);
     free_plan(p->vshadow);
     delete p;
 }
@@ -348,6 +351,46 @@ hipError_t timed(shtk_plan *p, const char *name, hipStream_t st, F &&launch) {
     if (a && b) p->pending.push_back({name, a, b});
     return e;
 }
+
+// Fork/join of the plan's second stream: fork() returns a stream on which work
+// runs after everything enqueued on st so far (the side stream, or st itself
+// when it cannot be made, e.g. st is being captured before the side stream
+// exists); join() makes st wait for what was enqueued there.  Under graph
+// capture the event record/wait pair becomes a fork and a join of the graph.
+struct SideStream {
+    shtk_plan *p;
+    hipStream_t st, s;
+    SideStream(shtk_plan *p_, hipStream_t st_) : p(p_), st(st_), s(st_) {
+        if (!p->side) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+            hipStream_t ns = nullptr;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (hipStreamCreateWithFlags(&ns, hipStreamNonBlocking) != hipSuccess) return;
+            if (hipEventCreateWithFlags(&e0, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&e1, hipEventDisableTiming) != hipSuccess) {
+                if (e0) (void)hipEventDestroy(e0);
+                (void)hipStreamDestroy(ns);
+                return;
+            }
+            p->side = ns;
+            p->side_fork = e0;
+            p->side_join = e1;
+        }
+        if (hipEventRecord(p->side_fork, st) != hipSuccess) return;
+        if (hipStreamWaitEvent(p->side, p->side_fork, 0) != hipSuccess) return;
+        s = p->side;
+    }
+    bool forked() const { return s != st; }
+    hipError_t join() {
+        if (s == st) return hipSuccess;
+        hipError_t e = hipEventRecord(p->side_join, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, p->side_join, 0);
+        s = st;
+        return e;
+    }
+    ~SideStream() { (void)join(); }
+};
 
 // Calls on one plan share its workspaces.  The host mutex orders their
 // enqueueing; on the device, a call issued on another stream than the previous
@@ -830,10 +873,11 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
         HIP_TRY(timed(p, "f32_synth", st, [&] { return shtk::launch_f32_synth(o, st); }),
                 "FP32 synthesis launch");
     } else {
-        HIP_TRY(timed(p, j.leg_name, st,
-                      [&] { return shtk::launch_leg_synth(a, nf, j.vector, j.vector ? p->vsynth_chains : p->synth_chains, st); }),
-                "Legendre synthesis launch");
-        if (j.vector && j.pole_fix == 1 && j.ltr >= 1) {
+        // vector: the order-0 rows come from the zonal kernel, on the side stream
+        // beside the main kernel (which skips order 0); disjoint output columns
+        const bool zonal = j.vector && j.pole_fix == 1 && j.ltr >= 1;
+        a.skip_m0 = zonal;
+        if (zonal) {
             shtk::MlArgs z = zonal_args(p, j.ltr, a.spec_fstride);
             z.io_f32 = dtype == SHTK_F32;
             z.conv = p->canonical ? nullptr : p->convscale;  // order 0: index l
@@ -842,8 +886,17 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
             z.in1 = s1;
             z.out0 = F0;
             z.out1 = F1;
-            HIP_TRY(timed(p, "leg_synth_sphtor_m0", st, [&] { return shtk::launch_ml_synth(z, nf, st); }),
+            SideStream side(p, st);
+            HIP_TRY(timed(p, "leg_synth_sphtor_m0", side.s, [&] { return shtk::launch_ml_synth(z, nf, side.s); }),
                     "order-0 rows launch");
+            HIP_TRY(timed(p, j.leg_name, st,
+                          [&] { return shtk::launch_leg_synth(a, nf, 1, p->vsynth_chains, st); }),
+                    "Legendre synthesis launch");
+            HIP_TRY(side.join(), "order-0 rows join");
+        } else {
+            HIP_TRY(timed(p, j.leg_name, st,
+                          [&] { return shtk::launch_leg_synth(a, nf, j.vector, j.vector ? p->vsynth_chains : p->synth_chains, st); }),
+                    "Legendre synthesis launch");
         }
         const bool zero_rows = j.pole_fix == 1 && p->d.robert_form;  // Robert rows are sin(theta) V
         if (j.pole_fix && p->poles && (zero_rows || (p->d.mres == 1 && p->d.mmax >= 1)))
@@ -913,7 +966,8 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
         const bool direct = p->canonical && dtype == SHTK_F64;
         RC_TRY(ensure_oz_ws(p, f32_ws_bytes(p, nf, !direct)));
         double *acc = direct ? (double *)s0 : (double *)((char *)p->oz_ws + f32_gbytes(p, nf));
-        HIP_TRY(shtk::launch_zero(acc, (size_t)nf * a.spec_fstride * 16, st), "output clear");
+        if (oz_npb(p) > 1)  // several pair blocks accumulate; one block stores every coefficient
+            HIP_TRY(shtk::launch_zero(acc, (size_t)nf * a.spec_fstride * 16, st), "output clear");
         const shtk::OzArgs o = f32_anal_args(p, F0, acc, nf, j.ltr, j.wlat, j.g);
         HIP_TRY(timed(p, "f32_anal", st, [&] { return shtk::launch_f32_anal(o, st); }), "FP32 analysis launch");
         if (!direct)
@@ -928,7 +982,8 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
         RC_TRY(ensure_oz_ws(p, oz_ws_bytes(p, nf, !direct)));
         const int nfg = (nf + shtk::OZ_FIELDS - 1) / shtk::OZ_FIELDS;
         double *acc = direct ? (double *)s0 : (double *)((char *)p->oz_ws + oz_ws_layout(p, nfg, true).total);
-        HIP_TRY(shtk::launch_zero(acc, (size_t)nf * a.spec_fstride * 16, st), "output clear");
+        if (oz_npb(p) > 1)  // several pair blocks accumulate; one block stores every coefficient
+            HIP_TRY(shtk::launch_zero(acc, (size_t)nf * a.spec_fstride * 16, st), "output clear");
         const shtk::OzArgs o = oz_anal_args(p, F0, acc, nf, j.ltr, j.wlat, j.g);
         HIP_TRY(timed(p, "oz_anal", st, [&] { return shtk::launch_oz_anal(o, st); }), "split analysis launch");
         if (!direct)
@@ -945,10 +1000,12 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
         for (int q = 0; q <= j.vector; ++q)
             HIP_TRY(shtk::launch_zero(outs[q], bytes, st), "output clear");
     }
-    HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_leg_anal(a, nf, j.vector, st); }),
-            "Legendre analysis launch");
-    if (j.vector && j.pole_fix == 1 && j.ltr >= 1) {
-        // the convention map of order 0 follows with the rest (non-canonical plans)
+    const bool zonal = j.vector && j.pole_fix == 1 && j.ltr >= 1;
+    if (zonal) {
+        // order 0 from the zonal kernel on the side stream, beside the main kernel
+        // (which skips it); the convention map of order 0 follows with the rest
+        // (non-canonical plans)
+        a.skip_m0 = 1;
         shtk::MlArgs z = zonal_args(p, j.ltr, a.spec_fstride);
         z.io_f32 = a.io_f32;
         z.g = j.g;
@@ -957,8 +1014,15 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
         z.out0 = a.out0;
         z.out1 = a.out1;
         z.acc = (double *)(ws + L.zonal_off);
-        HIP_TRY(timed(p, "leg_anal_sphtor_m0", st, [&] { return shtk::launch_ml_anal(z, nf, st); }),
+        SideStream side(p, st);
+        HIP_TRY(timed(p, "leg_anal_sphtor_m0", side.s, [&] { return shtk::launch_ml_anal(z, nf, side.s); }),
                 "order-0 rows launch");
+        HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_leg_anal(a, nf, 1, st); }),
+                "Legendre analysis launch");
+        HIP_TRY(side.join(), "order-0 rows join");
+    } else {
+        HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_leg_anal(a, nf, j.vector, st); }),
+                "Legendre analysis launch");
     }
     if (j.pole_fix && p->poles && p->d.mres == 1 && p->d.mmax >= 1)
         HIP_TRY(shtk::launch_vpole_anal(a, nf, j.pole_fix == 2 ? 1.0 : p->hw[0] * p->scale_phi, nullptr, st),
@@ -1113,7 +1177,7 @@ int run_vsplit_to_spec(shtk_plan *p, const Job &j, const void *Vt, const void *V
     double *G = (double *)(ws + 2 * L.f_bytes);
     const long long out_fs = (long long)(p->d.lmax + 1) * (p->d.mmax + 1);
     const long long g_fs = (long long)(p->d.lmax + 2) * (p->d.mmax + 1);
-    HIP_TRY(shtk::launch_zero(G, (size_t)2 * nf * g_fs * 16, st), "analysis sums clear");
+    if (oz_npb(s) > 1) HIP_TRY(shtk::launch_zero(G, (size_t)2 * nf * g_fs * 16, st), "analysis sums clear");
     if (j.prec == SHTK_PREC_FP32) {
         const shtk::OzArgs o = f32_anal_args(s, F0, G, 2 * nf, j.ltr + 1, j.wlat, j.g);
         HIP_TRY(timed(p, j.leg_name, st, [&] { return shtk::launch_f32_anal(o, st); }), "FP32 analysis launch");
@@ -1260,7 +1324,8 @@ int batch_split(shtk_batch *b, int dir, const void *const *in, void *const *out,
         const bool direct = p->canonical && dtype == SHTK_F64;
         double *acc = direct ? (double *)out[gi] : (double *)((char *)p->oz_ws + oz_ws_layout(p, nfg, true).total);
         accs[gi] = acc;
-        HIP_TRY(shtk::launch_zero(acc, (size_t)nf * (p->d.lmax + 1) * (p->d.mmax + 1) * 16, st), "output clear");
+        if (oz_npb(p) > 1)
+            HIP_TRY(shtk::launch_zero(acc, (size_t)nf * (p->d.lmax + 1) * (p->d.mmax + 1) * 16, st), "output clear");
         const shtk::OzArgs o = oz_anal_args(p, (double2 *)p->ws, acc, nf, p->d.lmax, p->wscaled, 1.0);
         // one launch pair per pair-block size: [0] two halves, [1] one half
         const int k = o.nh == 2 ? 0 : 1;

@@ -1081,6 +1081,18 @@ __device__ __forceinline__ void anal_w_run(const LegArgs &a, const Item &it, dou
     const long long co = as_const(a.coff)[it.im] - it.m;
     const cptr<double> qcm = as_const(a.qcc) + co;
     const int o = lane & 15, qg = lane >> 4;
+    // Checked path: ke = -4096 x scale count, so ldexp(cur, ke) is the lane's
+    // term -- the value itself when live, exactly 0 while scaled (|cur| < 2^1024)
+    // -- in one instruction; slow has bit j set while chain j holds a scaled
+    // lane, and only those chains take the rescale checks.
+    unsigned slow = 0;
+    if (!FAST) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            kk[j] *= -4096;
+            if (__any(kk[j] < 0)) slow |= 1u << j;
+        }
+    }
     // beta_{l+1} of the item's degrees, staged in the wave's LDS slice and read
     // as broadcasts (no scalar loads, and so no scalar-load waits, in the loop)
     double *bl = red + 16 * ANW_RS;
@@ -1096,7 +1108,7 @@ __device__ __forceinline__ void anal_w_run(const LegArgs &a, const Item &it, dou
             double pr = 0.0, pi = 0.0;
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                const double v = FAST ? cur[j] : (kk[j] == 0 ? cur[j] : 0.0);
+                const double v = FAST ? cur[j] : ldexp(cur[j], kk[j]);
                 pr = fma(v, (st & 1) ? gmr[j] : gpr[j], pr);
                 pi = fma(v, (st & 1) ? gmi[j] : gpi[j], pi);
                 const double nx = fma(x[j], cur[j], -(bq[st] * prev[j]));
@@ -1107,15 +1119,17 @@ __device__ __forceinline__ void anal_w_run(const LegArgs &a, const Item &it, dou
             red[(2 * st) * ANW_RS + lane] = pr;
             red[(2 * st + 1) * ANW_RS + lane] = pi;
         }
-        if (!FAST) {
+        if (!FAST && slow) {
             const double cl = qcm[lc + 7];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                if (kk[j] > 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
+                if (!(slow & (1u << j))) continue;
+                if (kk[j] < 0 && fmax(fabs(cur[j]), fabs(prev[j])) * cl > kHi) {
                     cur[j] *= kDown;
                     prev[j] *= kDown;
-                    kk[j] -= 1;
+                    kk[j] += 4096;
                 }
+                if (!__any(kk[j] < 0)) slow &= ~(1u << j);
             }
         }
         // wave-private transpose-reduce of the 16 partial sums

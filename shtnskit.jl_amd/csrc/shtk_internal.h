@@ -31,6 +31,7 @@ constexpr int SCALE_CHECK = 8;        // recurrence steps between checks
 // Q_{l+1} = x Q_l - beta_{l+1} Q_{l-1},  beta_l = b_l / (a_l a_{l-1}).
 // A plan-time kernel stores every segment's starting pair (Q_{l0-1}, Q_{l0}).
 constexpr int SEG_MAX = 256;
+constexpr int DSEG = 64;  // segment length of the register-record Legendre kernels
 
 // Coefficient tables, one block per stored order (offset coff[im], entries
 // l = m .. lmax+1):
@@ -70,6 +71,8 @@ struct LegArgs {
     int xcd;                // synthesis: 1 keeps the pair blocks of an order on one XCD
     int nm_run;             // synthesis: only orders im < nm_run (0: all)
     int skip_m0;            // vector: order 0 is computed by the zonal kernel (shtk_ml.hip)
+    int qscale_in;          // scalar synthesis (64-degree segments): in0 holds canonical FP64
+                            // coefficients, multiplied by c_l as they are loaded
 };
 
 // Work distribution over the 8 XCDs (each with its own 4 MB L2).  The

@@ -371,7 +371,6 @@ __device__ __forceinline__ void dsynth_seg(const SRec (&R)[4], cptr<double> qcm,
     }
 }
 
-constexpr int DSEG = 64;  // segment length of k_synth_d
 // waves per k_synth_d workgroup, taking the segments round-robin (2 waves time the
 // same; 1 wave per workgroup leaves a long tail of low orders: 82 against 65 us)
 constexpr int DSW = 4;
@@ -396,8 +395,10 @@ __global__ __launch_bounds__(64 * DSW) void k_synth_d(const LegArgs a) {
     const long long co = as_const(a.coff)[im] - m;
     const cptr<double> qcm = as_const(a.qcc) + co;
     const double *qbm = a.qb + co;  // padded: degrees to lmax + 64 are readable
-    // c_l a_l (k_spec_qscale), one order column; reads are clamped to lmax
+    // c_l a_l (k_spec_qscale, or a_l times c_l here), one order column; reads
+    // are clamped to lmax
     const double2 *cal = (const double2 *)a.in0 + (size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m;
+    const double *qcl = a.qcc + co;
     double acc[K][4];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0.0;
@@ -427,7 +428,14 @@ __global__ __launch_bounds__(64 * DSW) void k_synth_d(const LegArgs a) {
         for (int g = 0; g < 4; ++g) {
             const int l = m + sc * DSEG + 16 * g + (lane & 15);
             nr[g].b = qbm[l + 1];
-            nr[g].c = cal[min(l, lmax)];
+            const int lc = min(l, lmax);
+            double2 c = cal[lc];
+            if (a.qscale_in) {
+                const double q = qcl[lc];
+                c.x *= q;
+                c.y *= q;
+            }
+            nr[g].c = c;
         }
     };
     if (w < nseg) load_next(w);

@@ -367,8 +367,8 @@ struct SideStream {
             hipStream_t ns = nullptr;
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (hipStreamCreateWithFlags(&ns, hipStreamNonBlocking) != hipSuccess) return;
-            if (hipEventCreateWithFlags(&e0, hipEventDisableTiming) != hipSuccess ||
-                hipEventCreateWithFlags(&e1, hipEventDisableTiming) != hipSuccess) {
+            if (hipEventCreateWithFlags(&e0, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess ||
+                hipEventCreateWithFlags(&e1, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
                 if (e0) (void)hipEventDestroy(e0);
                 (void)hipStreamDestroy(ns);
                 return;
@@ -824,7 +824,11 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
     a.in0 = s0;
     a.in1 = s1;
     a.io_f32 = dtype == SHTK_F32;
-    if (!j.vector && j.prec == SHTK_PREC_FP64) {
+    if (!j.vector && j.prec == SHTK_PREC_FP64 && p->canonical && !a.io_f32 && p->seg == shtk::DSEG) {
+        // scalar FP64 synthesis of canonical FP64 coefficients: the kernel applies
+        // c_l (Q form) as it loads them
+        a.qscale_in = 1;
+    } else if (!j.vector && j.prec == SHTK_PREC_FP64) {
         // scalar FP64 synthesis: coefficients times c_l (Q form), converted to the
         // canonical convention on the way
         double *dst = (double *)spec;
@@ -1819,7 +1823,7 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
         delete p;
         return fail(SHTK_EHIP, "hipGetDevice failed (no ROCm device)");
     }
-    if (hipEventCreateWithFlags(&p->order_ev, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&p->order_ev, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
         delete p;
         return fail(SHTK_EHIP, "hipEventCreateWithFlags failed");
     }

@@ -73,6 +73,8 @@ struct LegArgs {
     int skip_m0;            // vector: order 0 is computed by the zonal kernel (shtk_ml.hip)
     int qscale_in;          // scalar synthesis (64-degree segments): in0 holds canonical FP64
                             // coefficients, multiplied by c_l as they are loaded
+    int vdirect;            // vector synthesis (64-degree segments): in0 / in1 hold canonical FP64
+                            // S and T, the records (S, T, c_l S, c_l T) built as they are loaded
 };
 
 // Work distribution over the 8 XCDs (each with its own 4 MB L2).  The

@@ -429,12 +429,12 @@ __global__ __launch_bounds__(64 * DSW) void k_synth_d(const LegArgs a) {
             const int l = m + sc * DSEG + 16 * g + (lane & 15);
             nr[g].b = qbm[l + 1];
             const int lc = min(l, lmax);
+            // no branch around the loads (it would complete them at its join)
             double2 c = cal[lc];
-            if (a.qscale_in) {
-                const double q = qcl[lc];
-                c.x *= q;
-                c.y *= q;
-            }
+            const double qv = qcl[lc];  // the table is padded past lmax: always readable
+            const double q = a.qscale_in ? qv : 1.0;
+            c.x *= q;
+            c.y *= q;
             nr[g].c = c;
         }
     };
@@ -784,7 +784,10 @@ __device__ __forceinline__ void vsynth_out(const LegArgs &a, double (*acc)[2][8]
     }
 }
 
-template <int K>
+// VD: the records are built from the caller's canonical FP64 S and T as they
+// are loaded (LegArgs.vdirect); a template parameter so that neither form
+// branches around its loads (a branch would complete them at its join).
+template <int K, bool VD>
 __global__ __launch_bounds__(256) void k_synth_vd(const LegArgs a) {
     __shared__ double red[4][16][64];
     int b, unit;
@@ -806,7 +809,11 @@ __global__ __launch_bounds__(256) void k_synth_vd(const LegArgs a) {
     const cptr<double> qcm = as_const(a.qcc) + co;
     const double *qbm = a.qb + co;                              // padded past lmax + 1
     const double2 *qdm = a.qd + co;                             // entries l = m .. lmax + 1
-    const double4 *rec = (const double4 *)a.in0 + 2 * ((size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m);
+    const size_t cbase = (size_t)f * a.spec_fstride + (size_t)(lmax + 1) * m;
+    const double4 *rec = (const double4 *)a.in0 + 2 * cbase;
+    // vdirect: the caller's canonical FP64 S and T, records built as they are loaded
+    const double2 *Sd = (const double2 *)a.in0 + cbase, *Td = (const double2 *)a.in1 + cbase;
+    const double *qcl = a.qcc + co;
     double acc[K][2][8];
 #pragma unroll
     for (int j = 0; j < K; ++j)
@@ -824,15 +831,26 @@ __global__ __launch_bounds__(256) void k_synth_vd(const LegArgs a) {
         d.dl = dd.x;
         d.dg = dd.y;
         const int lc = min(l, lmax);
-        const double4 r0 = rec[2 * (size_t)lc], r1 = rec[2 * (size_t)lc + 1];
-        d.R[0] = r0.x;
-        d.R[1] = r0.y;
-        d.R[2] = r0.z;
-        d.R[3] = r0.w;
-        d.R[4] = r1.x;
-        d.R[5] = r1.y;
-        d.R[6] = r1.z;
-        d.R[7] = r1.w;
+        if constexpr (VD) {
+            // raw values only: the products with c_l are formed when the group
+            // starts (a use here would wait for the loads at once)
+            const double2 sv = Sd[lc], tv = Td[lc];
+            d.R[0] = sv.x;
+            d.R[1] = sv.y;
+            d.R[2] = tv.x;
+            d.R[3] = tv.y;
+            d.R[4] = qcl[lc];
+        } else {
+            const double4 r0 = rec[2 * (size_t)lc], r1 = rec[2 * (size_t)lc + 1];
+            d.R[0] = r0.x;
+            d.R[1] = r0.y;
+            d.R[2] = r0.z;
+            d.R[3] = r0.w;
+            d.R[4] = r1.x;
+            d.R[5] = r1.y;
+            d.R[6] = r1.z;
+            d.R[7] = r1.w;
+        }
     };
     VRec nr;
     if (w < nseg) load_rec(nr, w, 0);
@@ -863,7 +881,14 @@ __global__ __launch_bounds__(256) void k_synth_vd(const LegArgs a) {
         for (int j = 0; j < K; ++j) cm[j] = kk[j] == 0 ? 1.0 : 0.0;
         for (int g = 0; g < 4; ++g) {
             const int l0g = l0 + 16 * g;
-            const VRec r = nr;
+            VRec r = nr;
+            if constexpr (VD) {
+                const double c = r.R[4];
+                r.R[4] = c * r.R[0];
+                r.R[5] = c * r.R[1];
+                r.R[6] = c * r.R[2];
+                r.R[7] = c * r.R[3];
+            }
             // the next group: this segment's next, or the first of the wave's next segment
             if (g < 3 && l0g + 16 < lstop) load_rec(nr, s, g + 1);
             else if (s + 4 < nseg) load_rec(nr, s + 4, 0);
@@ -948,13 +973,25 @@ __global__ __launch_bounds__(256) void k_vpole_synth(const LegArgs a, int robert
         }
         return;
     }
-    // prologue records (S', T', c S', c T') of order 1
-    const double *rec = (const double *)a.in0 + 8 * ((size_t)f * a.spec_fstride + (size_t)(a.lmax + 1));
+    // order 1: prologue records (S', T', c S', c T'), or with vdirect the caller's S and T
+    const size_t c1 = (size_t)f * a.spec_fstride + (size_t)(a.lmax + 1);
+    const double *rec = (const double *)a.in0 + 8 * c1;
+    const double2 *Sd = (const double2 *)a.in0 + c1, *Td = (const double2 *)a.in1 + c1;
     double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int lt = min(a.ltr, a.lmax);
     for (int l = 1 + (int)threadIdx.x; l <= lt; l += 256) {
         const double h = pole_h(l), hs = (l & 1) ? -h : h;
-        const double *r = rec + 8 * (size_t)l;
+        double r[4];
+        if (a.vdirect) {
+            const double2 sv = Sd[l], tv = Td[l];
+            r[0] = sv.x;
+            r[1] = sv.y;
+            r[2] = tv.x;
+            r[3] = tv.y;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = rec[8 * (size_t)l + k];
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             v[k] = fma(h, r[k], v[k]);
@@ -1511,8 +1548,10 @@ static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
     const dim3 grid(grid_for_units(a.xcd, a.gx, a.gy * a.gz));
     if (vector) {
         // inputs are the FP64 prologue records of k_vspec_prep
-        if (a.seg == DSEG)
-            hipLaunchKernelGGL((k_synth_vd<K>), grid, dim3(256), 0, st, a);
+        if (a.seg == DSEG && a.vdirect)
+            hipLaunchKernelGGL((k_synth_vd<K, true>), grid, dim3(256), 0, st, a);
+        else if (a.seg == DSEG)
+            hipLaunchKernelGGL((k_synth_vd<K, false>), grid, dim3(256), 0, st, a);
         else
             hipLaunchKernelGGL((k_synth_v<K>), grid, dim3(256), 0, st, a);
     } else {

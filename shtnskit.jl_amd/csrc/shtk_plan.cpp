@@ -837,6 +837,11 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
                 "coefficient scaling");
         a.in0 = dst;
         a.io_f32 = 0;
+    } else if (j.vector && j.prec == SHTK_PREC_FP64 && p->canonical && !a.io_f32 && !j.ll1_mode &&
+               p->seg == shtk::DSEG) {
+        // vector FP64 synthesis of canonical FP64 S, T: the kernel builds its records
+        // (S, T, c_l S, c_l T) as it loads them
+        a.vdirect = 1;
     } else if (j.vector && j.prec == SHTK_PREC_FP64) {
         // vector FP64 synthesis: records (S', T', c_l S', c_l T') with the convention
         // map and the l(l+1) factor applied

@@ -73,6 +73,8 @@ struct LegArgs {
     int skip_m0;            // vector: order 0 is computed by the zonal kernel (shtk_ml.hip)
     int qscale_in;          // scalar synthesis (64-degree segments): in0 holds canonical FP64
                             // coefficients, multiplied by c_l as they are loaded
+    int apblk;              // analysis: latitude pairs per work item (512, or 256 for the vector
+                            // analysis when it lets the kernel run 3 waves per SIMD)
     int vdirect;            // vector synthesis (64-degree segments): in0 / in1 hold canonical FP64
                             // S and T, the records (S, T, c_l S, c_l T) built as they are loaded
 };
@@ -157,7 +159,8 @@ struct shtk_plan {
     shtk_desc d{};
     int np = 0, nm = 0, seg = 128, nseg_total = 0;
     int xcd_map = 0, synth_chains = 4, fft_tile = 0;  // launch-shape knobs
-    int vsynth_chains = 1;  // latitude pairs per lane of the vector synthesis (1 or 2)
+    int vsynth_chains = 1;
+    int vanal_pblk = 256;  // pairs per vector analysis work item (SHTK_VANAL_PBLK: 256 or 512)  // latitude pairs per lane of the vector synthesis (1 or 2)
     int oz_nh_force = 0;  // split analysis pair halves per block (0: by grid size)
     bool canonical = true;
     bool poles = false;  // nodes at theta = 0 and pi (the first pair)

@@ -1437,7 +1437,7 @@ __global__ __launch_bounds__(64 * AVW_WAVES, KMAX <= 4 ? 3 : 1) void k_anal_vw(c
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int item = blockIdx.x * AVW_WAVES + w;
     if (item >= a.gx * a.gy * a.gz) return;
-    const Item it = make_item_at(a, item, 512);
+    const Item it = make_item_at(a, item, a.apblk);
     const size_t co = 2 * ((size_t)it.f * a.spec_fstride + (size_t)(a.lmax + 1) * it.m);
     OT *cs = (OT *)a.out0 + co;
     OT *ct = (OT *)a.out1 + co;
@@ -1578,8 +1578,9 @@ hipError_t launch_leg_synth(const LegArgs &a, int nfields, int vector, int chain
 
 hipError_t launch_leg_anal(const LegArgs &a0, int nfields, int vector, hipStream_t st) {
     if (a0.seg > SEG_MAX || (a0.seg & 7)) return hipErrorInvalidValue;
-    const int pblk = 512;  // pairs per work item
     LegArgs a = a0;
+    if (!vector || a.apblk != 256) a.apblk = 512;  // pairs per work item
+    const int pblk = a.apblk;
     a.gx = a.nseg_total;
     a.gy = (a.np + pblk - 1) / pblk;
     a.gz = nfields;

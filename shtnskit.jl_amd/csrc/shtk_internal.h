@@ -30,6 +30,15 @@ constexpr int SCALE_CHECK = 8;        // recurrence steps between checks
 // and c_l = c_{l-1} a_l, which turns P_l = a_l x P_{l-1} - b_l P_{l-2} into
 // Q_{l+1} = x Q_l - beta_{l+1} Q_{l-1},  beta_l = b_l / (a_l a_{l-1}).
 // A plan-time kernel stores every segment's starting pair (Q_{l0-1}, Q_{l0}).
+// dst += (NEG) bc[lane N of this lane's 16-lane row] * v: v_fmac_f64 with the
+// first operand broadcast by DPP row_newbcast (full FP64 rate).  A VALU write
+// of bc needs wait states before this reads it: callers put "s_nop 4" after
+// refilling bc registers.
+#define SHTK_FMAC_BC(dst, bc, v, N, NEG)                                                             \
+    asm volatile("v_fmac_f64_dpp %0, " NEG "%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" \
+                 : "+v"(dst)                                                                    \
+                 : "v"(bc), "v"(v), "i"(N))
+
 constexpr int SEG_MAX = 256;
 constexpr int DSEG = 64;  // segment length of the register-record Legendre kernels
 

@@ -672,10 +672,6 @@ struct VRec {
     double R[8];       // S'r, S'i, T'r, T'i, c S'r, c S'i, c T'r, c T'i
 };
 
-#define SHTK_FMAC_BC(dst, bc, v, N, NEG)                                                             \
-    asm volatile("v_fmac_f64_dpp %0, " NEG "%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" \
-                 : "+v"(dst)                                                                    \
-                 : "v"(bc), "v"(v), "i"(N))
 
 template <int K, int PAR, int N, bool NOP, bool FAST>
 __device__ __forceinline__ void vdstep(const VRec &r, const double *x, double *prev, double *cur, const double *cm,

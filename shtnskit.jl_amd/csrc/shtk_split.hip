@@ -411,8 +411,8 @@ __global__ __launch_bounds__(256) void k_oz_asc(const OzAscArgs a) {
 // column sums of every kstep.  A pair block is the 256 NH latitude pairs that
 // one analysis workgroup contracts (NH = 1 or 2 halves of 256 pairs).  Within
 // a block, K slot q (kstep q >> 6, lane group (q >> 4) & 3, byte q & 15) holds
-// local pair 64 (q % NW) + q / NW, NW = 4 NH: the order in which the analysis
-// waves write their digit bytes to LDS without bank conflicts.
+// local pair q: the analysis waves write the digit bytes of 4 consecutive
+// pairs as one dword (lane quads exchange bytes through DPP).
 //
 // k_oz_gprep: one workgroup of 256 NH threads per (pair block, order, field
 // group).  Thread (jj = tid % 16 NH, field tid / 16 NH) holds the 16 slots of
@@ -431,7 +431,28 @@ constexpr int OZ_MAX_KS = 32;  // np <= 2048
 
 template <int NH>
 __device__ __forceinline__ int oz_pair_of_slot(int q) {
-    return 64 * (q % (4 * NH)) + q / (4 * NH);
+    return q;
+}
+
+// LDS of the analysis A digits: [parity][slice][row 16][256 NH pair bytes + 16],
+// slices padded by 64 bytes so that the four slices a lane quad writes at once
+// fall on two bank groups 16 dwords apart (two lanes per bank, the minimum).
+template <int NH>
+struct OzLds {
+    static constexpr int ROWB = 256 * NH + 16;          // row stride (bytes)
+    static constexpr int SLS = 16 * ROWB + 64;          // slice stride
+    static constexpr int PS = OZ_SLICES * SLS;          // parity stride
+    static constexpr int BYTES = 2 * PS;
+};
+
+// 4x4 byte transpose inside each lane quad: lane q returns byte q of the
+// words of lanes 4j .. 4j+3 (in lane order).  Two DPP exchanges (lane ^ 1,
+// lane ^ 2) and two byte permutes.
+__device__ __forceinline__ unsigned quad_byte_transpose(unsigned v, unsigned sel1, unsigned sel2) {
+    const unsigned x = (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+    const unsigned b = __builtin_amdgcn_perm(x, v, sel1);
+    const unsigned y = (unsigned)__builtin_amdgcn_mov_dpp((int)b, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+    return __builtin_amdgcn_perm(y, b, sel2);
 }
 
 template <int NH>
@@ -525,12 +546,67 @@ __global__ __launch_bounds__(256 * NH) void k_oz_gprep(const OzArgs a) {
 // has one writer: plain stores when one block covers all pairs, atomics only
 // when several pair blocks add to it (np > 512).
 // ---------------------------------------------------------------------------
+// Per-degree constants of a 32-degree chunk in VGPRs: degree 16 h + n of the
+// chunk sits in lane n of each 16-lane row of a[h] (c_l 2^(46 - epl_l)), b[h]
+// (beta_{l+1}) and c[h] (c_l), and reaches the FMAs as a DPP row_newbcast
+// operand, so the digit loop has no scalar loads (whose waits also drained the
+// digit stores in flight).  The next chunk's are loaded while this one runs.
+struct OzRec {
+    double a[2], b[2], c[2];
+};
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Degree I of the chunk: digits of P_l 2^(46-epl) into the LDS rows, the Q-form
+// step, and after every 8th degree the scale check of a chain still in the
+// scaled range.
+template <int I, int NH>
+__device__ __forceinline__ void oz_digit_steps(const OzRec &R, double x, double &prev, double &cur, int &kk,
+                                               double &live, unsigned char *aul, unsigned char *auh, unsigned sel1,
+                                               unsigned sel2, unsigned &uh_even) {
+    if constexpr (I < 32) {
+        constexpr int H = I >> 4, N = I & 15, ROWB = OzLds<NH>::ROWB, PS = OzLds<NH>::PS;
+        // U = 2^47 + round(P 2^(46-epl)) lands in the low 48 bits of the mantissa
+        double dm = kMagic;
+        const double cl = cur * live;
+        SHTK_FMAC_BC(dm, R.a[H], cl, N, "");
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(dm);
+        const unsigned uh = (unsigned)(bits >> 32), ul = (unsigned)bits;
+        // slices 2..5 (bytes 3..0 of the low word); XOR 0x80 per byte: balanced digits
+        *(unsigned *)(aul + (I & 1) * PS + (I >> 1) * ROWB) = quad_byte_transpose(ul, sel1, sel2) ^ 0x80808080u;
+        if constexpr (I & 1) {
+            // slices 0, 1 (bytes 1, 0 of the high word) of degrees I - 1 and I
+            const unsigned pk = __builtin_amdgcn_perm(uh, uh_even, 0x05040100u);
+            *(unsigned *)(auh + (I >> 1) * ROWB) = quad_byte_transpose(pk, sel1, sel2) ^ 0x80808080u;
+        } else {
+            uh_even = uh;
+        }
+        double nx = x * cur;
+        SHTK_FMAC_BC(nx, R.b[H], prev, N, "-");
+        prev = cur;
+        cur = nx;
+        if constexpr ((I & 7) == 7) {
+            const double cq = readlane_f64(R.c[H], N);  // c_l of this degree
+            if (kk > 0 && fmax(fabs(cur), fabs(prev)) * cq > kHiS) {
+                cur *= kDownS;
+                prev *= kDownS;
+                --kk;
+                live = kk == 0 ? 1.0 : 0.0;
+            }
+        }
+        oz_digit_steps<I + 1, NH>(R, x, prev, cur, kk, live, aul, auh, sel1, sel2, uh_even);
+    }
+}
+
 template <int NH>
 __device__ __forceinline__ void anal_oz_body(const OzArgs &a, const double *__restrict__ asc,
                                              const double2 *__restrict__ qc, const int *__restrict__ epl,
                                              const int *__restrict__ pfirst, int pb, int run, int fg,
                                              unsigned char *Alds, int *Xch) {
-    constexpr int NW = 4 * NH, ROWB = 256 * NH + 16, RPW = NH == 2 ? 2 : 4;
+    constexpr int ROWB = OzLds<NH>::ROWB, SLS = OzLds<NH>::SLS, PS = OzLds<NH>::PS, RPW = NH == 2 ? 2 : 4;
     const cptr<int> ru = as_const((const int *)a.run) + 4 * run;
     const int im = ru[0], gs0 = ru[1], nsr = ru[2], m = im * a.mres;
     const int s0 = gs0 - as_const(a.segbase)[im];
@@ -540,9 +616,8 @@ __device__ __forceinline__ void anal_oz_body(const OzArgs &a, const double *__re
     const bool act = p < a.np;
     const double x = act ? a.x[p] : 0.0;
     const long long co = as_const(a.coff)[im] - m;
-    const cptr<double> ascm = as_const(asc) + co;
-    const cptr<double> qbm = as_const(a.qb) + co;
-    const cptr<double> qccm = as_const(a.qcc) + co;
+    // per-degree tables of the order (padded past lmax + 1), read per lane into OzRec
+    const double *ascl = asc + co, *qbl = a.qb + co, *qccl = a.qcc + co;
     const cptr<int> eplm = as_const(epl) + co;
     const cptr<int> pfc = as_const(pfirst);
 
@@ -565,7 +640,15 @@ __device__ __forceinline__ void anal_oz_body(const OzArgs &a, const double *__re
     const bool fok = f < a.nf;
     double *outc = a.spec_out + 2 * ((size_t)(fok ? f : 0) * a.spec_fstride + (size_t)(a.lmax + 1) * m) + (c & 1);
     const bool atom = a.npb > 1;
-    const int wpos = NW * (tid & 63) + (tid >> 6);  // LDS position of this lane's pair (as k_oz_gprep)
+    // digit stores: lane q of a quad writes slice 5 - q of the low word for the
+    // quad's 4 pairs, and (every other degree) slice 1 - (q & 1) of degree
+    // pair member q >> 1 of the high words
+    const int q4 = lane & 3;
+    const unsigned sel1 = (q4 & 1) ? 0x03070105u : 0x06020400u;
+    const unsigned sel2 = (q4 & 2) ? 0x03020706u : 0x05040100u;
+    const int dw = 4 * (16 * w + (lane >> 2));  // byte offset of the quad's dword in a row
+    unsigned char *aul = Alds + (5 - q4) * SLS + dw;
+    unsigned char *auh = Alds + (1 - (q4 & 1)) * SLS + (q4 >> 1) * PS + dw;
 
     // Segments of the run in which every pair of this block is still in the
     // scaled range contribute nothing (pfirst does not increase along an
@@ -586,6 +669,17 @@ __device__ __forceinline__ void anal_oz_body(const OzArgs &a, const double *__re
         kk = act ? a.seedk[q] : 0;
     }
     double live = kk == 0 ? 1.0 : 0.0;  // lanes still in the scaled range contribute zeros
+    const int rn = lane & 15;
+    auto load_rec = [&](OzRec &r, int lc) {
+        r.a[0] = ascl[lc + rn];
+        r.a[1] = ascl[lc + 16 + rn];
+        r.b[0] = qbl[lc + rn + 1];
+        r.b[1] = qbl[lc + 16 + rn + 1];
+        r.c[0] = qccl[lc + rn];
+        r.c[1] = qccl[lc + 16 + rn];
+    };
+    OzRec nrec;
+    load_rec(nrec, m + (s0 + sr0) * a.seg);
     const int wpu = __builtin_amdgcn_readfirstlane(wp);
     const int r0u = __builtin_amdgcn_readfirstlane(NH == 2 ? 2 * kh : 0);  // first row index i of this wave
     const int grp = lane >> 4;
@@ -618,64 +712,16 @@ __device__ __forceinline__ void anal_oz_body(const OzArgs &a, const double *__re
             for (int gg = 0; gg < 4; ++gg)
 #pragma unroll
                 for (int r = 0; r < RPW; ++r) ev[gg][r] = eplm[lc + wpu + 2 * (4 * gg + r0u + r)];
-            // Recurrence and digits of 32 degrees.  The per-degree constants of a
-            // block of 8 degrees arrive in one batch of scalar loads, issued
-            // during the previous block (after its own wait), so the steps of a
-            // block never wait for memory.
-            double aq[8], bq[8], cq;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                aq[j] = ascm[lc + j];
-                bq[j] = qbm[lc + j + 1];
-            }
-            cq = qccm[lc + 7];
-#pragma unroll
-            for (int blk = 0; blk < 4; ++blk) {
-                double na[8], nb[8], nc = 0.0;
-#pragma unroll
-                for (int qq = 0; qq < 8; ++qq) {
-                    const int i = blk * 8 + qq;
-                    // U = 2^47 + round(P 2^(46-epl)) lands in the low 48 bits of the mantissa
-                    const unsigned long long bits =
-                        (unsigned long long)__double_as_longlong(fma(cur * live, aq[qq], kMagic));
-                    const unsigned uh = (unsigned)(bits >> 32) ^ 0x8080u, ul = (unsigned)bits ^ 0x80808080u;
-                    unsigned char *row = Alds + ((i & 1) * OZ_SLICES * 16 + (i >> 1)) * ROWB + wpos;
-                    row[0 * 16 * ROWB] = (unsigned char)(uh >> 8);
-                    row[1 * 16 * ROWB] = (unsigned char)uh;
-                    row[2 * 16 * ROWB] = (unsigned char)(ul >> 24);
-                    row[3 * 16 * ROWB] = (unsigned char)(ul >> 16);
-                    row[4 * 16 * ROWB] = (unsigned char)(ul >> 8);
-                    row[5 * 16 * ROWB] = (unsigned char)ul;
-                    const double nx = fma(x, cur, -(bq[qq] * prev));
-                    prev = cur;
-                    cur = nx;
-                    if (qq == 0 && blk < 3) {
-                        // keeps the next batch below this block's wait
-                        __asm__ volatile("" ::: "memory");
-                        const int ln = lc + 8 * (blk + 1);
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            na[j] = ascm[ln + j];
-                            nb[j] = qbm[ln + j + 1];
-                        }
-                        nc = qccm[ln + 7];
-                    }
-                }
-                if (kk > 0 && fmax(fabs(cur), fabs(prev)) * cq > kHiS) {
-                    cur *= kDownS;
-                    prev *= kDownS;
-                    --kk;
-                    live = kk == 0 ? 1.0 : 0.0;
-                }
-                if (blk < 3) {
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        aq[j] = na[j];
-                        bq[j] = nb[j];
-                    }
-                    cq = nc;
-                }
-            }
+            // Recurrence and digits of 32 degrees from this chunk's records; the
+            // next chunk's (this segment's next, or the next segment's first) load
+            // meanwhile.
+            const OzRec rec = nrec;
+            // (clamped to lmax: reads stay inside the order's padded tables; a chunk
+            // that does not exist never uses them)
+            load_rec(nrec, min(t0 + 32 < nst ? lc + 32 : m + (s0 + sr + 1) * a.seg, a.lmax));
+            asm volatile("s_nop 4");  // wait states before DPP reads the refilled records
+            unsigned uh_even = 0;
+            oz_digit_steps<0, NH>(rec, x, prev, cur, kk, live, aul, auh, sel1, sel2, uh_even);
             __syncthreads();
             v4i acc[OZ_SLICES];
 #pragma unroll
@@ -685,7 +731,7 @@ __device__ __forceinline__ void anal_oz_body(const OzArgs &a, const double *__re
                 v4i Aop[OZ_SLICES];
 #pragma unroll
                 for (int sl = 0; sl < OZ_SLICES; ++sl)
-                    Aop[sl] = *(const v4i *)(Alds + ((wp * OZ_SLICES + sl) * 16 + (lane & 15)) * ROWB +
+                    Aop[sl] = *(const v4i *)(Alds + wp * PS + sl * SLS + (lane & 15) * ROWB +
                                              64 * (4 * kh + ks) + 16 * (lane >> 4));
 #pragma unroll
                 for (int t = 0; t < OZ_SLICES; ++t)
@@ -736,7 +782,7 @@ __global__ __launch_bounds__(256 * NH) void k_anal_oz(const OzArgs a, const doub
                                                        const double2 *__restrict__ qc, const int *__restrict__ epl,
                                                        const int *__restrict__ pfirst) {
     // rows padded by 16 bytes: the 16 lanes of an A-operand read hit distinct banks
-    __shared__ __align__(16) unsigned char Alds[2 * OZ_SLICES * 16 * (256 * NH + 16)];
+    __shared__ __align__(16) unsigned char Alds[OzLds<NH>::BYTES];
     __shared__ int Xch[NH == 2 ? 8 * 2 * OZ_SLICES * 64 : 1];
     const int item = blockIdx.x;
     anal_oz_body<NH>(a, asc, qc, epl, pfirst, item % a.npb, (item / a.npb) % a.nrun, item / (a.npb * a.nrun), Alds,
@@ -811,7 +857,7 @@ __global__ __launch_bounds__(256) void k_synth_oz_grp(const OzGroups G) {
 // every group of one launch has the same pair-block size NH.
 template <int NH>
 __global__ __launch_bounds__(256 * NH) void k_anal_oz_grp(const OzGroups G) {
-    __shared__ __align__(16) unsigned char Alds[2 * OZ_SLICES * 16 * (256 * NH + 16)];
+    __shared__ __align__(16) unsigned char Alds[OzLds<NH>::BYTES];
     __shared__ int Xch[NH == 2 ? 8 * 2 * OZ_SLICES * 64 : 1];
     __shared__ int s_unit;
     for (;;) {

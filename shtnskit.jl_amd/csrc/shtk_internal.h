@@ -331,6 +331,21 @@ __device__ inline void oz_zero_holes(const OzArgs &a, int im, int s0, int nsr, i
     }
 }
 
+// Per-degree constants of a 32-degree chunk of the matrix-core analyses in
+// VGPRs: degree 16 h + n of the chunk sits in lane n of each 16-lane row of
+// a[h] (c_l 2^(46 - epl_l)), b[h] (beta_{l+1}) and c[h] (c_l), and reaches the
+// FMAs as a DPP row_newbcast operand, so the digit loops have no scalar loads
+// (whose waits also drain the LDS stores in flight).  The next chunk's are
+// loaded while the current one runs.
+struct OzRec {
+    double a[2], b[2], c[2];
+};
+
+__device__ inline double readlane_f64(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
 struct OzSeedArgs {
     int lmax, mres, np, nm;
     const double *x;

@@ -222,6 +222,36 @@ __global__ __launch_bounds__(256) void k_f32_gprep(const OzArgs a) {
 // of the writes); a parity row holds NH halves of 4 * 72 and 4 more floats.
 __device__ __forceinline__ int f32_pos(int q) { return (q & 3) * 72 + (q >> 2); }
 
+// Degree I of a 32-degree chunk: P_l 2^-epl (FP32) into the LDS rows, the
+// Q-form step, and after every 8th degree the scale check of a chain still in
+// the scaled range.  lv46 = live x 2^-46 (live: 1 once the lane left the
+// scaled range, else 0).
+template <int I, int RSF>
+__device__ __forceinline__ void f32_digit_steps(const OzRec &R, double x, double &prev, double &cur, int &kk,
+                                                double &lv46, float *Alds, int wpos) {
+    if constexpr (I < 32) {
+        constexpr int H = I >> 4, N = I & 15;
+        double v = 0.0;
+        const double cl = cur * lv46;
+        SHTK_FMAC_BC(v, R.a[H], cl, N, "");
+        Alds[((I & 1) * 16 + (I >> 1)) * RSF + wpos] = (float)v;
+        double nx = x * cur;
+        SHTK_FMAC_BC(nx, R.b[H], prev, N, "-");
+        prev = cur;
+        cur = nx;
+        if constexpr ((I & 7) == 7) {
+            const double cq = readlane_f64(R.c[H], N);  // c_l of this degree
+            if (kk > 0 && fmax(fabs(cur), fabs(prev)) * cq > kHiS) {
+                cur *= kDownS;
+                prev *= kDownS;
+                --kk;
+                lv46 = kk == 0 ? 0x1p-46 : 0.0;
+            }
+        }
+        f32_digit_steps<I + 1, RSF>(R, x, prev, cur, kk, lv46, Alds, wpos);
+    }
+}
+
 template <int NH>
 __device__ __forceinline__ void anal_f32_body(const OzArgs &a, const double *__restrict__ asc,
                                               const double2 *__restrict__ qc, const int *__restrict__ epl,
@@ -237,9 +267,7 @@ __device__ __forceinline__ void anal_f32_body(const OzArgs &a, const double *__r
     const bool act = p < a.np;
     const double x = act ? a.x[p] : 0.0;
     const long long co = as_const(a.coff)[im] - m;
-    const cptr<double> ascm = as_const(asc) + co;
-    const cptr<double> qbm = as_const(a.qb) + co;
-    const cptr<double> qccm = as_const(a.qcc) + co;
+    const double *ascl = asc + co, *qbl = a.qb + co, *qccl = a.qcc + co;  // padded past lmax + 1
     const cptr<int> eplm = as_const(epl) + co;
     const cptr<int> pfc = as_const(pfirst);
 
@@ -270,7 +298,18 @@ __device__ __forceinline__ void anal_f32_body(const OzArgs &a, const double *__r
         cur = act ? a.seedq1[q] : 0.0;
         kk = act ? a.seedk[q] : 0;
     }
-    double live = kk == 0 ? 1.0 : 0.0;  // lanes still in the scaled range contribute zeros
+    double lv46 = kk == 0 ? 0x1p-46 : 0.0;  // lanes still in the scaled range contribute zeros
+    const int rn = lane & 15;
+    auto load_rec = [&](OzRec &r, int lc) {
+        r.a[0] = ascl[lc + rn];
+        r.a[1] = ascl[lc + 16 + rn];
+        r.b[0] = qbl[lc + rn + 1];
+        r.b[1] = qbl[lc + 16 + rn + 1];
+        r.c[0] = qccl[lc + rn];
+        r.c[1] = qccl[lc + 16 + rn];
+    };
+    OzRec nrec;
+    load_rec(nrec, m + (s0 + sr0) * a.seg);
     const int grp = lane >> 4;
     const int r0 = NH == 2 ? 2 * kh : 0;  // first of the rows 4 grp + e this wave emits
     for (int sr = sr0; sr < nsr; ++sr) {
@@ -288,35 +327,17 @@ __device__ __forceinline__ void anal_f32_body(const OzArgs &a, const double *__r
                 prev = act ? a.seedq0[q] : 0.0;
                 cur = act ? a.seedq1[q] : 0.0;
                 kk = act ? a.seedk[q] : 0;
-                live = kk == 0 ? 1.0 : 0.0;
+                lv46 = kk == 0 ? 0x1p-46 : 0.0;
             }
         }
         for (int t0 = 0; t0 < nst; t0 += 32) {
             const int lc = __builtin_amdgcn_readfirstlane(l0 + t0);
-#pragma unroll
-            for (int blk = 0; blk < 4; ++blk) {
-                double aq[8], bq[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    aq[j] = ascm[lc + 8 * blk + j] * 0x1p-46;  // c_l 2^-epl
-                    bq[j] = qbm[lc + 8 * blk + j + 1];
-                }
-                const double cq = qccm[lc + 8 * blk + 7];
-#pragma unroll
-                for (int qq = 0; qq < 8; ++qq) {
-                    const int i = blk * 8 + qq;
-                    Alds[((i & 1) * 16 + (i >> 1)) * RSF + wpos] = (float)(cur * live * aq[qq]);
-                    const double nx = fma(x, cur, -(bq[qq] * prev));
-                    prev = cur;
-                    cur = nx;
-                }
-                if (kk > 0 && fmax(fabs(cur), fabs(prev)) * cq > kHiS) {
-                    cur *= kDownS;
-                    prev *= kDownS;
-                    --kk;
-                    live = kk == 0 ? 1.0 : 0.0;
-                }
-            }
+            // this chunk's records; the next chunk's (clamped to lmax: reads stay
+            // inside the padded tables, a chunk that does not exist never uses them)
+            const OzRec rec = nrec;
+            load_rec(nrec, min(t0 + 32 < nst ? lc + 32 : m + (s0 + sr + 1) * a.seg, a.lmax));
+            asm volatile("s_nop 4");  // wait states before DPP reads the refilled records
+            f32_digit_steps<0, RSF>(rec, x, prev, cur, kk, lv46, Alds, wpos);
             __syncthreads();
             // rows: the 16 degrees of parity wp; K: the 256 pairs of half kh
             double accd[4] = {0.0, 0.0, 0.0, 0.0};

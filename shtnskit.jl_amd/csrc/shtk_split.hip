@@ -546,20 +546,6 @@ __global__ __launch_bounds__(256 * NH) void k_oz_gprep(const OzArgs a) {
 // has one writer: plain stores when one block covers all pairs, atomics only
 // when several pair blocks add to it (np > 512).
 // ---------------------------------------------------------------------------
-// Per-degree constants of a 32-degree chunk in VGPRs: degree 16 h + n of the
-// chunk sits in lane n of each 16-lane row of a[h] (c_l 2^(46 - epl_l)), b[h]
-// (beta_{l+1}) and c[h] (c_l), and reaches the FMAs as a DPP row_newbcast
-// operand, so the digit loop has no scalar loads (whose waits also drained the
-// digit stores in flight).  The next chunk's are loaded while this one runs.
-struct OzRec {
-    double a[2], b[2], c[2];
-};
-
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
-                            __builtin_amdgcn_readlane(__double2loint(v), lane));
-}
-
 // Degree I of the chunk: digits of P_l 2^(46-epl) into the LDS rows, the Q-form
 // step, and after every 8th degree the scale check of a chain still in the
 // scaled range.

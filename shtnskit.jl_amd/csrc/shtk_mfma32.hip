@@ -87,25 +87,37 @@ __device__ __forceinline__ void synth_f32_body(const OzArgs &a, int pbk, int im,
             for (int e = 0; e < 4; ++e) accd[par][ct][e] = 0.0;
         }
     const float *stf = (const float *)stg;
-    for (int rho = 0; rho < nch; ++rho) {
-        __syncthreads();  // the previous chunk's operands are consumed
+    // chunk operands (coefficients, recurrence coefficients) are loaded one chunk
+    // ahead into registers and staged in LDS at the chunk start
+    C2 nv[8];
+    double2 nab = make_double2(0.0, 0.0);
+    auto load_chunk = [&](int rho) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int e = tid + 256 * k;
             const int l32 = e & 31, gg = (e >> 5) & 3, fl = e >> 7;
             const int f = fg * OZ_FIELDS + fl;
             const int l = m + gg * Kq + 32 * rho + l32;
-            float2 v = make_float2(0.f, 0.f);
-            if (f < a.nf && l <= lend) {
-                const C2 c = spec[(size_t)f * a.spec_fstride + l];
-                v = make_float2((float)c.x, (float)c.y);
-            }
-            stg[fl * F32_FS + gg * F32_GS + l32] = v;
+            const bool ok = f < a.nf && l <= lend;
+            nv[k] = spec[ok ? (size_t)f * a.spec_fstride + l : 0];
+            if (!ok) nv[k].x = nv[k].y = 0;
         }
         if (tid < 128) {
             const int l = m + (tid >> 5) * Kq + 32 * rho + (tid & 31);
-            abch[tid] = ab[min(l, lmax) + 1];
+            nab = ab[min(l, lmax) + 1];
         }
+    };
+    load_chunk(0);
+    for (int rho = 0; rho < nch; ++rho) {
+        __syncthreads();  // the previous chunk's operands are consumed
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int e = tid + 256 * k;
+            const int l32 = e & 31, gg = (e >> 5) & 3, fl = e >> 7;
+            stg[fl * F32_FS + gg * F32_GS + l32] = make_float2((float)nv[k].x, (float)nv[k].y);
+        }
+        if (tid < 128) abch[tid] = nab;
+        if (rho + 1 < nch) load_chunk(rho + 1);
         __syncthreads();
         // a lane still in the scaled range contributes zeros (scale 0); so does a
         // degree past lend: the recurrence keeps running there with the last

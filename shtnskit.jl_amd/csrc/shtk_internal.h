@@ -240,6 +240,7 @@ struct shtk_plan {
     hipEvent_t order_ev = nullptr;
     hipStream_t order_st = nullptr;
     bool order_used = false;
+    bool order_multi = false;  // calls came from more than one stream: events are recorded
     // Second stream of the plan (made on first use, outside graph capture) and
     // its fork/join events: the order-0 rows of the vector transforms run on it
     // beside the main Legendre kernel.

@@ -395,8 +395,11 @@ struct SideStream {
 // Calls on one plan share its workspaces.  The host mutex orders their
 // enqueueing; on the device, a call issued on another stream than the previous
 // call first waits for the event recorded at the end of that call, so kernels
-// of two streams never overlap on the workspaces.  Work being captured into a
-// graph skips both steps (the graph's owner orders its launches).
+// of two streams never overlap on the workspaces.  A plan used from one stream
+// only records no events (each record is a queue marker between launches): the
+// first call from a second stream synchronises the device once, and from then
+// on every call records.  Work being captured into a graph skips all of this
+// (the graph's owner orders its launches).
 struct StreamOrder {
     shtk_plan *p;
     hipStream_t st;
@@ -405,14 +408,20 @@ struct StreamOrder {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
         active = true;
-        if (p->order_used && p->order_st != st) (void)hipStreamWaitEvent(st, p->order_ev, 0);
+        if (p->order_used && p->order_st != st) {
+            if (p->order_multi) {
+                (void)hipStreamWaitEvent(st, p->order_ev, 0);
+            } else {
+                (void)hipDeviceSynchronize();
+                p->order_multi = true;
+            }
+        }
     }
     ~StreamOrder() {
-        if (!active || !p->order_ev) return;  // the event is made with the plan
-        if (hipEventRecord(p->order_ev, st) == hipSuccess) {
-            p->order_st = st;
-            p->order_used = true;
-        }
+        if (!active) return;
+        p->order_st = st;
+        p->order_used = true;
+        if (p->order_multi && p->order_ev) (void)hipEventRecord(p->order_ev, st);
     }
     StreamOrder(const StreamOrder &) = delete;
     StreamOrder &operator=(const StreamOrder &) = delete;

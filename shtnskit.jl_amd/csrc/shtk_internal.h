@@ -82,6 +82,7 @@ struct LegArgs {
     int skip_m0;            // vector: order 0 is computed by the zonal kernel (shtk_ml.hip)
     int qscale_in;          // scalar synthesis (64-degree segments): in0 holds canonical FP64
                             // coefficients, multiplied by c_l as they are loaded
+    int vwaves;             // vector synthesis (64-degree segments): waves per block, 1 or 4
     int apblk;              // analysis: latitude pairs per work item (512, or 256 for the vector
                             // analysis when it lets the kernel run 3 waves per SIMD)
     int vdirect;            // vector synthesis (64-degree segments): in0 / in1 hold canonical FP64
@@ -169,6 +170,7 @@ struct shtk_plan {
     int np = 0, nm = 0, seg = 128, nseg_total = 0;
     int xcd_map = 0, synth_chains = 4, fft_tile = 0;  // launch-shape knobs
     int vsynth_chains = 1;
+    int vsynth_waves = 1;  // waves per vector synthesis block (SHTK_VSYNTH_WAVES: 1 or 4)
     int vanal_pblk = 256;  // pairs per vector analysis work item (SHTK_VANAL_PBLK: 256 or 512)  // latitude pairs per lane of the vector synthesis (1 or 2)
     int oz_nh_force = 0;  // split analysis pair halves per block (0: by grid size)
     bool canonical = true;

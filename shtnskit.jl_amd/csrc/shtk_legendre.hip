@@ -735,7 +735,7 @@ __device__ __forceinline__ bool vdgroup_from(const VRec &r, cptr<double> qcm, in
 
 // The accumulated sums of the pair block -> Fourier columns of both components
 // (north and south rows by parity), as k_synth_v.
-template <int K>
+template <int K, int NW = 4>
 __device__ __forceinline__ void vsynth_out(const LegArgs &a, double (*acc)[2][8], double (*red)[16][64], int w,
                                            int lane, int pb0, int im, int f) {
     const int m = im * a.mres, np = a.np;
@@ -744,19 +744,26 @@ __device__ __forceinline__ void vsynth_out(const LegArgs &a, double (*acc)[2][8]
     const size_t fo = ((size_t)f * a.nm + im) * a.nlat;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        if (j > 0) __syncthreads();
+        if constexpr (NW > 1) {
+            if (j > 0) __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+            for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int r = 0; r < 8; ++r) red[w][8 * q + r][lane] = acc[j][q][r];
-        __syncthreads();
+                for (int r = 0; r < 8; ++r) red[w][8 * q + r][lane] = acc[j][q][r];
+            __syncthreads();
+        }
         const int p = pb0 + lane + 64 * j;
         if (w == 0 && p < np) {
             double e[8], o[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                e[r] = ((red[0][r][lane] + red[1][r][lane]) + red[2][r][lane]) + red[3][r][lane];
-                o[r] = ((red[0][8 + r][lane] + red[1][8 + r][lane]) + red[2][8 + r][lane]) + red[3][8 + r][lane];
+                if constexpr (NW > 1) {
+                    e[r] = ((red[0][r][lane] + red[1][r][lane]) + red[2][r][lane]) + red[3][r][lane];
+                    o[r] = ((red[0][8 + r][lane] + red[1][8 + r][lane]) + red[2][8 + r][lane]) + red[3][8 + r][lane];
+                } else {
+                    e[r] = acc[j][0][r];
+                    o[r] = acc[j][1][r];
+                }
             }
 #pragma unroll
             for (int r = 2; r < 6; ++r) {
@@ -783,16 +790,20 @@ __device__ __forceinline__ void vsynth_out(const LegArgs &a, double (*acc)[2][8]
 // VD: the records are built from the caller's canonical FP64 S and T as they
 // are loaded (LegArgs.vdirect); a template parameter so that neither form
 // branches around its loads (a branch would complete them at its join).
-template <int K, bool VD>
-__global__ __launch_bounds__(256) void k_synth_vd(const LegArgs a) {
-    __shared__ double red[4][16][64];
+// NW = 4: the block's 4 waves take the order's segments round-robin and add
+// their sums through LDS (units in order-major order); NW = 1: one wave runs all
+// segments of its (order, pair block, field) with no barrier, units ordered
+// lowest order first for every field (longest work first).
+template <int K, bool VD, int NW>
+__global__ __launch_bounds__(64 * NW) void k_synth_vd(const LegArgs a) {
+    __shared__ double red[NW > 1 ? 4 : 1][16][64];
     int b, unit;
     if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
-    const int im = unit % a.gy, f = unit / a.gy;
+    const int im = NW > 1 ? unit % a.gy : unit / a.gz, f = NW > 1 ? unit / a.gy : unit % a.gz;
     if (a.skip_m0 && im == 0) return;  // written by the zonal kernel
     const int m = im * a.mres;
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = NW > 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0;
     const int np = a.np, lmax = a.lmax, ltr = a.ltr;
     const int pb0 = b * 64 * K, pb1 = min(pb0 + 64 * K, np);
     double x[K];
@@ -850,14 +861,14 @@ __global__ __launch_bounds__(256) void k_synth_vd(const LegArgs a) {
     };
     VRec nr;
     if (w < nseg) load_rec(nr, w, 0);
-    for (int s = w; s < nseg; s += 4) {
+    for (int s = w; s < nseg; s += NW) {
         const int l0 = m + s * DSEG;
         if (l0 > ltr) break;
         const int gs = segb + s;
         const int pf = as_const(a.pfirst)[gs], pfa = as_const(a.pfast)[gs];
         const int lstop = min(l0 + DSEG, ltr + 1);
         if (pb1 <= pf) {
-            if (s + 4 < nseg) load_rec(nr, s + 4, 0);
+            if (s + NW < nseg) load_rec(nr, s + NW, 0);
             continue;
         }
         double prev[K], cur[K];
@@ -887,13 +898,13 @@ __global__ __launch_bounds__(256) void k_synth_vd(const LegArgs a) {
             }
             // the next group: this segment's next, or the first of the wave's next segment
             if (g < 3 && l0g + 16 < lstop) load_rec(nr, s, g + 1);
-            else if (s + 4 < nseg) load_rec(nr, s + 4, 0);
+            else if (s + NW < nseg) load_rec(nr, s + NW, 0);
             const bool full = fast ? vdgroup_from<K, true, 0>(r, qcm, l0g, lstop - l0g, x, prev, cur, kk, cm, acc)
                                    : vdgroup_from<K, false, 0>(r, qcm, l0g, lstop - l0g, x, prev, cur, kk, cm, acc);
             if (!full || l0g + 16 >= lstop) break;
         }
     }
-    vsynth_out<K>(a, acc, red, w, lane, pb0, im, f);
+    vsynth_out<K, NW>(a, acc, red, w, lane, pb0, im, f);
 }
 
 // Prologue of the vector synthesis: per (order, field) and l >= m the record
@@ -1544,10 +1555,13 @@ static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
     const dim3 grid(grid_for_units(a.xcd, a.gx, a.gy * a.gz));
     if (vector) {
         // inputs are the FP64 prologue records of k_vspec_prep
-        if (a.seg == DSEG && a.vdirect)
-            hipLaunchKernelGGL((k_synth_vd<K, true>), grid, dim3(256), 0, st, a);
+        if (a.seg == DSEG && a.vwaves == 1) {
+            if (a.vdirect) hipLaunchKernelGGL((k_synth_vd<K, true, 1>), grid, dim3(64), 0, st, a);
+            else hipLaunchKernelGGL((k_synth_vd<K, false, 1>), grid, dim3(64), 0, st, a);
+        } else if (a.seg == DSEG && a.vdirect)
+            hipLaunchKernelGGL((k_synth_vd<K, true, 4>), grid, dim3(256), 0, st, a);
         else if (a.seg == DSEG)
-            hipLaunchKernelGGL((k_synth_vd<K, false>), grid, dim3(256), 0, st, a);
+            hipLaunchKernelGGL((k_synth_vd<K, false, 4>), grid, dim3(256), 0, st, a);
         else
             hipLaunchKernelGGL((k_synth_v<K>), grid, dim3(256), 0, st, a);
     } else {

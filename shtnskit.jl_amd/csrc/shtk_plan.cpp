@@ -833,6 +833,7 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
     a.in0 = s0;
     a.in1 = s1;
     a.io_f32 = dtype == SHTK_F32;
+    a.vwaves = p->vsynth_waves;
     if (!j.vector && j.prec == SHTK_PREC_FP64 && p->canonical && !a.io_f32 && p->seg == shtk::DSEG) {
         // scalar FP64 synthesis of canonical FP64 coefficients: the kernel applies
         // c_l (Q form) as it loads them
@@ -1076,6 +1077,7 @@ int ensure_vshadow(shtk_plan *p) {
     s->synth_chains = p->synth_chains;
     s->vsynth_chains = p->vsynth_chains;
     s->vanal_pblk = p->vanal_pblk;
+    s->vsynth_waves = p->vsynth_waves;
     s->fft_tile = p->fft_tile;
     s->oz_run_len = p->oz_run_len;
     s->oz_nh_force = p->oz_nh_force;
@@ -1863,6 +1865,7 @@ int shtk_plan_create(const shtk_desc *desc, shtk_plan **out) {
     // atomically when np > 256) instead of 512 (8 chains, 256 VGPRs, 1 wave/SIMD): 2.9 -> 2.3 ms
     // for 8 fields at lmax 1023
     p->vanal_pblk = env_int("SHTK_VANAL_PBLK", 256) == 512 ? 512 : 256;
+    p->vsynth_waves = env_int("SHTK_VSYNTH_WAVES", 1) == 4 ? 4 : 1;
     p->fft_tile = env_int("SHTK_FFT_PAIRS", 0);
     p->oz_run_len = std::max(1, env_int("SHTK_OZ_RUN", 8));
     {

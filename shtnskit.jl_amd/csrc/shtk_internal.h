@@ -246,6 +246,8 @@ struct shtk_plan {
     // Second stream of the plan (made on first use, outside graph capture) and
     // its fork/join events: the order-0 rows of the vector transforms run on it
     // beside the main Legendre kernel.
+    double *zseed = nullptr;  // order-0 segment starts of the vector pair (made on first use)
+    int zsegs = 0;
     hipStream_t side = nullptr;
     hipEvent_t side_fork = nullptr, side_join = nullptr;
     // Incremented whenever ws or oz_ws is reallocated: captured graphs that hold
@@ -432,9 +434,17 @@ struct MlArgs {
     int nrow;            // rows of the Fourier columns (nlat, or nlat - 1: no south-pole row)
     long long ld_spec, ld_grid;
     int fourier_f64;     // the Fourier columns are FP64 whatever io_f32 says (plan workspace)
+    // order 0 of the vector pair in segments of ZSEG degrees run in parallel (nullptr:
+    // one serial pass): [3][zsegs][np] = P_{l0-1}, P_{l0}, dP_{l0-1}/dtheta at l0 = ZSEG s
+    const double *zseed;
+    int zsegs;
 };
+constexpr int ZSEG = 64;  // degrees per order-0 segment (the staging chunk of the kernels)
 hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st);
 hipError_t launch_ml_anal(const MlArgs &a, int nfields, hipStream_t st);
+// Plan-time order-0 segment starts for MlArgs.zseed (one thread per pair runs the
+// zonal recurrences once): out [3][nseg][np].
+hipError_t launch_zonal_seed(const MlArgs &a, double *out, int nseg, hipStream_t st);
 // Pole rows of order 1 for the vector transforms on grids with nodes at the
 // poles (one block per field): synthesis rewrites them from the prologue
 // records in a.in0 times wscale (robert = 1: zeros every order's pole rows),

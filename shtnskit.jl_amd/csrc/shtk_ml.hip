@@ -226,8 +226,19 @@ __global__ __launch_bounds__(64) void k_ml_anal(const MlArgs a) {
     double *acc = a.acc + 4 * ((size_t)f * gridDim.x + blockIdx.x) * nl;
     const double mm = (double)m;
     double dz = 0.0;
-    for (int c0 = m; c0 <= a.ltr; c0 += ML_CH) {
-        const int n = min(ML_CH, a.ltr + 1 - c0);
+    int cfirst = m, clast = a.ltr;
+    if (a.zseed) {
+        // order 0 in segments: this block's segment blockIdx.z from its start values
+        const int sgm = blockIdx.z;
+        const size_t zq = (size_t)sgm * a.np + p;
+        r.prev = a.zseed[zq];
+        r.cur = a.zseed[(size_t)a.zsegs * a.np + zq];
+        dz = a.zseed[(size_t)2 * a.zsegs * a.np + zq];
+        cfirst = sgm * ZSEG;
+        clast = min(a.ltr, cfirst + ZSEG - 1);
+    }
+    for (int c0 = cfirst; c0 <= clast; c0 += ML_CH) {
+        const int n = min(ML_CH, clast + 1 - c0);
         __syncthreads();
         ml_stage<IT>(a, S, c0, n, nullptr, nullptr, false);
         __syncthreads();
@@ -278,6 +289,123 @@ __global__ __launch_bounds__(64) void k_ml_anal(const MlArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Order 0 of the vector pair in parallel segments of ZSEG degrees.  Order 0 is
+// never in the scaled range, so a segment needs only P_{l0-1}, P_{l0} and the
+// zonal derivative dP_{l0-1}/dtheta at its start (k_zonal_seed, once per plan);
+// from there the kernels run the same two recurrences as k_ml_synth /
+// k_ml_anal over the segment alone.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_zonal_seed(const MlArgs a, double *out, int nseg) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.np) return;
+    const double x = a.x[p], s = a.s[p];
+    double prev = 0.0, cur = a.sectv[p], dz = 0.0;  // order 0: scale count 0
+    for (int l = 0; l <= a.lmax; ++l) {
+        if (l % ZSEG == 0) {
+            const int sg = l / ZSEG;
+            out[((size_t)0 * nseg + sg) * a.np + p] = prev;
+            out[((size_t)1 * nseg + sg) * a.np + p] = cur;
+            out[((size_t)2 * nseg + sg) * a.np + p] = dz;
+        }
+        const double g = a.gtab[l];
+        if (l > 0) dz = ml_zonal_step(g, g / (double)l, x, s, prev, dz);
+        if (l < a.lmax) {
+            const double2 co = a.ab[l + 1];
+            const double nx = fma(co.x * x, cur, -(co.y * prev));
+            prev = cur;
+            cur = nx;
+        }
+    }
+}
+
+// Synthesis: block of ZW waves per (64 pairs, field); wave w runs segments
+// w, w + ZW, ... (operands staged in its own LDS slice), the waves' sums are
+// added through LDS at the end.
+constexpr int ZW = 8;
+template <typename IT, typename OT>
+__global__ __launch_bounds__(64 * ZW) void k_zonal_synth(const MlArgs a) {
+    __shared__ MlStage Sw[ZW];
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int p0 = blockIdx.x * 64 + lane, f = blockIdx.y;
+    const bool act = p0 < a.np;
+    const int p = act ? p0 : 0;
+    const double x = a.x[p], s = a.s[p];
+    const IT *in0 = (const IT *)a.in0 + 2 * (size_t)f * a.ld_spec;
+    const IT *in1 = (const IT *)a.in1 + 2 * (size_t)f * a.ld_spec;
+    double2 n0 = make_double2(0.0, 0.0), s0 = n0, n1 = n0, s1 = n0;
+    const int nseg = a.ltr / ZSEG + 1;
+    MlStage &S = Sw[w];
+    for (int sgm = w; sgm < nseg; sgm += ZW) {
+        const int l0 = sgm * ZSEG, n = min(ZSEG, a.ltr + 1 - l0);
+        const size_t zq = (size_t)sgm * a.np + p;
+        double prev = a.zseed[zq], cur = a.zseed[(size_t)a.zsegs * a.np + zq];
+        double dz = a.zseed[(size_t)2 * a.zsegs * a.np + zq];
+        if (lane < n) {
+            const int l = l0 + lane;
+            S.ab[lane] = a.ab[l + 1];
+            S.g[lane] = a.gtab[l];
+            S.gl[lane] = l > 0 ? S.g[lane] / (double)l : 0.0;
+            S.c[lane] = a.conv ? a.conv[l] : 1.0;
+            S.q0[lane] = ld_c(in0, (size_t)l);
+            S.q1[lane] = ld_c(in1, (size_t)l);
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // this wave's staging is in
+#pragma unroll 4
+        for (int t = 0; t < n; ++t) {
+            const int l = l0 + t;
+            if (l > 0) dz = ml_zonal_step(S.g[t], S.gl[t], x, s, prev, dz);
+            if (l >= 1) {
+                // order 0: V_theta = dP/dtheta S, V_phi = dP/dtheta T (Robert form: times
+                // sin(theta)); south rows take the parity (-1)^l with the opposite sign
+                const double dn = a.robert ? s * dz : dz;
+                const double c = S.c[t];
+                const double2 Sq = S.q0[t], Tq = S.q1[t];
+                const double Sr = c * Sq.x, Si = c * Sq.y, Tr = c * Tq.x, Ti = c * Tq.y;
+                const double sg = (l & 1) ? -1.0 : 1.0;
+                n0.x += dn * Sr;
+                n0.y += dn * Si;
+                n1.x += dn * Tr;
+                n1.y += dn * Ti;
+                s0.x += sg * (-dn * Sr);
+                s0.y += sg * (-dn * Si);
+                s1.x += sg * (-dn * Tr);
+                s1.y += sg * (-dn * Ti);
+            }
+            const double2 co = S.ab[t];
+            const double nx = fma(co.x * x, cur, -(co.y * prev));
+            prev = cur;
+            cur = nx;
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // reads done before restaging
+    }
+    __syncthreads();
+    double *red = (double *)Sw;  // [ZW][8][64] doubles, within the staging slices
+    const double v8[8] = {n0.x, n0.y, s0.x, s0.y, n1.x, n1.y, s1.x, s1.y};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) red[(w * 8 + q) * 64 + lane] = v8[q];
+    __syncthreads();
+    if (w != 0 || !act) return;
+    double t8[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        double u = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < ZW; ++ww) u += red[(ww * 8 + q) * 64 + lane];
+        t8[q] = u;
+    }
+    const double g = a.g;
+    OT *o0 = (OT *)a.out0 + 2 * (size_t)f * a.ld_grid;
+    OT *o1 = (OT *)a.out1 + 2 * (size_t)f * a.ld_grid;
+    const int ps = a.nlat - 1 - p;
+    st_c2(o0, p, g * t8[0], g * t8[1]);
+    st_c2(o1, p, g * t8[4], g * t8[5]);
+    if (ps != p && ps < a.nrow) {
+        st_c2(o0, ps, g * t8[2], g * t8[3]);
+        st_c2(o1, ps, g * t8[6], g * t8[7]);
+    }
+}
+
 template <typename OT>
 __global__ __launch_bounds__(256) void k_ml_fin(const MlArgs a) {
     const int f = blockIdx.x, m = a.m, nl = a.ltr - m + 1, nb = (a.np + 63) / 64;
@@ -298,8 +426,20 @@ __global__ __launch_bounds__(256) void k_ml_fin(const MlArgs a) {
     }
 }
 
+hipError_t launch_zonal_seed(const MlArgs &a, double *out, int nseg, hipStream_t st) {
+    if (a.m != 0 || nseg != a.lmax / ZSEG + 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zonal_seed, dim3((unsigned)((a.np + 255) / 256)), dim3(256), 0, st, a, out, nseg);
+    return hipGetLastError();
+}
+
 hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st) {
     const dim3 grid((unsigned)((a.np + 63) / 64), (unsigned)nfields);
+    if (a.zseed && a.vector && a.m == 0 && a.fourier_f64) {
+        if (a.ltr / ZSEG + 1 > a.zsegs) return hipErrorInvalidValue;
+        if (a.io_f32) hipLaunchKernelGGL((k_zonal_synth<float, double>), grid, dim3(64 * ZW), 0, st, a);
+        else hipLaunchKernelGGL((k_zonal_synth<double, double>), grid, dim3(64 * ZW), 0, st, a);
+        return hipGetLastError();
+    }
     if (a.io_f32 && a.fourier_f64)
         hipLaunchKernelGGL((k_ml_synth<float, double>), grid, dim3(64), 0, st, a);
     else if (a.fourier_f64)
@@ -312,7 +452,10 @@ hipError_t launch_ml_synth(const MlArgs &a, int nfields, hipStream_t st) {
 }
 
 hipError_t launch_ml_anal(const MlArgs &a, int nfields, hipStream_t st) {
-    const dim3 grid((unsigned)((a.np + 63) / 64), (unsigned)nfields);
+    const bool zs = a.zseed && a.vector && a.m == 0;
+    if (zs && a.ltr / ZSEG + 1 > a.zsegs) return hipErrorInvalidValue;
+    if (a.zseed && !zs) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((a.np + 63) / 64), (unsigned)nfields, zs ? (unsigned)(a.ltr / ZSEG + 1) : 1u);
     if (a.fourier_f64) {
         hipLaunchKernelGGL(k_ml_anal<double>, grid, dim3(64), 0, st, a);
         if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();

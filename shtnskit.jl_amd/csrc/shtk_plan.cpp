@@ -246,7 +246,7 @@ void free_plan(shtk_plan *p) {
                     p->qd,     p->coff,   p->segbase, p->seedq0, p->seedq1, p->seedk, p->pfirst,
                     p->pfast,  p->segm,   p->convscale, p->tw, p->chirp, p->bhat, p->ws,     p->sectv, p->sectk, p->oz_q0,
                     p->oz_q1,  p->oz_qk,  p->oz_ep,     p->oz_chbase, p->oz_ws, p->oz_emaxl, p->oz_epl,
-                    p->oz_asc, p->oz_chm, p->oz_run, p->qb, p->qcc};
+                    p->oz_asc, p->oz_chm, p->oz_run, p->qb, p->qcc, p->zseed};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (p->order_ev) (void)hipEventDestroy(p->order_ev);
@@ -806,7 +806,23 @@ shtk::MlArgs zonal_args(const shtk_plan *p, int ltr, long long spec_fstride) {
     A.ld_grid = (long long)p->nm * p->d.nlat;
     A.nrow = p->d.nlat;
     A.fourier_f64 = 1;
+    A.zseed = p->zseed;
+    A.zsegs = p->zsegs;
     return A;
+}
+
+// Order-0 segment starts of the vector pair (shtk::launch_zonal_seed), made on
+// the first vector call; without them the order-0 kernels run one serial pass.
+int ensure_zseed(shtk_plan *p, hipStream_t st) {
+    if (p->zseed) return SHTK_OK;
+    const int nseg = p->d.lmax / shtk::ZSEG + 1;
+    double *z = nullptr;
+    HIP_TRY(hipMalloc((void **)&z, sizeof(double) * 3 * (size_t)nseg * p->np), "order-0 seeds");
+    p->zseed = z;  // freed with the plan even if the launch fails
+    shtk::MlArgs A = zonal_args(p, p->d.lmax, 0);
+    HIP_TRY(shtk::launch_zonal_seed(A, z, nseg, st), "order-0 seed kernel");
+    p->zsegs = nseg;
+    return SHTK_OK;
 }
 
 // spectral -> grid: [convert] -> Legendre synthesis -> inverse phi FFT
@@ -897,6 +913,7 @@ int run_to_grid(shtk_plan *p, const Job &j, const void *s0, const void *s1, void
         const bool zonal = j.vector && j.pole_fix == 1 && j.ltr >= 1;
         a.skip_m0 = zonal;
         if (zonal) {
+            RC_TRY(ensure_zseed(p, st));
             shtk::MlArgs z = zonal_args(p, j.ltr, a.spec_fstride);
             z.io_f32 = dtype == SHTK_F32;
             z.conv = p->canonical ? nullptr : p->convscale;  // order 0: index l
@@ -1026,6 +1043,7 @@ int run_to_spec(shtk_plan *p, const Job &j, const void *g0, const void *g1, void
         // (which skips it); the convention map of order 0 follows with the rest
         // (non-canonical plans)
         a.skip_m0 = 1;
+        RC_TRY(ensure_zseed(p, st));
         shtk::MlArgs z = zonal_args(p, j.ltr, a.spec_fstride);
         z.io_f32 = a.io_f32;
         z.g = j.g;

@@ -455,39 +455,62 @@ __device__ __forceinline__ unsigned quad_byte_transpose(unsigned v, unsigned sel
     return __builtin_amdgcn_perm(y, b, sel2);
 }
 
+// LDS staging of the Fourier rows of 4 fields of a pair block (north, south),
+// loaded with consecutive threads on consecutive latitudes, plus the block's
+// latitude weights.
 template <int NH>
-__device__ __forceinline__ void oz_gprep_body(const OzArgs &a, int pb, int im, int fg) {
-    constexpr int TPF = 16 * NH;  // threads per field
+struct GprepLds {
+    double2 F[4][2][256 * NH];
+    double w[256 * NH];
+};
+
+template <int NH>
+__device__ __forceinline__ void oz_gprep_body(const OzArgs &a, int pb, int im, int fg, GprepLds<NH> &L) {
+    constexpr int TPF = 16 * NH, NP = 256 * NH;  // threads per field, pairs per block
     const int tid = threadIdx.x, jj = tid % TPF, fl = tid / TPF;
     const int f = fg * OZ_FIELDS + fl;
-    const bool fok = f < a.nf;
-    const double2 *Fi = a.F0 + ((size_t)(fok ? f : 0) * a.nm + im) * a.nlat;
     const int ksl = jj >> 2, g = jj & 3;
     double v[2][2][16];  // [parity][re, im][4 j + d]: byte j of word d
     double mx[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    {
+        const int p = pb * NP + tid;
+        L.w[tid] = p < a.np ? a.wlat[p] : 0.0;
+    }
+    // four rounds of four fields: the rows go through LDS so that the global reads
+    // are coalesced (each thread then needs 16 consecutive pairs of its field)
+    for (int rd = 0; rd < 4; ++rd) {
+        __syncthreads();  // the previous round's rows are consumed
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            const int p = pb * 256 * NH + oz_pair_of_slot<NH>(64 * ksl + 16 * g + 4 * d + j);
-            double2 FN = make_double2(0.0, 0.0), FS = make_double2(0.0, 0.0);
-            double wl = 0.0;
-            if (fok && p < a.np) {
-                FN = Fi[p];
+        for (int i = 0; i < 8; ++i) {
+            const int e = tid + NP * i, fr = e / (2 * NP), ns = (e / NP) & 1, pl = e % NP;
+            const int ff = fg * OZ_FIELDS + 4 * rd + fr, p = pb * NP + pl;
+            double2 val = make_double2(0.0, 0.0);
+            if (ff < a.nf && p < a.np) {
                 const int ps = a.nlat - 1 - p;
-                if (ps != p) FS = Fi[ps];
-                wl = a.wlat[p];
+                if (!ns || ps != p) val = a.F0[((size_t)ff * a.nm + im) * a.nlat + (ns ? ps : p)];
             }
-            const int i = 4 * j + d;
-            v[0][0][i] = wl * (FN.x + FS.x);
-            v[0][1][i] = wl * (FN.y + FS.y);
-            v[1][0][i] = wl * (FN.x - FS.x);
-            v[1][1][i] = wl * (FN.y - FS.y);
-#pragma unroll
-            for (int par = 0; par < 2; ++par)
-#pragma unroll
-                for (int ri = 0; ri < 2; ++ri) mx[par][ri] = fmax(mx[par][ri], fabs(v[par][ri][i]));
+            L.F[fr][ns][pl] = val;
         }
+        __syncthreads();
+        if ((fl >> 2) != rd) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const int pl = oz_pair_of_slot<NH>(64 * ksl + 16 * g + 4 * d + j);
+                const double2 FN = L.F[fl & 3][0][pl], FS = L.F[fl & 3][1][pl];
+                const double wl = L.w[pl];
+                const int i = 4 * j + d;
+                v[0][0][i] = wl * (FN.x + FS.x);
+                v[0][1][i] = wl * (FN.y + FS.y);
+                v[1][0][i] = wl * (FN.x - FS.x);
+                v[1][1][i] = wl * (FN.y - FS.y);
+#pragma unroll
+                for (int par = 0; par < 2; ++par)
+#pragma unroll
+                    for (int ri = 0; ri < 2; ++ri) mx[par][ri] = fmax(mx[par][ri], fabs(v[par][ri][i]));
+            }
+    }
     const int ks = pb * 4 * NH + ksl;
     const size_t ubase = ((size_t)fg * a.nm + im) * a.nks;
     v4i *dst = (v4i *)a.gdig + (ubase + ks) * OZ_CHUNK_V4;
@@ -531,7 +554,8 @@ __device__ __forceinline__ void oz_gprep_body(const OzArgs &a, int pb, int im, i
 
 template <int NH>
 __global__ __launch_bounds__(256 * NH) void k_oz_gprep(const OzArgs a) {
-    oz_gprep_body<NH>(a, blockIdx.x % a.npb, (blockIdx.x / a.npb) % a.nm, blockIdx.x / (a.npb * a.nm));
+    __shared__ GprepLds<NH> L;
+    oz_gprep_body<NH>(a, blockIdx.x % a.npb, (blockIdx.x / a.npb) % a.nm, blockIdx.x / (a.npb * a.nm), L);
 }
 
 // ---------------------------------------------------------------------------
@@ -811,7 +835,8 @@ __global__ __launch_bounds__(256 * NH) void k_oz_gprep_grp(const OzGroups G) {
     const int gi = oz_group_of(G, blockIdx.x);
     const OzArgs &a = G.args[gi];
     const int b = blockIdx.x - G.prefix[gi];
-    oz_gprep_body<NH>(a, b % a.npb, (b / a.npb) % a.nm, b / (a.npb * a.nm));
+    __shared__ GprepLds<NH> L;
+    oz_gprep_body<NH>(a, b % a.npb, (b / a.npb) % a.nm, b / (a.npb * a.nm), L);
 }
 
 __global__ void k_oz_reset(int *counter) {

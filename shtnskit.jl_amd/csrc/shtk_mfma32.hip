@@ -210,6 +210,10 @@ __global__ __launch_bounds__(256) void k_f32_gprep(const OzArgs a) {
     const double wl = act ? a.wlat[p] : 0.0;
     const int ps = a.nlat - 1 - p;
     float *base = a.gf32 + ((((size_t)fg * a.nm + im) * a.npb + pb) * a.nh + kh) * 4 * 4096;
+    // all 32 columns of this pair per parity in registers, then written as the
+    // pair's two 16-column runs (consecutive threads: consecutive 64-byte runs)
+    float v[2][OZ_COLS];
+#pragma unroll
     for (int fl = 0; fl < OZ_FIELDS; ++fl) {
         const int f = fg * OZ_FIELDS + fl;
         double2 FN = make_double2(0.0, 0.0), FS = make_double2(0.0, 0.0);
@@ -218,15 +222,21 @@ __global__ __launch_bounds__(256) void k_f32_gprep(const OzArgs a) {
             FN = Fi[p];
             if (ps != p) FS = Fi[ps];
         }
-        const double v[2][2] = {{wl * (FN.x + FS.x), wl * (FN.y + FS.y)}, {wl * (FN.x - FS.x), wl * (FN.y - FS.y)}};
-#pragma unroll
-        for (int par = 0; par < 2; ++par)
-#pragma unroll
-            for (int ri = 0; ri < 2; ++ri) {
-                const int c = 2 * fl + ri;
-                base[((par * 2 + (c >> 4)) * 64 + j) * 64 + (c & 15) + 16 * gq] = (float)v[par][ri];
-            }
+        v[0][2 * fl] = (float)(wl * (FN.x + FS.x));
+        v[0][2 * fl + 1] = (float)(wl * (FN.y + FS.y));
+        v[1][2 * fl] = (float)(wl * (FN.x - FS.x));
+        v[1][2 * fl + 1] = (float)(wl * (FN.y - FS.y));
     }
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            float4 *dst = (float4 *)(base + ((par * 2 + ct) * 64 + j) * 64 + 16 * gq);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                dst[k] = make_float4(v[par][16 * ct + 4 * k], v[par][16 * ct + 4 * k + 1], v[par][16 * ct + 4 * k + 2],
+                                     v[par][16 * ct + 4 * k + 3]);
+        }
 }
 
 // LDS position of pair q (0..255) of a K half: pairs 4 j + g are read four j at a

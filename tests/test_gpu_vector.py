@@ -179,3 +179,33 @@ def test_sphtor_zonal_lmax1023(shtk, oracle, cuda, robert):
     S2, T2 = shtk.analysis_sphtor(cfg, to_dev(Vt_r), to_dev(Vp_r))
     S_r, T_r = ocfg.analysis_sphtor(Vt_r, Vp_r)
     assert rel_l2(to_host(S2), S_r) < TOL and rel_l2(to_host(T2), T_r) < TOL
+
+
+@pytest.mark.parametrize("grid", ["gauss", "driscoll_healy"])
+def test_sphtor_zonal_segments_f32_and_truncation(shtk, oracle, cuda, grid):
+    """Order 0 runs in parallel 64-degree segments from plan-time start values:
+    a zonal-only field over several segments with FP32 coefficients, truncation
+    inside a segment (ltr 200 of lmax 300) and, on the Driscoll-Healy grid, the
+    missing south-pole row.  FP32 inputs: 1e-6; FP64: 1e-12."""
+    lmax, ltr = 300, 200
+    if grid == "gauss":
+        nlat, nlon = 302, 608
+        cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+        ocfg = oracle.Config(lmax, nlat, nlon=nlon)
+    else:
+        nlat, nlon = 2 * (lmax + 1), 608
+        cfg = shtk.create_regular_config(lmax, nlat, nlon=nlon, include_poles=True, use_dh_weights=True)
+        ocfg = oracle.Config(lmax, nlat, nlon=nlon, grid="driscoll_healy")
+    S, T = _st(oracle, lmax, lmax, 57)
+    S[:, 1:] = 0
+    T[:, 1:] = 0
+    S32, T32 = S.astype(np.complex64), T.astype(np.complex64)
+    Vt_r, Vp_r = ocfg.synthesis_sphtor(S32.astype(np.complex128), T32.astype(np.complex128))
+    Vt, Vp = shtk.synthesis_sphtor(cfg, to_dev(S32), to_dev(T32))
+    assert rel_l2(to_host(Vt), Vt_r) < 1e-6 and rel_l2(to_host(Vp), Vp_r) < 1e-6
+    Vt_r, Vp_r = ocfg.synthesis_sphtor(S, T, ltr=ltr)
+    Vt, Vp = shtk.synthesis_sphtor_l(cfg, to_dev(S), to_dev(T), ltr)
+    assert rel_l2(to_host(Vt), Vt_r) < TOL and rel_l2(to_host(Vp), Vp_r) < TOL
+    S_r, T_r = ocfg.analysis_sphtor(Vt_r, Vp_r, ltr=ltr)
+    S2, T2 = shtk.analysis_sphtor_l(cfg, to_dev(Vt_r), to_dev(Vp_r), ltr)
+    assert rel_l2(to_host(S2), S_r) < TOL and rel_l2(to_host(T2), T_r) < TOL

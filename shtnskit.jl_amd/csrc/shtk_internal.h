@@ -83,6 +83,8 @@ struct LegArgs {
     int qscale_in;          // scalar synthesis (64-degree segments): in0 holds canonical FP64
                             // coefficients, multiplied by c_l as they are loaded
     int vwaves;             // vector synthesis (64-degree segments): waves per block, 1 or 4
+    int s1w_units;          // scalar synthesis (64-degree segments): one wave per block when the
+                            // launch has at least this many (order, pair block, field) units
     int apblk;              // analysis: latitude pairs per work item (512, or 256 for the vector
                             // analysis when it lets the kernel run 3 waves per SIMD)
     int vdirect;            // vector synthesis (64-degree segments): in0 / in1 hold canonical FP64
@@ -171,6 +173,8 @@ struct shtk_plan {
     int xcd_map = 0, synth_chains = 4, fft_tile = 0;  // launch-shape knobs
     int vsynth_chains = 1;
     int vsynth_waves = 1;  // waves per vector synthesis block (SHTK_VSYNTH_WAVES: 1 or 4)
+    int ssynth_1w_units = 8192;  // scalar synthesis: units from which blocks hold one wave
+                                 // (SHTK_SSYNTH_1W_UNITS)
     int vanal_pblk = 256;  // pairs per vector analysis work item (SHTK_VANAL_PBLK: 256 or 512)  // latitude pairs per lane of the vector synthesis (1 or 2)
     int oz_nh_force = 0;  // split analysis pair halves per block (0: by grid size)
     bool canonical = true;

@@ -371,19 +371,23 @@ __device__ __forceinline__ void dsynth_seg(const SRec (&R)[4], cptr<double> qcm,
     }
 }
 
-// waves per k_synth_d workgroup, taking the segments round-robin (2 waves time the
-// same; 1 wave per workgroup leaves a long tail of low orders: 82 against 65 us)
+// waves per k_synth_d workgroup, taking the segments round-robin and adding
+// their sums through LDS (2 waves time the same; for one field, 1 wave per
+// workgroup leaves a long tail of low orders: 82 against 65 us)
 constexpr int DSW = 4;
 
-template <int K>
-__global__ __launch_bounds__(64 * DSW) void k_synth_d(const LegArgs a) {
-    __shared__ double red[DSW][4 * K][64];
+// NW = DSW: units order-major.  NW = 1 (launches with many units, i.e. batches
+// of fields): one wave per (order, pair block, field) runs all its segments,
+// units lowest order first for every field so the long columns start first.
+template <int K, int NW>
+__global__ __launch_bounds__(64 * NW) void k_synth_d(const LegArgs a) {
+    __shared__ double red[NW > 1 ? NW : 1][NW > 1 ? 4 * K : 1][64];
     int b, unit;
     if (!unit_of_block(a.xcd, a.gx, a.gy * a.gz, &unit, &b)) return;
-    const int im = unit % a.gy, f = unit / a.gy;
+    const int im = NW > 1 ? unit % a.gy : unit / a.gz, f = NW > 1 ? unit / a.gy : unit % a.gz;
     const int m = im * a.mres;
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = NW > 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0;
     const int np = a.np, lmax = a.lmax, ltr = a.ltr;
     const int pb0 = b * 64 * K, pb1 = min(pb0 + 64 * K, np);
     double x[K];
@@ -439,7 +443,7 @@ __global__ __launch_bounds__(64 * DSW) void k_synth_d(const LegArgs a) {
         }
     };
     if (w < nseg) load_next(w);
-    for (int s = w; s < nseg; s += DSW) {
+    for (int s = w; s < nseg; s += NW) {
         const int l0 = m + s * DSEG;
         if (l0 > ltr) break;
         const int pf = npf, pfa = npa;
@@ -454,19 +458,21 @@ __global__ __launch_bounds__(64 * DSW) void k_synth_d(const LegArgs a) {
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) R[g] = nr[g];
-        if (s + DSW < nseg) load_next(s + DSW);
+        if (s + NW < nseg) load_next(s + NW);
         if (pb1 <= pf) continue;
         const bool fast = pb0 >= pfa;
         const int lstop = min(l0 + DSEG, ltr + 1);
         if (fast) dsynth_seg<K, true>(R, qcm, l0, lstop, x, prev, cur, kk, acc);
         else dsynth_seg<K, false>(R, qcm, l0, lstop, x, prev, cur, kk, acc);
     }
+    if constexpr (NW > 1) {
 #pragma unroll
-    for (int j = 0; j < K; ++j)
+        for (int j = 0; j < K; ++j)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) red[w][4 * j + q][lane] = acc[j][q];
-    __syncthreads();
-    if (w != 0) return;
+            for (int q = 0; q < 4; ++q) red[w][4 * j + q][lane] = acc[j][q];
+        __syncthreads();
+        if (w != 0) return;
+    }
     const double gm = a.g * mfac_of(a.mfac_mode, m);
     double2 *Fo = a.F0 + ((size_t)f * a.nm + im) * a.nlat;
 #pragma unroll
@@ -476,10 +482,14 @@ __global__ __launch_bounds__(64 * DSW) void k_synth_d(const LegArgs a) {
         double t[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            double v = red[0][4 * j + q][lane];
+            if constexpr (NW > 1) {
+                double v = red[0][4 * j + q][lane];
 #pragma unroll
-            for (int ww = 1; ww < DSW; ++ww) v += red[ww][4 * j + q][lane];
-            t[q] = v;
+                for (int ww = 1; ww < NW; ++ww) v += red[ww][4 * j + q][lane];
+                t[q] = v;
+            } else {
+                t[q] = acc[j][q];
+            }
         }
         const double sc = gm * (a.wlat ? a.wlat[p] : 1.0);
         Fo[p] = make_double2((t[0] + t[2]) * sc, (t[1] + t[3]) * sc);
@@ -1565,8 +1575,10 @@ static void launch_synth_k(LegArgs a, int nfields, int vector, hipStream_t st) {
         else
             hipLaunchKernelGGL((k_synth_v<K>), grid, dim3(256), 0, st, a);
     } else {
-        if (a.seg == DSEG)
-            hipLaunchKernelGGL((k_synth_d<K>), grid, dim3(64 * DSW), 0, st, a);
+        if (a.seg == DSEG && (long long)a.gx * a.gy * a.gz >= a.s1w_units)
+            hipLaunchKernelGGL((k_synth_d<K, 1>), grid, dim3(64), 0, st, a);
+        else if (a.seg == DSEG)
+            hipLaunchKernelGGL((k_synth_d<K, DSW>), grid, dim3(64 * DSW), 0, st, a);
         else if (a.io_f32)
             hipLaunchKernelGGL((k_synth_s<K, float>), grid, dim3(256), 0, st, a);
         else

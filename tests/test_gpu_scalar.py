@@ -176,6 +176,31 @@ def test_launch_shapes(shtk, oracle, cuda, monkeypatch, chains, fft_pairs):
     cfg.close()
 
 
+@pytest.mark.parametrize("units", ["1", "1073741824"])
+@pytest.mark.parametrize("lmax,nlat,nlon,mmax,mres,ltr,nf", [
+    (300, 302, 640, None, 1, 290, 3),  # truncation inside a segment, 2 pair blocks
+    (64, 67, 160, 60, 3, -1, 2),       # mres > 1, odd nlat, mmax < lmax
+    (199, 200, 402, None, 1, -1, 5),
+])
+def test_synthesis_waves_per_block(shtk, oracle, cuda, monkeypatch, units, lmax, nlat, nlon, mmax, mres, ltr, nf):
+    """Scalar FP64 synthesis with blocks of one wave (each running all segments of
+    its order, field-minor unit order; units = 1) and of four waves adding their
+    segment sums in LDS (units above any launch) against the oracle, per field."""
+    monkeypatch.setenv("SHTK_SSYNTH_1W_UNITS", units)
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon, mmax=mmax, mres=mres)
+    ocfg = oracle.Config(lmax, nlat, nlon=nlon, mmax=mmax, mres=mres)
+    mm = lmax if mmax is None else mmax
+    alm = oracle.random_alm(lmax, mm, seed=lmax + nf, power=1.0, nf=nf)
+    for m in range(mm + 1):
+        if m % mres:
+            alm[:, m] = 0
+    f_ref = ocfg.synthesis(alm, ltr=ltr)
+    f = to_host(shtk.synthesis_batch(cfg, to_dev(alm), ltr=ltr))
+    for k in range(nf):
+        assert rel_l2(f[..., k], f_ref[..., k]) < TOL, k
+    cfg.close()
+
+
 @pytest.mark.parametrize("lmax,nlat,nlon,mmax,mres,nf", [
     (3, 8, 10, None, 1, 1),
     (40, 42, 96, None, 1, 3),

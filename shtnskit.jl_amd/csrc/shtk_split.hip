@@ -415,11 +415,13 @@ __global__ __launch_bounds__(256) void k_oz_asc(const OzAscArgs a) {
 // pairs as one dword (lane quads exchange bytes through DPP).
 //
 // k_oz_gprep: one workgroup of 256 NH threads per (pair block, order, field
-// group).  Thread (jj = tid % 16 NH, field tid / 16 NH) holds the 16 slots of
-// one B lane word (kstep jj >> 2 of the block, lane group jj & 3), so the
-// Fourier data is read once and each digit word is written once from
-// registers; the column maximum is a shuffle reduction over the 16 NH threads
-// of a field and the kstep column sums one over 4 threads.
+// group), taking the group's fields in four rounds of four (their north and
+// south rows staged in LDS with coalesced reads, the next round's reads issued
+// before the current round's digits).  In a round, thread (field, parity,
+// re/im, jj = tid % 16 NH) holds the 16 slots of one B lane word of one column
+// (kstep jj >> 2 of the block, lane group jj & 3), so each digit word is
+// written once from registers; the column maximum is a shuffle reduction over
+// the 16 NH threads of a column and the kstep column sums one over 4 threads.
 //
 // Digits: with kMagicB = 2^52 + 0x808080808080, fma(v, 2^(46-e), kMagicB)
 // leaves Y + 0x808080808080 in the low 48 mantissa bits (Y = round(v 2^(46-e)),
@@ -457,99 +459,101 @@ __device__ __forceinline__ unsigned quad_byte_transpose(unsigned v, unsigned sel
 
 // LDS staging of the Fourier rows of 4 fields of a pair block (north, south),
 // loaded with consecutive threads on consecutive latitudes, plus the block's
-// latitude weights.
+// latitude weights.  Pair pl sits at gp_slot(pl) = pl + pl / 16: the threads of
+// a column read pairs 16 apart, which the padding spreads over all banks.
+__device__ __forceinline__ int gp_slot(int pl) { return pl + (pl >> 4); }
+
 template <int NH>
 struct GprepLds {
-    double2 F[4][2][256 * NH];
-    double w[256 * NH];
+    static constexpr int ROW = 256 * NH + 16 * NH;
+    double2 F[4][2][ROW];
+    double w[ROW];
 };
 
 template <int NH>
 __device__ __forceinline__ void oz_gprep_body(const OzArgs &a, int pb, int im, int fg, GprepLds<NH> &L) {
-    constexpr int TPF = 16 * NH, NP = 256 * NH;  // threads per field, pairs per block
-    const int tid = threadIdx.x, jj = tid % TPF, fl = tid / TPF;
-    const int f = fg * OZ_FIELDS + fl;
+    constexpr int TPF = 16 * NH, NP = 256 * NH;  // threads per column, pairs per block
+    // four rounds of four fields; in each, thread (field fr of the round, parity
+    // par, re/im ri, word jj) builds one B lane word of one column, while the
+    // next round's rows are in flight
+    const int tid = threadIdx.x, jj = tid % TPF, combo = tid / TPF;
+    const int fr = combo >> 2, par = (combo >> 1) & 1, ri = combo & 1;
     const int ksl = jj >> 2, g = jj & 3;
-    double v[2][2][16];  // [parity][re, im][4 j + d]: byte j of word d
-    double mx[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    const int ks = pb * 4 * NH + ksl;
+    const size_t ubase = ((size_t)fg * a.nm + im) * a.nks;
+    v4i *dst = (v4i *)a.gdig + (ubase + ks) * OZ_CHUNK_V4;
     {
         const int p = pb * NP + tid;
-        L.w[tid] = p < a.np ? a.wlat[p] : 0.0;
+        L.w[gp_slot(tid)] = p < a.np ? a.wlat[p] : 0.0;
     }
-    // four rounds of four fields: the rows go through LDS so that the global reads
-    // are coalesced (each thread then needs 16 consecutive pairs of its field)
-    for (int rd = 0; rd < 4; ++rd) {
-        __syncthreads();  // the previous round's rows are consumed
+    double2 ld[8];
+    auto issue = [&](int rd) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = tid + NP * i, fr = e / (2 * NP), ns = (e / NP) & 1, pl = e % NP;
-            const int ff = fg * OZ_FIELDS + 4 * rd + fr, p = pb * NP + pl;
+            const int e = tid + NP * i, fe = e / (2 * NP), ns = (e / NP) & 1, pl = e % NP;
+            const int ff = fg * OZ_FIELDS + 4 * rd + fe, p = pb * NP + pl;
             double2 val = make_double2(0.0, 0.0);
             if (ff < a.nf && p < a.np) {
                 const int ps = a.nlat - 1 - p;
                 if (!ns || ps != p) val = a.F0[((size_t)ff * a.nm + im) * a.nlat + (ns ? ps : p)];
             }
-            L.F[fr][ns][pl] = val;
+            ld[i] = val;
+        }
+    };
+    issue(0);
+    for (int rd = 0; rd < 4; ++rd) {
+        __syncthreads();  // the previous round's rows are consumed
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = tid + NP * i;
+            L.F[e / (2 * NP)][(e / NP) & 1][gp_slot(e % NP)] = ld[i];
         }
         __syncthreads();
-        if ((fl >> 2) != rd) continue;
+        if (rd < 3) issue(rd + 1);
+        // G+- = w (F_N +- F_S) of 16 consecutive pairs: word d, byte j = pair 4 d + j
+        double v[16], mx = 0.0;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
-                const int pl = oz_pair_of_slot<NH>(64 * ksl + 16 * g + 4 * d + j);
-                const double2 FN = L.F[fl & 3][0][pl], FS = L.F[fl & 3][1][pl];
-                const double wl = L.w[pl];
-                const int i = 4 * j + d;
-                v[0][0][i] = wl * (FN.x + FS.x);
-                v[0][1][i] = wl * (FN.y + FS.y);
-                v[1][0][i] = wl * (FN.x - FS.x);
-                v[1][1][i] = wl * (FN.y - FS.y);
-#pragma unroll
-                for (int par = 0; par < 2; ++par)
-#pragma unroll
-                    for (int ri = 0; ri < 2; ++ri) mx[par][ri] = fmax(mx[par][ri], fabs(v[par][ri][i]));
+                const int pl = gp_slot(oz_pair_of_slot<NH>(64 * ksl + 16 * g + 4 * d + j));
+                const double2 FN = L.F[fr][0][pl], FS = L.F[fr][1][pl];
+                const double fn = ri ? FN.y : FN.x, fs = ri ? FS.y : FS.x;
+                const double x = L.w[pl] * (par ? fn - fs : fn + fs);
+                v[4 * j + d] = x;
+                mx = fmax(mx, fabs(x));
             }
-    }
-    const int ks = pb * 4 * NH + ksl;
-    const size_t ubase = ((size_t)fg * a.nm + im) * a.nks;
-    v4i *dst = (v4i *)a.gdig + (ubase + ks) * OZ_CHUNK_V4;
 #pragma unroll
-    for (int par = 0; par < 2; ++par)
+        for (int msk = 1; msk < TPF; msk <<= 1) mx = fmax(mx, __shfl_xor(mx, msk));
+        const int e = mx > 0.0 ? ilogb(mx) + 1 : 0;
+        const int c = 2 * (4 * rd + fr) + ri;
+        if (jj == 0) a.eg[((((size_t)fg * a.nm + im) * a.npb + pb) * 2 + par) * OZ_COLS + c] = e;
+        const double sc = ldexp(1.0, 46 - e);
+        unsigned hw[16], lw[16];
+        double ys = 0.0;
 #pragma unroll
-        for (int ri = 0; ri < 2; ++ri) {
-            double m = mx[par][ri];
-#pragma unroll
-            for (int msk = 1; msk < TPF; msk <<= 1) m = fmax(m, __shfl_xor(m, msk));
-            const int e = m > 0.0 ? ilogb(m) + 1 : 0;
-            const int c = 2 * fl + ri;
-            if (jj == 0) a.eg[((((size_t)fg * a.nm + im) * a.npb + pb) * 2 + par) * OZ_COLS + c] = e;
-            const double sc = ldexp(1.0, 46 - e);
-            unsigned hw[16], lw[16];
-            double ys = 0.0;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const double r = fma(v[par][ri][i], sc, kMagicB);
-                ys += r - kMagicB;
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(r);
-                hw[i] = (unsigned)(bits >> 32) ^ 0x8080u;
-                lw[i] = (unsigned)bits ^ 0x80808080u;
-            }
-            ys += __shfl_xor(ys, 1);
-            ys += __shfl_xor(ys, 2);
-            if (g == 0) a.gcolsum[((ubase + ks) * 2 + par) * OZ_COLS + c] = ys;
-            // word d of the lane holds bytes j = 0..3 (j in byte j)
-            v4i *row = dst + ((par * OZ_SLICES) * 2 + (c >> 4)) * 64 + (c & 15) + 16 * g;
-#pragma unroll
-            for (int t = 0; t < OZ_SLICES; ++t) {
-                v4i wd;
-#pragma unroll
-                for (int d = 0; d < 4; ++d)
-                    wd[d] = (int)(t < 2 ? pack4(hw[d], hw[4 + d], hw[8 + d], hw[12 + d], 1 - t)
-                                        : pack4(lw[d], lw[4 + d], lw[8 + d], lw[12 + d], 5 - t));
-                row[t * 2 * 64] = wd;
-            }
+        for (int i = 0; i < 16; ++i) {
+            const double r = fma(v[i], sc, kMagicB);
+            ys += r - kMagicB;
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(r);
+            hw[i] = (unsigned)(bits >> 32) ^ 0x8080u;
+            lw[i] = (unsigned)bits ^ 0x80808080u;
         }
+        ys += __shfl_xor(ys, 1);
+        ys += __shfl_xor(ys, 2);
+        if (g == 0) a.gcolsum[((ubase + ks) * 2 + par) * OZ_COLS + c] = ys;
+        // word d of the lane holds bytes j = 0..3 (j in byte j)
+        v4i *row = dst + ((par * OZ_SLICES) * 2 + (c >> 4)) * 64 + (c & 15) + 16 * g;
+#pragma unroll
+        for (int t = 0; t < OZ_SLICES; ++t) {
+            v4i wd;
+#pragma unroll
+            for (int d = 0; d < 4; ++d)
+                wd[d] = (int)(t < 2 ? pack4(hw[d], hw[4 + d], hw[8 + d], hw[12 + d], 1 - t)
+                                    : pack4(lw[d], lw[4 + d], lw[8 + d], lw[12 + d], 5 - t));
+            row[t * 2 * 64] = wd;
+        }
+    }
 }
 
 template <int NH>

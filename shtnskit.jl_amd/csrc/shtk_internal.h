@@ -270,6 +270,10 @@ namespace shtk {
 constexpr int OZ_SLICES = 6;
 constexpr int OZ_COLS = 32;
 constexpr int OZ_FIELDS = OZ_COLS / 2;
+// LDS row stride (double2) of the matrix-core synthesis kernels' staged recurrence
+// coefficients, one row of 32 degrees per lane group: 33 puts the four rows a wave
+// reads at once (one broadcast address per lane group) on different banks.
+constexpr int OZ_ABS = 33;
 constexpr int OZ_CHUNK_V4 = 2 * OZ_SLICES * 2 * 64;  // 16-byte B operands per chunk (1536 = 24 KB)
 
 struct OzArgs {

@@ -116,7 +116,7 @@ __device__ __forceinline__ void synth_f32_body(const OzArgs &a, int pbk, int im,
             const int l32 = e & 31, gg = (e >> 5) & 3, fl = e >> 7;
             stg[fl * F32_FS + gg * F32_GS + l32] = make_float2((float)nv[k].x, (float)nv[k].y);
         }
-        if (tid < 128) abch[tid] = nab;
+        if (tid < 128) abch[(tid >> 5) * OZ_ABS + (tid & 31)] = nab;
         if (rho + 1 < nch) load_chunk(rho + 1);
         __syncthreads();
         // a lane still in the scaled range contributes zeros (scale 0); so does a
@@ -136,7 +136,7 @@ __device__ __forceinline__ void synth_f32_body(const OzArgs &a, int pbk, int im,
                     const float bv = stf[2 * ((c >> 1) * F32_FS + g * F32_GS + i) + (c & 1)];
                     acc[i & 1][ct] = mfma4(af, bv, acc[i & 1][ct]);
                 }
-                const double2 co = abch[g * 32 + i];
+                const double2 co = abch[g * OZ_ABS + i];
                 const double nx = fma(co.x * x, cur, -(co.y * prev));
                 prev = cur;
                 cur = nx;
@@ -185,7 +185,7 @@ __device__ __forceinline__ void synth_f32_body(const OzArgs &a, int pbk, int im,
 template <typename IT>
 __global__ __launch_bounds__(256) void k_synth_f32(const OzArgs a) {
     __shared__ float2 stg[OZ_FIELDS * F32_FS];
-    __shared__ double2 abch[4 * 32];
+    __shared__ double2 abch[4 * OZ_ABS];
     const int gx = (a.np + 63) / 64;
     int pbk, unit;
     if (!unit_of_block(a.xcd, gx, a.nm * a.nfg, &unit, &pbk)) return;

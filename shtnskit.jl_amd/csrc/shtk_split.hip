@@ -284,7 +284,7 @@ __device__ __forceinline__ void synth_oz_body(const OzArgs &a, int pbk, int im, 
     fetch(0);
     for (int rho = 0; rho < nch; ++rho) {
         __syncthreads();
-        if (tid < 128) abch[tid] = abnext;
+        if (tid < 128) abch[(tid >> 5) * OZ_ABS + (tid & 31)] = abnext;
 #pragma unroll
         for (int k = 0; k < NB; ++k) bch[tid + 256 * k] = bnext[k];
         __syncthreads();
@@ -304,7 +304,7 @@ __device__ __forceinline__ void synth_oz_body(const OzArgs &a, int pbk, int im, 
                 const unsigned long long bits = (unsigned long long)__double_as_longlong(fma(cur, sl, kMagic));
                 h[q] = (unsigned)(bits >> 32) ^ 0x8080u;
                 lo[q] = (unsigned)bits ^ 0x80808080u;
-                const double2 co = abch[g * 32 + i];
+                const double2 co = abch[g * OZ_ABS + i];
                 const double nx = fma(co.x * x, cur, -(co.y * prev));
                 prev = cur;
                 cur = nx;
@@ -375,7 +375,7 @@ __device__ __forceinline__ void synth_oz_body(const OzArgs &a, int pbk, int im, 
 }
 
 __global__ __launch_bounds__(256) void k_synth_oz(const OzArgs a) {
-    __shared__ double2 abch[4 * 32];
+    __shared__ double2 abch[4 * OZ_ABS];
     __shared__ v4i bch[OZ_CHUNK_V4];
     const int gx = (a.np + 63) / 64;
     int pbk, unit;
@@ -850,7 +850,7 @@ __global__ void k_oz_reset(int *counter) {
 // Persistent synthesis: unit u of group gi is (pair block, order, field group)
 // with the pair block fastest, as the single-configuration grid.
 __global__ __launch_bounds__(256) void k_synth_oz_grp(const OzGroups G) {
-    __shared__ double2 abch[4 * 32];
+    __shared__ double2 abch[4 * OZ_ABS];
     __shared__ v4i bch[OZ_CHUNK_V4];
     __shared__ int s_unit;
     for (;;) {

@@ -437,12 +437,14 @@ __device__ __forceinline__ int oz_pair_of_slot(int q) {
 }
 
 // LDS of the analysis A digits: [parity][slice][row 16][256 NH pair bytes + 16],
-// slices padded by 64 bytes so that the four slices a lane quad writes at once
-// fall on two bank groups 16 dwords apart (two lanes per bank, the minimum).
+// slices padded by 160 bytes (40 dwords): the four slices a lane quad writes at
+// once then start 8 banks apart (modulo 32) and 24 apart (modulo 64), so the
+// 16 quads x 4 slices of a wave's digit store hit every bank once, and the two
+// high-word slices of the parity pair likewise.
 template <int NH>
 struct OzLds {
     static constexpr int ROWB = 256 * NH + 16;          // row stride (bytes)
-    static constexpr int SLS = 16 * ROWB + 64;          // slice stride
+    static constexpr int SLS = 16 * ROWB + 160;         // slice stride
     static constexpr int PS = OZ_SLICES * SLS;          // parity stride
     static constexpr int BYTES = 2 * PS;
 };

@@ -305,6 +305,8 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
     extern __shared__ double2 zs[];
     constexpr int L = 2 * G;
     const int N = a.nlon, nlat = a.nlat, RS = a.nfft + (a.nfft >> 4) + 1;
+    // orders 0 .. N/2 - 1 in steps of 1 fill every bin of the row except N / 2
+    const bool full = !a.blue && a.mres == 1 && 2 * a.nm == N;
     const int tid = threadIdx.x, nt = blockDim.x, tr = nt / G;
     const int row = tid / tr, lt = tid % tr;
     const int k = tid % L;                // lane role within its group of 2G
@@ -316,8 +318,14 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
         if (ug >= T.total) break;
         const int f = ug / T.ntile, lat = (ug % T.ntile) * L + k;
         const bool live = lat < nlat;
-        for (int q = tid; q < G * RS; q += nt) zs[q] = make_double2(0.0, 0.0);
-        __syncthreads();
+        if (full) {
+            // every bin but N / 2 is written below (the previous tile's barrier
+            // ordered its reads before these writes)
+            if (tid < G) zs[tid * RS + zp(N / 2)] = make_double2(0.0, 0.0);
+        } else {
+            for (int q = tid; q < G * RS; q += nt) zs[q] = make_double2(0.0, 0.0);
+            __syncthreads();
+        }
         // packed rows: lane k < G writes z[m], its partner z[N - m]
         const double2 *Ff = a.F + (size_t)f * a.nm * nlat + lat;
         double2 *z = zs + gk * RS;
@@ -353,10 +361,11 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
         GT *gf = (GT *)a.grid + (size_t)f * nrow * N + lat;
         if (live && lat < nrow) {
             const double sc = fft_row_scale(a, lat);
+            // this lane's latitude is the real (k < G) or imaginary part of the packed row
+            const double *zr = (const double *)z + (k < G ? 0 : 1);
             for (int e = tid; e < N * L; e += nt) {
                 const int j = e / L;
-                const double2 v = z[zp(j)];
-                gf[(size_t)j * nrow] = (GT)((k < G ? v.x : v.y) * sc);
+                gf[(size_t)j * nrow] = (GT)(zr[2 * zp(j)] * sc);
             }
         }
         __syncthreads();

@@ -474,6 +474,8 @@ __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfield
         const bool has_row = live && lat < nrow;  // rows past nrow read as zero
         const GT *gf = (const GT *)a.grid + (size_t)f * nrow * N + lat;
         double2 *z = zs + gk * RS;
+        // this lane's latitude is the real (k < G) or imaginary part of the packed row
+        double *zr = (double *)z + (k < G ? 0 : 1);
         constexpr int PF = 8;  // loads in flight per thread
         for (int e0 = 0; e0 < N * L; e0 += PF * nt) {
             double ld[PF];
@@ -485,9 +487,7 @@ __global__ __launch_bounds__(1024) void k_fft_anal_t(const FftArgs a, int nfield
 #pragma unroll
             for (int q = 0; q < PF; ++q) {
                 const int j = (e0 + tid + q * nt) / L;
-                const double mine = ld[q] * sc;
-                const double other = __shfl_xor(mine, L - 1);
-                if (k < G && j < N) z[zp(j)] = make_double2(mine, other);
+                if (j < N) zr[2 * zp(j)] = ld[q] * sc;
             }
         }
         __syncthreads();

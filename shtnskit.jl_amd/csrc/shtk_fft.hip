@@ -273,6 +273,19 @@ __device__ void fft_lds(double2 *z, const FftArgs &a, int tid, int nt) {
 // stride of the XCD's workgroup count, so neighbouring tiles (which share
 // cache lines) run on one XCD at the same time.
 // ---------------------------------------------------------------------------
+// Value of lane k ^ (L - 1) within each group of L = 2, 4 or 8 lanes (the other
+// latitude of a packed row), through DPP: quad_perm [1,0,3,2], quad_perm
+// [3,2,1,0] or row_half_mirror.
+template <int L>
+__device__ __forceinline__ double tile_partner(double v) {
+    static_assert(L == 2 || L == 4 || L == 8, "tile of 2, 4 or 8 latitudes");
+    constexpr int ctrl = L == 2 ? 0xB1 : L == 4 ? 0x1B : 0x141;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), ctrl, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 struct FftTile {
     int total, per, nslot, ntile;
 };
@@ -341,7 +354,7 @@ __global__ __launch_bounds__(1024) void k_fft_synth_t(const FftArgs a, int nfiel
             for (int q = 0; q < PF; ++q) {
                 const int im = (e0 + tid + q * nt) / L;
                 const double2 mine = ld[q];
-                const double2 other = make_double2(__shfl_xor(mine.x, L - 1), __shfl_xor(mine.y, L - 1));
+                const double2 other = make_double2(tile_partner<L>(mine.x), tile_partner<L>(mine.y));
                 const double2 Fa = k < G ? mine : other, Fb = k < G ? other : mine;
                 const int m = im * a.mres;
                 if (im < a.nm) {

@@ -136,3 +136,43 @@ def test_split_analysis_conventions_ltr_f32(shtk, oracle, cuda):
     b = to_host(shtk.analysis_batch(cfg2, to_dev(f), precision="split", ltr=25))
     assert rel_l2(b, a) < TOL_SPLIT
     assert np.all(b[26:] == 0)
+
+
+def test_matrix_core_paths_with_two_pair_blocks(shtk, oracle, cuda):
+    """nlat = 1536: 768 latitude pairs, so the analyses contract two 512-pair
+    blocks per order and add them into outputs cleared first.  Split (1e-11)
+    and FP32-MFMA (2e-6) scalar and vector transforms against the plain FP64
+    path per field, the split scalar analysis of one field also against the
+    oracle."""
+    import torch
+
+    lmax, nlat, nlon, nf = 1535, 1536, 3072, 3
+    cfg = shtk.create_gauss_config(lmax, nlat, nlon=nlon)
+    alm = oracle.random_alm(lmax, lmax, seed=21, power=1.5, nf=nf)
+    da = to_dev(alm)
+    f64 = shtk.synthesis_batch(cfg, da)
+    a64 = shtk.analysis_batch(cfg, f64)
+    S = oracle.random_alm(lmax, lmax, 22, 2.0, nf=nf, lmin=1)
+    T = oracle.random_alm(lmax, lmax, 23, 2.0, nf=nf, lmin=1)
+    dS, dT = to_dev(S), to_dev(T)
+    Vt64, Vp64 = shtk.synthesis_sphtor_batch(cfg, dS, dT)
+    S64, T64 = shtk.analysis_sphtor_batch(cfg, Vt64, Vp64)
+
+    def worst(x, ref):
+        x, ref = to_host(x), to_host(ref)
+        return max(_per_field(x, ref))
+
+    for prec, tol in (("split", TOL_SPLIT), ("fp32", 2e-6)):
+        e_syn = worst(shtk.synthesis_batch(cfg, da, precision=prec), f64)
+        e_ana = worst(shtk.analysis_batch(cfg, f64, precision=prec), a64)
+        Vt, Vp = shtk.synthesis_sphtor_batch(cfg, dS, dT, precision=prec)
+        Sx, Tx = shtk.analysis_sphtor_batch(cfg, Vt64, Vp64, precision=prec)
+        e_v = max(worst(Vt, Vt64), worst(Vp, Vp64), worst(Sx, S64), worst(Tx, T64))
+        print(f"{prec} lmax={lmax}: synthesis {e_syn:.2e} analysis {e_ana:.2e} sphtor {e_v:.2e}", flush=True)
+        assert e_syn < tol and e_ana < tol and e_v < tol
+    torch.cuda.synchronize()
+    f1 = np.asfortranarray(to_host(f64)[..., 1])
+    ref = oracle.Config(lmax, nlat, nlon=nlon).analysis(f1)
+    got = to_host(shtk.analysis_batch(cfg, f64, precision="split"))[..., 1]
+    assert rel_l2(got, ref) < TOL_SPLIT
+    cfg.close()

@@ -449,12 +449,14 @@ struct OzLds {
     static constexpr int BYTES = 2 * PS;
 };
 
-// 4x4 byte transpose inside each lane quad: lane q returns byte q of the
-// words of lanes 4j .. 4j+3 (in lane order).  Two DPP exchanges (lane ^ 1,
-// lane ^ 2) and two byte permutes.
-__device__ __forceinline__ unsigned quad_byte_transpose(unsigned v, unsigned sel1, unsigned sel2) {
+// 4x4 byte transpose inside each lane quad: lane q ends with byte q of the
+// words of lanes 4j .. 4j+3 (in lane order), in two halves, each a DPP
+// exchange (lane ^ 1, then lane ^ 2) and a byte permute.
+__device__ __forceinline__ unsigned qbt_first(unsigned v, unsigned sel1) {
     const unsigned x = (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
-    const unsigned b = __builtin_amdgcn_perm(x, v, sel1);
+    return __builtin_amdgcn_perm(x, v, sel1);
+}
+__device__ __forceinline__ unsigned qbt_second(unsigned b, unsigned sel2) {
     const unsigned y = (unsigned)__builtin_amdgcn_mov_dpp((int)b, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
     return __builtin_amdgcn_perm(y, b, sel2);
 }
@@ -578,25 +580,43 @@ __global__ __launch_bounds__(256 * NH) void k_oz_gprep(const OzArgs a) {
 // ---------------------------------------------------------------------------
 // Degree I of the chunk: digits of P_l 2^(46-epl) into the LDS rows, the Q-form
 // step, and after every 8th degree the scale check of a chain still in the
-// scaled range.
+// scaled range.  The quad byte transpose of a digit word is spread over the two
+// following degrees (its first DPP exchange one degree after the word is
+// formed, its second one degree later), so no DPP read follows the VALU write
+// of its source directly and the wave needs no wait states for them; I = 32
+// drains the last two degrees.
 template <int I, int NH>
 __device__ __forceinline__ void oz_digit_steps(const OzRec &R, double x, double &prev, double &cur, int &kk,
                                                double &live, unsigned char *aul, unsigned char *auh, unsigned sel1,
-                                               unsigned sel2, unsigned &uh_even) {
+                                               unsigned sel2, unsigned &uh_even, unsigned &w1, unsigned &w1h,
+                                               unsigned &b2, unsigned &b2h) {
+    constexpr int ROWB = OzLds<NH>::ROWB, PS = OzLds<NH>::PS;
+    // second half of the transposes of degree I - 2's words: store them
+    if constexpr (I >= 2) {
+        constexpr int D = I - 2;
+        *(unsigned *)(aul + (D & 1) * PS + (D >> 1) * ROWB) = qbt_second(b2, sel2) ^ 0x80808080u;
+        if constexpr (D & 1) *(unsigned *)(auh + (D >> 1) * ROWB) = qbt_second(b2h, sel2) ^ 0x80808080u;
+    }
+    // first half of the transposes of degree I - 1's words
+    if constexpr (I >= 1 && I <= 32) {
+        constexpr int D = I - 1;
+        b2 = qbt_first(w1, sel1);
+        if constexpr (D & 1) b2h = qbt_first(w1h, sel1);
+    }
     if constexpr (I < 32) {
-        constexpr int H = I >> 4, N = I & 15, ROWB = OzLds<NH>::ROWB, PS = OzLds<NH>::PS;
+        constexpr int H = I >> 4, N = I & 15;
         // U = 2^47 + round(P 2^(46-epl)) lands in the low 48 bits of the mantissa
         double dm = kMagic;
         const double cl = cur * live;
         SHTK_FMAC_BC(dm, R.a[H], cl, N, "");
         const unsigned long long bits = (unsigned long long)__double_as_longlong(dm);
-        const unsigned uh = (unsigned)(bits >> 32), ul = (unsigned)bits;
-        // slices 2..5 (bytes 3..0 of the low word); XOR 0x80 per byte: balanced digits
-        *(unsigned *)(aul + (I & 1) * PS + (I >> 1) * ROWB) = quad_byte_transpose(ul, sel1, sel2) ^ 0x80808080u;
+        const unsigned uh = (unsigned)(bits >> 32);
+        // slices 2..5 (bytes 3..0 of the low word); the XOR 0x80 per byte at the
+        // store gives balanced digits
+        w1 = (unsigned)bits;
         if constexpr (I & 1) {
             // slices 0, 1 (bytes 1, 0 of the high word) of degrees I - 1 and I
-            const unsigned pk = __builtin_amdgcn_perm(uh, uh_even, 0x05040100u);
-            *(unsigned *)(auh + (I >> 1) * ROWB) = quad_byte_transpose(pk, sel1, sel2) ^ 0x80808080u;
+            w1h = __builtin_amdgcn_perm(uh, uh_even, 0x05040100u);
         } else {
             uh_even = uh;
         }
@@ -613,7 +633,10 @@ __device__ __forceinline__ void oz_digit_steps(const OzRec &R, double x, double 
                 live = kk == 0 ? 1.0 : 0.0;
             }
         }
-        oz_digit_steps<I + 1, NH>(R, x, prev, cur, kk, live, aul, auh, sel1, sel2, uh_even);
+        oz_digit_steps<I + 1, NH>(R, x, prev, cur, kk, live, aul, auh, sel1, sel2, uh_even, w1, w1h, b2, b2h);
+    } else if constexpr (I == 32) {
+        // degree 31: its second half right after the first
+        oz_digit_steps<33, NH>(R, x, prev, cur, kk, live, aul, auh, sel1, sel2, uh_even, w1, w1h, b2, b2h);
     }
 }
 
@@ -736,8 +759,8 @@ __device__ __forceinline__ void anal_oz_body(const OzArgs &a, const double *__re
             // that does not exist never uses them)
             load_rec(nrec, min(t0 + 32 < nst ? lc + 32 : m + (s0 + sr + 1) * a.seg, a.lmax));
             asm volatile("s_nop 4");  // wait states before DPP reads the refilled records
-            unsigned uh_even = 0;
-            oz_digit_steps<0, NH>(rec, x, prev, cur, kk, live, aul, auh, sel1, sel2, uh_even);
+            unsigned uh_even = 0, w1 = 0, w1h = 0, b2 = 0, b2h = 0;
+            oz_digit_steps<0, NH>(rec, x, prev, cur, kk, live, aul, auh, sel1, sel2, uh_even, w1, w1h, b2, b2h);
             __syncthreads();
             v4i acc[OZ_SLICES];
 #pragma unroll

@@ -105,19 +105,25 @@ constexpr int FFT_VPT = 8;
 // R Ns elements) over all LDS banks.
 __device__ __forceinline__ int zp(int i) { return i + (i >> 4); }
 
-template <int R, int SIGN>
+// P2: Ns is a power of two, so the butterfly's position j splits into j / Ns and
+// j mod Ns by a shift and a mask (a division by a run-time value is a sequence
+// of some twenty instructions).
+template <int R, int SIGN, bool P2>
 __device__ __forceinline__ void stockham_pass(double2 *z, int N, int Ns, const double2 *tw, int tid, int nt) {
     // nt >= N / FFT_VPT (launch_fft), so a thread has at most ceil(FFT_VPT / R) butterflies
     constexpr int MAXB = (FFT_VPT + R - 1) / R;
     const int nb = N / R;
     const int stride = N / (Ns * R);
+    const int sh = __builtin_ctz((unsigned)Ns);
+    auto kmod = [&](int j) { return P2 ? (j & (Ns - 1)) : j % Ns; };
+    auto kdiv = [&](int j) { return P2 ? (j >> sh) : j / Ns; };
     // base twiddle w = W^(k stride) of each butterfly (the others are its
     // powers): independent of the LDS data, so its load overlaps the barrier
     double2 w1[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
         const int j = tid + q * nt;
-        w1[q] = (Ns > 1 && j < nb) ? tw[(j % Ns) * stride] : make_double2(1.0, 0.0);
+        w1[q] = (Ns > 1 && j < nb) ? tw[kmod(j) * stride] : make_double2(1.0, 0.0);
     }
     double2 v[MAXB][R];
 #pragma unroll
@@ -133,7 +139,7 @@ __device__ __forceinline__ void stockham_pass(double2 *z, int N, int Ns, const d
     for (int q = 0; q < MAXB; ++q) {
         const int j = tid + q * nt;
         if (j < nb) {
-            const int k = j % Ns;
+            const int k = kmod(j);
             if (Ns > 1) {
                 double2 w = w1[q];
                 if (SIGN > 0) w.y = -w.y;
@@ -145,7 +151,7 @@ __device__ __forceinline__ void stockham_pass(double2 *z, int N, int Ns, const d
                 }
             }
             dft_small<R, SIGN>(v[q]);
-            const int base = (j / Ns) * Ns * R + k;
+            const int base = kdiv(j) * Ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) z[zp(base + r * Ns)] = v[q][r];
         }
@@ -202,11 +208,21 @@ __device__ void fft_passes(double2 *z, int N, int nrad, unsigned long long radix
     int Ns = 1;
     for (int q = 0; q < nrad; ++q) {
         const int R = (int)((radix >> (FFT_RADIX_BITS * q)) & ((1ull << FFT_RADIX_BITS) - 1));
+        const bool p2 = (Ns & (Ns - 1)) == 0;
         switch (R) {
-            case 8: stockham_pass<8, SIGN>(z, N, Ns, tw, tid, nt); break;
-            case 4: stockham_pass<4, SIGN>(z, N, Ns, tw, tid, nt); break;
-            case 2: stockham_pass<2, SIGN>(z, N, Ns, tw, tid, nt); break;
-            case 3: stockham_pass<3, SIGN>(z, N, Ns, tw, tid, nt); break;
+            case 8:
+                if (p2) stockham_pass<8, SIGN, true>(z, N, Ns, tw, tid, nt);
+                else stockham_pass<8, SIGN, false>(z, N, Ns, tw, tid, nt);
+                break;
+            case 4:
+                if (p2) stockham_pass<4, SIGN, true>(z, N, Ns, tw, tid, nt);
+                else stockham_pass<4, SIGN, false>(z, N, Ns, tw, tid, nt);
+                break;
+            case 2:
+                if (p2) stockham_pass<2, SIGN, true>(z, N, Ns, tw, tid, nt);
+                else stockham_pass<2, SIGN, false>(z, N, Ns, tw, tid, nt);
+                break;
+            case 3: stockham_pass<3, SIGN, false>(z, N, Ns, tw, tid, nt); break;
             default:
                 if constexpr (GEN) {
                     if (R != 5) {
@@ -214,7 +230,7 @@ __device__ void fft_passes(double2 *z, int N, int nrad, unsigned long long radix
                         break;
                     }
                 }
-                stockham_pass<5, SIGN>(z, N, Ns, tw, tid, nt);
+                stockham_pass<5, SIGN, false>(z, N, Ns, tw, tid, nt);
                 break;
         }
         Ns *= R;
